@@ -1,0 +1,779 @@
+// imexsw.hip -- the C-ABI of include/imexsw.h on top of the gfx950 kernels.
+//
+// A context owns one grid's device state (structure-of-arrays planes), one
+// HIP stream (default priority) and, when enabled, one pair of HIP events per
+// launch for per-kernel timing.  Host arrays cross the ABI in the
+// reference's Fortran layout and are converted on the device.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "imexsw_kernels.h"
+
+using namespace imexsw;
+
+namespace {
+
+enum KernelId { K_MASK = 0, K_STENCIL, K_LOCAL, K_TOPO, K_DTPOLICY, K_POST, K_NK };
+const char* kKernelNames[K_NK] = {"mask", "stencil", "local", "topo", "dtpolicy", "post"};
+
+thread_local std::string g_create_error;
+
+}  // namespace
+
+struct imexsw_ctx {
+  imexsw_params p{};
+  DevParams P{};
+  DevPtrs D{};
+  hipStream_t stream = nullptr;
+  std::string err;
+  int ns = 0, nv = 0;
+  size_t plane = 0;
+  double* stage = nullptr;  // layout-conversion staging
+  size_t stage_bytes = 0;
+  dim3 grid_tiles, grid_strips, grid_stag;
+  int strip_h = 32;
+  bool d_valid = false;   // D holds the residual of the current q
+  bool dt_hist = false;   // dt_old / dt_old_old initialised
+  bool timing = false;
+  double ms[K_NK] = {0};
+  int64_t calls[K_NK] = {0};
+  std::vector<hipEvent_t> ev_pool;
+  std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> pending;
+  std::vector<void*> allocs;
+};
+
+namespace {
+
+int fail(imexsw_ctx* c, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  if (c) c->err = buf;
+  else g_create_error = buf;
+  return -1;
+}
+
+#define HIPCHK(c, call)                                                            \
+  do {                                                                             \
+    hipError_t e_ = (call);                                                        \
+    if (e_ != hipSuccess) return fail((c), "%s: %s", #call, hipGetErrorString(e_)); \
+  } while (0)
+
+template <typename T>
+int dalloc(imexsw_ctx* c, T** ptr, size_t count) {
+  void* p = nullptr;
+  hipError_t e = hipMalloc(&p, std::max<size_t>(count, 1) * sizeof(T));
+  if (e != hipSuccess) return fail(c, "hipMalloc(%zu bytes): %s", count * sizeof(T), hipGetErrorString(e));
+  e = hipMemsetAsync(p, 0, std::max<size_t>(count, 1) * sizeof(T), c->stream);
+  if (e != hipSuccess) return fail(c, "hipMemsetAsync: %s", hipGetErrorString(e));
+  c->allocs.push_back(p);
+  *ptr = static_cast<T*>(p);
+  return 0;
+}
+
+// event-bracketed launch for per-kernel timing on the context's stream
+struct Timed {
+  imexsw_ctx* c;
+  int id;
+  hipEvent_t a = nullptr, b = nullptr;
+  Timed(imexsw_ctx* ctx, int kid) : c(ctx), id(kid) {
+    if (!c->timing) return;
+    if (c->ev_pool.size() < 2) {
+      hipEvent_t e1, e2;
+      (void)hipEventCreate(&e1);
+      (void)hipEventCreate(&e2);
+      c->ev_pool.push_back(e1);
+      c->ev_pool.push_back(e2);
+    }
+    a = c->ev_pool.back();
+    c->ev_pool.pop_back();
+    b = c->ev_pool.back();
+    c->ev_pool.pop_back();
+    (void)hipEventRecord(a, c->stream);
+  }
+  ~Timed() {
+    if (!c->timing) return;
+    (void)hipEventRecord(b, c->stream);
+    c->pending.push_back({id, {a, b}});
+  }
+};
+
+void collect_timing(imexsw_ctx* c) {
+  for (auto& pe : c->pending) {
+    float ms = 0.f;
+    if (hipEventSynchronize(pe.second.second) == hipSuccess &&
+        hipEventElapsedTime(&ms, pe.second.first, pe.second.second) == hipSuccess) {
+      c->ms[pe.first] += ms;
+      c->calls[pe.first] += 1;
+    }
+    c->ev_pool.push_back(pe.second.first);
+    c->ev_pool.push_back(pe.second.second);
+  }
+  c->pending.clear();
+}
+
+int check_launch(imexsw_ctx* c, const char* what) {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(c, "launch %s: %s", what, hipGetErrorString(e));
+  if (c->pending.size() > 4096) collect_timing(c);
+  return 0;
+}
+
+// ---- NS dispatch -----------------------------------------------------------
+template <int NS>
+void launch_stencil(imexsw_ctx* c, int compute_dt) {
+  if (c->P.ny > 1)
+    hipLaunchKernelGGL((k_stencil<NS, true>), c->grid_strips, dim3(64), 0, c->stream, c->P,
+                       c->D, c->strip_h, compute_dt);
+  else
+    hipLaunchKernelGGL((k_stencil<NS, false>), c->grid_strips, dim3(64), 0, c->stream, c->P,
+                       c->D, c->strip_h, compute_dt);
+}
+template <int NS>
+void launch_local(imexsw_ctx* c, int ops) {
+  hipLaunchKernelGGL((k_local<NS>), c->grid_tiles, dim3(256), 0, c->stream, c->P, c->D, ops);
+}
+template <int NS>
+void launch_qp(imexsw_ctx* c, double* dst) {
+  hipLaunchKernelGGL((k_qp_out<NS>), c->grid_tiles, dim3(256), 0, c->stream, c->P, c->D.q, dst);
+}
+
+#define NS_SWITCH(ns, CALL)        \
+  switch (ns) {                    \
+    case 0: CALL(0); break;        \
+    case 1: CALL(1); break;        \
+    case 2: CALL(2); break;        \
+    case 3: CALL(3); break;        \
+    case 4: CALL(4); break;        \
+    case 5: CALL(5); break;        \
+    default: CALL(6); break;       \
+  }
+
+int run_stencil(imexsw_ctx* c, int compute_dt) {
+  {
+    Timed tm(c, K_STENCIL);
+#define CALL(N) launch_stencil<N>(c, compute_dt)
+    NS_SWITCH(c->ns, CALL)
+#undef CALL
+  }
+  c->d_valid = true;
+  return check_launch(c, "k_stencil");
+}
+
+int run_local(imexsw_ctx* c, int ops, int kid) {
+  {
+    Timed tm(c, kid);
+#define CALL(N) launch_local<N>(c, ops)
+    NS_SWITCH(c->ns, CALL)
+#undef CALL
+  }
+  return check_launch(c, "k_local");
+}
+
+int run_mask(imexsw_ctx* c, int solve_all) {
+  {
+    Timed tm(c, K_MASK);
+    hipLaunchKernelGGL(k_mask, c->grid_tiles, dim3(256), 0, c->stream, c->P, c->D, solve_all);
+  }
+  return check_launch(c, "k_mask");
+}
+
+int run_topo(imexsw_ctx* c, int sparse) {
+  {
+    Timed tm(c, K_TOPO);
+    hipLaunchKernelGGL(k_topo, c->grid_tiles, dim3(256), 0, c->stream, c->P, c->D, sparse);
+    if (c->D.gcoef)
+      hipLaunchKernelGGL(k_gstag, c->grid_stag, dim3(256), 0, c->stream, c->P, c->D);
+  }
+  return check_launch(c, "k_topo");
+}
+
+int set_scalar(imexsw_ctx* c, int slot, double v) {
+  HIPCHK(c, hipMemcpyAsync(c->D.scal + slot, &v, sizeof(double), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int get_scalars(imexsw_ctx* c, double* out) {
+  HIPCHK(c, hipMemcpyAsync(out, c->D.scal, S_NSLOTS * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int ensure_stage(imexsw_ctx* c, size_t bytes) {
+  if (bytes <= c->stage_bytes) return 0;
+  if (c->stage) (void)hipFree(c->stage);
+  c->stage = nullptr;
+  c->stage_bytes = 0;
+  HIPCHK(c, hipMalloc(&c->stage, bytes));
+  c->stage_bytes = bytes;
+  return 0;
+}
+
+// host (nvar, nx, ny) Fortran array -> nvar device planes
+int upload_aos(imexsw_ctx* c, const double* host, int nvar, double* dst) {
+  const size_t n = (size_t)nvar * c->P.nx * c->P.ny;
+  if (ensure_stage(c, n * sizeof(double))) return -1;
+  HIPCHK(c, hipMemcpyAsync(c->stage, host, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  hipLaunchKernelGGL(k_aos_to_planes, c->grid_tiles, dim3(256), 0, c->stream, c->P, c->stage, nvar, dst);
+  if (check_launch(c, "k_aos_to_planes")) return -1;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int download_aos(imexsw_ctx* c, const double* src, int nvar, double* host) {
+  const size_t n = (size_t)nvar * c->P.nx * c->P.ny;
+  if (ensure_stage(c, n * sizeof(double))) return -1;
+  hipLaunchKernelGGL(k_planes_to_aos, c->grid_tiles, dim3(256), 0, c->stream, c->P, src, nvar, c->stage);
+  if (check_launch(c, "k_planes_to_aos")) return -1;
+  HIPCHK(c, hipMemcpyAsync(host, c->stage, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+// host (nx, ny) Fortran plane <-> device plane with pitch
+template <typename T>
+int upload_plane(imexsw_ctx* c, const T* host, T* dst, int w, int h, size_t dpitch_elems) {
+  HIPCHK(c, hipMemcpy2DAsync(dst, dpitch_elems * sizeof(T), host, (size_t)w * sizeof(T),
+                             (size_t)w * sizeof(T), h, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return 0;
+}
+template <typename T>
+int download_plane(imexsw_ctx* c, const T* src, T* host, int w, int h, size_t spitch_elems) {
+  HIPCHK(c, hipMemcpy2DAsync(host, (size_t)w * sizeof(T), src, spitch_elems * sizeof(T),
+                             (size_t)w * sizeof(T), h, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* imexsw_version(void) { return "imexsw 0.1 gfx950"; }
+
+const char* imexsw_last_error(const imexsw_ctx* ctx) {
+  return ctx ? ctx->err.c_str() : g_create_error.c_str();
+}
+
+int imexsw_create(const imexsw_params* params, imexsw_ctx** out) {
+  if (!params || !out) return fail(nullptr, "null argument");
+  *out = nullptr;
+  const imexsw_params& p = *params;
+  if (p.nx < 2 || p.ny < 1) return fail(nullptr, "grid must have nx >= 2 and ny >= 1");
+  if (p.n_solid < 0 || p.n_solid > IMEXSW_MAX_SOLID) return fail(nullptr, "n_solid out of range");
+  if (p.n_rk != 2)
+    return fail(nullptr, "the device path implements N_RK = 2 (IMEX-Euler, the tableau of every shipped deck)");
+  if (p.rheology_flag && p.rheology_model != 1)
+    return fail(nullptr, "rheology model %d is not supported (only 1, Voellmy-Salm)", p.rheology_model);
+  if (p.n_thickness_levels < 0 || p.n_dyn_pres_levels < 0 ||
+      p.n_thickness_levels * p.n_dyn_pres_levels > 32 ||
+      p.n_thickness_levels > IMEXSW_MAX_LEVELS || p.n_dyn_pres_levels > IMEXSW_MAX_LEVELS)
+    return fail(nullptr, "hazard table larger than 32 entries");
+  for (int i = 0; i < 4 + p.n_solid + 2; i++)
+    if (p.limiter[i] < 0 || p.limiter[i] > 7) return fail(nullptr, "limiter out of range");
+
+  imexsw_ctx* c = new imexsw_ctx();
+  c->p = p;
+  c->ns = p.n_solid;
+  c->nv = 4 + p.n_solid;
+  DevParams& P = c->P;
+  P.nx = p.nx;
+  P.ny = p.ny;
+  P.pitch = (p.nx + 31) / 32 * 32;
+  P.ns = p.n_solid;
+  P.nv = c->nv;
+  P.nrk = p.n_rk;
+  P.energy = p.energy_flag;
+  P.alpha = p.alpha_flag;
+  P.slope = p.slope_correction_flag;
+  P.curv = p.curvature_term_flag;
+  P.topo = p.topo_change_flag;
+  P.radial = p.radial_source_flag;
+  P.rheo = p.rheology_flag;
+  P.rheo_model = p.rheology_model;
+  P.settling = p.settling_flag;
+  P.entrain = p.entrainment_flag;
+  P.erod_dep = p.erodible_deposit_flag;
+  P.n_thk = p.n_thickness_levels;
+  P.n_pdyn = p.n_thickness_levels > 0 ? p.n_dyn_pres_levels : 0;
+  P.x0 = p.x0;
+  P.y0 = p.y0;
+  // init_grid (src/geometry_2d.f90:251-300)
+  P.dx = (p.nx > 1) ? p.cell_size : 1.0;
+  P.dy = (p.ny > 1) ? p.cell_size : 1.0;
+  P.dx2 = P.dx / 2.0;
+  P.dy2 = P.dy / 2.0;
+  P.one_by_dx = 1.0 / P.dx;
+  P.one_by_dy = 1.0 / P.dy;
+  P.eps_sing = std::fmin(std::fmin(std::pow(P.dx, 4.0), std::pow(P.dy, 4.0)), 1.0e-10);
+  {
+    double e2 = P.eps_sing * P.eps_sing;
+    P.eps_sing4 = e2 * e2;
+  }
+  P.cfl = p.cfl;
+  P.max_dt = p.max_dt;
+  P.reconstr_coeff = p.reconstr_coeff;
+  P.theta = p.theta;
+  for (int i = 0; i < IMEXSW_MAX_VARS + 2; i++) P.limiter[i] = p.limiter[i];
+  for (int s = 0; s < 4; s++)
+    for (int i = 0; i < IMEXSW_MAX_VARS; i++) {
+      P.bc_flag[s][i] = p.bc_flag[s][i];
+      P.bc_value[s][i] = p.bc_value[s][i];
+    }
+  P.grav = p.grav;
+  P.pres = p.pres;
+  P.inv_pres = 1.0 / p.pres;  // src/inpout_2d.f90:1330
+  P.T_ambient = p.T_ambient;
+  P.sp_heat_a = p.sp_heat_a;
+  P.sp_gas_const_a = p.sp_gas_const_a;
+  P.rho_a_amb = p.pres / (p.sp_gas_const_a * p.T_ambient);  // :1353
+  P.kin_visc_c = p.kin_visc_c;
+  for (int i = 0; i < IMEXSW_MAX_SOLID; i++) {
+    P.rho_s[i] = p.rho_s[i];
+    P.inv_rho_s[i] = (i < p.n_solid) ? 1.0 / p.rho_s[i] : 0.0;  // :1882
+    P.diam_s[i] = p.diam_s[i];
+    P.sp_heat_s[i] = p.sp_heat_s[i];
+    P.erodible_fract[i] = p.erodible_fract[i];
+    P.alphas_source[i] = p.alphas_source[i];
+  }
+  P.mu = p.mu;
+  P.xi = p.xi;
+  P.erosion_coeff = p.erosion_coeff;
+  P.erodible_porosity = p.erodible_porosity;
+  P.T_erodible = p.T_erodible;
+  P.rho_c_sub = p.pres / (p.sp_gas_const_a * p.T_erodible);  // :1733
+  P.coeff_porosity = (p.erosion_coeff > 0.0) ? p.erodible_porosity / (1.0 - p.erodible_porosity) : 0.0;
+  P.alphastot_min = p.alphastot_min;
+  P.max_packing = p.maximum_solid_packing;
+  P.x_source = p.x_source;
+  P.y_source = p.y_source;
+  P.r_source = p.r_source;
+  P.h_source = p.h_source;
+  P.vel_source = p.vel_source;
+  P.T_source = p.T_source;
+  for (int i = 0; i < 4; i++) P.time_param[i] = p.time_param[i];
+  P.pi_g = 4.0 * std::atan(1.0);
+  for (int i = 0; i < IMEXSW_MAX_LEVELS; i++) {
+    P.thickness_levels[i] = p.thickness_levels[i];
+    P.dyn_pres_levels[i] = p.dyn_pres_levels[i];
+  }
+  P.h_cs = (4 + p.n_solid) * kEps;  // src/solver_2d.f90:273-274
+  P.one_by_h = 1.0 / P.h_cs;
+
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    fail(nullptr, "hipStreamCreate failed");
+    delete c;
+    return -1;
+  }
+  c->plane = (size_t)P.pitch * P.ny;
+  const size_t pl = c->plane;
+  DevPtrs& D = c->D;
+  int rc = 0;
+  rc |= dalloc(c, &D.q, c->nv * pl);
+  rc |= dalloc(c, &D.D, c->nv * pl);
+  rc |= dalloc(c, &D.mask, pl);
+  rc |= dalloc(c, &D.B, pl);
+  rc |= dalloc(c, &D.Bx, pl);
+  rc |= dalloc(c, &D.By, pl);
+  if (P.curv || P.slope) {
+    rc |= dalloc(c, &D.Bxx, pl);
+    rc |= dalloc(c, &D.Bxy, pl);
+    rc |= dalloc(c, &D.Byy, pl);
+  }
+  if (P.slope) {
+    rc |= dalloc(c, &D.gcoef, pl);
+    rc |= dalloc(c, &D.dgdx, pl);
+    rc |= dalloc(c, &D.gsx, (size_t)(P.pitch + 1) * P.ny);
+    rc |= dalloc(c, &D.gsy, (size_t)P.pitch * (P.ny + 1));
+  }
+  if (c->ns > 0) {
+    rc |= dalloc(c, &D.erodible, c->ns * pl);
+    rc |= dalloc(c, &D.deposit, c->ns * pl);
+    rc |= dalloc(c, &D.erosion, c->ns * pl);
+  }
+  rc |= dalloc(c, &D.hmax, pl);
+  rc |= dalloc(c, &D.pdynmax, pl);
+  rc |= dalloc(c, &D.modvel, pl);
+  if (P.n_thk * P.n_pdyn > 0) rc |= dalloc(c, &D.vuln, pl);
+  rc |= dalloc(c, &D.dt_bits, 1);
+  rc |= dalloc(c, &D.scal, S_NSLOTS);
+  rc |= dalloc(c, &D.counters, C_NCOUNTERS);
+  c->grid_tiles = dim3((P.nx + kTileX - 1) / kTileX, (P.ny + kTileY - 1) / kTileY);
+  c->grid_stag = dim3((P.nx + 1 + kTileX - 1) / kTileX, (P.ny + 1 + kTileY - 1) / kTileY);
+  rc |= dalloc(c, &D.tile_sums, 4 * (size_t)c->grid_tiles.x * c->grid_tiles.y);
+  c->strip_h = (P.ny > 1) ? 32 : 1;
+  c->grid_strips = dim3((P.nx + kStripW - 1) / kStripW, (P.ny + c->strip_h - 1) / c->strip_h);
+  if (rc) {
+    g_create_error = c->err;
+    imexsw_destroy(c);
+    return -1;
+  }
+  if (hipStreamSynchronize(c->stream) != hipSuccess) {
+    fail(nullptr, "stream synchronise failed after allocation");
+    imexsw_destroy(c);
+    return -1;
+  }
+  *out = c;
+  return 0;
+}
+
+void imexsw_destroy(imexsw_ctx* c) {
+  if (!c) return;
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  collect_timing(c);
+  for (void* p : c->allocs) (void)hipFree(p);
+  if (c->stage) (void)hipFree(c->stage);
+  for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+int imexsw_set_topography(imexsw_ctx* c, const double* B_cent, const uint8_t* nodata) {
+  if (!c || !B_cent) return fail(c, "null argument");
+  const DevParams& P = c->P;
+  if (upload_plane(c, B_cent, c->D.B, P.nx, P.ny, P.pitch)) return -1;
+  if (nodata) {
+    uint8_t* nd = nullptr;
+    if (!c->D.nodata) {
+      if (dalloc(c, &nd, c->plane)) return -1;
+      c->D.nodata = nd;
+    }
+    if (upload_plane(c, nodata, const_cast<uint8_t*>(c->D.nodata), P.nx, P.ny, P.pitch)) return -1;
+  }
+  if (run_topo(c, 0)) return -1;
+  c->d_valid = false;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int imexsw_set_geometry(imexsw_ctx* c, const double* B_cent, const double* Bx,
+                        const double* By, const double* Bxx, const double* Bxy,
+                        const double* Byy, const double* gcoef, const double* gsx,
+                        const double* gsy, const double* dgdx, const uint8_t* nodata) {
+  if (!c || !B_cent) return fail(c, "null argument");
+  const DevParams& P = c->P;
+  DevPtrs& D = c->D;
+  if (upload_plane(c, B_cent, D.B, P.nx, P.ny, P.pitch)) return -1;
+  if (Bx && upload_plane(c, Bx, D.Bx, P.nx, P.ny, P.pitch)) return -1;
+  if (By && upload_plane(c, By, D.By, P.nx, P.ny, P.pitch)) return -1;
+  if (D.Bxx) {
+    if (Bxx && upload_plane(c, Bxx, D.Bxx, P.nx, P.ny, P.pitch)) return -1;
+    if (Bxy && upload_plane(c, Bxy, D.Bxy, P.nx, P.ny, P.pitch)) return -1;
+    if (Byy && upload_plane(c, Byy, D.Byy, P.nx, P.ny, P.pitch)) return -1;
+  }
+  if (D.gcoef) {
+    if (gcoef && upload_plane(c, gcoef, D.gcoef, P.nx, P.ny, P.pitch)) return -1;
+    if (dgdx && upload_plane(c, dgdx, D.dgdx, P.nx, P.ny, P.pitch)) return -1;
+    if (gsx && upload_plane(c, gsx, D.gsx, P.nx + 1, P.ny, P.pitch + 1)) return -1;
+    if (gsy && upload_plane(c, gsy, D.gsy, P.nx, P.ny + 1, P.pitch)) return -1;
+  }
+  if (nodata) {
+    uint8_t* nd = nullptr;
+    if (!D.nodata) {
+      if (dalloc(c, &nd, c->plane)) return -1;
+      D.nodata = nd;
+    }
+    if (upload_plane(c, nodata, const_cast<uint8_t*>(D.nodata), P.nx, P.ny, P.pitch)) return -1;
+  }
+  c->d_valid = false;
+  return 0;
+}
+
+int imexsw_set_source(imexsw_ctx* c, const int32_t* source_cell, const uint8_t* dir_bits,
+                      const double* vect) {
+  if (!c || !source_cell || !dir_bits || !vect) return fail(c, "null argument");
+  const DevParams& P = c->P;
+  DevPtrs& D = c->D;
+  if (!D.scell) {
+    int8_t* sc = nullptr;
+    uint8_t* sb = nullptr;
+    double* sv = nullptr;
+    if (dalloc(c, &sc, c->plane) || dalloc(c, &sb, c->plane) || dalloc(c, &sv, 8 * c->plane)) return -1;
+    D.scell = sc;
+    D.sbits = sb;
+    D.svect = sv;
+  }
+  std::vector<int8_t> sc8((size_t)P.nx * P.ny);
+  for (size_t i = 0; i < sc8.size(); i++) {
+    if (source_cell[i] < 0 || source_cell[i] > 2) return fail(c, "source_cell values must be 0, 1 or 2");
+    sc8[i] = (int8_t)source_cell[i];
+  }
+  if (upload_plane(c, sc8.data(), const_cast<int8_t*>(D.scell), P.nx, P.ny, P.pitch)) return -1;
+  if (upload_plane(c, dir_bits, const_cast<uint8_t*>(D.sbits), P.nx, P.ny, P.pitch)) return -1;
+  if (upload_aos(c, vect, 8, const_cast<double*>(D.svect))) return -1;
+  c->d_valid = false;
+  return 0;
+}
+
+int imexsw_upload_q(imexsw_ctx* c, const double* q) {
+  if (!c || !q) return fail(c, "null argument");
+  if (upload_aos(c, q, c->nv, c->D.q)) return -1;
+  c->d_valid = false;
+  return 0;
+}
+
+int imexsw_upload_erosion(imexsw_ctx* c, const double* erodible, const double* deposit,
+                          const double* erosion) {
+  if (!c) return fail(c, "null argument");
+  if (c->ns == 0) return 0;
+  const DevParams& P = c->P;
+  if (erodible && upload_aos(c, erodible, c->ns, c->D.erodible)) return -1;
+  for (int i = 0; i < c->ns; i++) {
+    const size_t off = (size_t)i * P.nx * P.ny;
+    if (deposit && upload_plane(c, deposit + off, c->D.deposit + i * c->plane, P.nx, P.ny, P.pitch)) return -1;
+    if (erosion && upload_plane(c, erosion + off, c->D.erosion + i * c->plane, P.nx, P.ny, P.pitch)) return -1;
+  }
+  return 0;
+}
+
+int imexsw_check_solve(imexsw_ctx* c, int solve_all, double t, int32_t* solve_cells) {
+  if (!c) return fail(c, "null argument");
+  if (set_scalar(c, S_T, t)) return -1;
+  if (set_scalar(c, S_DONE, 0.0)) return -1;
+  HIPCHK(c, hipMemsetAsync(c->D.counters + C_SOLVE_CELLS, 0, sizeof(unsigned long long), c->stream));
+  if (run_mask(c, solve_all)) return -1;
+  c->d_valid = false;
+  if (solve_cells) {
+    unsigned long long n = 0;
+    HIPCHK(c, hipMemcpyAsync(&n, c->D.counters + C_SOLVE_CELLS, sizeof n, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    *solve_cells = (int32_t)n;
+  }
+  return 0;
+}
+
+int imexsw_timestep(imexsw_ctx* c, double* dt) {
+  if (!c || !dt) return fail(c, "null argument");
+  const unsigned long long init = (unsigned long long)(*(const long long*)&c->P.max_dt);
+  HIPCHK(c, hipMemcpyAsync(c->D.dt_bits, &init, sizeof init, hipMemcpyHostToDevice, c->stream));
+  if (run_stencil(c, c->P.cfl != -1.0)) return -1;
+  unsigned long long bits = 0;
+  HIPCHK(c, hipMemcpyAsync(&bits, c->D.dt_bits, sizeof bits, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  double v;
+  std::memcpy(&v, &bits, sizeof v);
+  *dt = (c->P.cfl != -1.0) ? v : c->P.max_dt;
+  return 0;
+}
+
+int imexsw_rk_step(imexsw_ctx* c, double t, double dt) {
+  if (!c) return fail(c, "null argument");
+  if (!(dt > 0.0)) return fail(c, "dt must be positive");
+  if (set_scalar(c, S_T, t) || set_scalar(c, S_DT, dt) || set_scalar(c, S_DONE, 0.0)) return -1;
+  if (!c->d_valid && run_stencil(c, 0)) return -1;
+  if (run_local(c, OP_RK, K_LOCAL)) return -1;
+  c->d_valid = false;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int imexsw_erosion_deposition(imexsw_ctx* c, double dt) {
+  if (!c) return fail(c, "null argument");
+  if (set_scalar(c, S_DT, dt)) return -1;
+  if (run_local(c, OP_EROSION, K_LOCAL)) return -1;
+  c->d_valid = false;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int imexsw_topography_reconstruction(imexsw_ctx* c) {
+  if (!c) return fail(c, "null argument");
+  if (run_topo(c, 0)) return -1;
+  c->d_valid = false;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int imexsw_post_step(imexsw_ctx* c, double sums[4]) {
+  if (!c) return fail(c, "null argument");
+  if (run_local(c, OP_POST | (sums ? OP_SUMS : 0), K_POST)) return -1;
+  if (sums) {
+    const size_t nt = (size_t)c->grid_tiles.x * c->grid_tiles.y;
+    std::vector<double> ts(4 * nt);
+    HIPCHK(c, hipMemcpyAsync(ts.data(), c->D.tile_sums, ts.size() * sizeof(double),
+                             hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (int m = 0; m < 4; m++) sums[m] = 0.0;
+    for (size_t t = 0; t < nt; t++)
+      for (int m = 0; m < 4; m++) sums[m] += ts[4 * t + m];
+  } else {
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+  }
+  return 0;
+}
+
+int imexsw_advance(imexsw_ctx* c, int32_t nsteps, double dt0, double* t_io, double t_end,
+                   double t_output, int32_t* steps_done) {
+  if (!c || !t_io) return fail(c, "null argument");
+  double s[S_NSLOTS];
+  if (get_scalars(c, s)) return -1;
+  s[S_T] = *t_io;
+  s[S_T_END] = t_end;
+  s[S_T_OUT] = t_output;
+  s[S_DONE] = 0.0;
+  if (!c->dt_hist) {
+    // src/IMEX_SfloW2D.f90:250-251
+    s[S_DT_OLD] = dt0;
+    s[S_DT_OLD_OLD] = dt0;
+    c->dt_hist = true;
+  }
+  HIPCHK(c, hipMemcpyAsync(c->D.scal, s, sizeof s, hipMemcpyHostToDevice, c->stream));
+  unsigned long long steps0 = 0;
+  HIPCHK(c, hipMemcpyAsync(&steps0, c->D.counters + C_STEPS, sizeof steps0, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  const int ops = OP_RK | OP_EROSION | OP_POST;
+  for (int n = 0; n < nsteps; n++) {
+    hipLaunchKernelGGL(k_reset_step, dim3(1), dim3(1), 0, c->stream, c->P, c->D);
+    if (run_mask(c, 0)) return -1;
+    if (run_stencil(c, c->P.cfl != -1.0)) return -1;
+    {
+      Timed tm(c, K_DTPOLICY);
+      hipLaunchKernelGGL(k_dtpolicy, dim3(1), dim3(1), 0, c->stream, c->P, c->D);
+    }
+    if (run_local(c, ops, K_LOCAL)) return -1;
+    if (c->P.topo && run_topo(c, 1)) return -1;
+    hipLaunchKernelGGL(k_advance_time, dim3(1), dim3(1), 0, c->stream, c->D);
+    if (check_launch(c, "k_advance_time")) return -1;
+  }
+  c->d_valid = false;
+  if (get_scalars(c, s)) return -1;
+  *t_io = s[S_T];
+  unsigned long long steps1 = 0;
+  HIPCHK(c, hipMemcpyAsync(&steps1, c->D.counters + C_STEPS, sizeof steps1, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (steps_done) *steps_done = (int32_t)(steps1 - steps0);
+  return 0;
+}
+
+int imexsw_download(imexsw_ctx* c, double* q, double* qp, double* hmax, double* pdynmax,
+                    double* mod_vel_max, uint8_t* vuln) {
+  if (!c) return fail(c, "null argument");
+  const DevParams& P = c->P;
+  if (q && download_aos(c, c->D.q, c->nv, q)) return -1;
+  if (qp) {
+    const size_t n = (size_t)(c->nv + 2) * P.nx * P.ny;
+    if (ensure_stage(c, n * sizeof(double))) return -1;
+#define CALL(N) launch_qp<N>(c, c->stage)
+    NS_SWITCH(c->ns, CALL)
+#undef CALL
+    if (check_launch(c, "k_qp_out")) return -1;
+    HIPCHK(c, hipMemcpyAsync(qp, c->stage, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+  }
+  if (hmax && download_plane(c, c->D.hmax, hmax, P.nx, P.ny, P.pitch)) return -1;
+  if (pdynmax && download_plane(c, c->D.pdynmax, pdynmax, P.nx, P.ny, P.pitch)) return -1;
+  if (mod_vel_max && download_plane(c, c->D.modvel, mod_vel_max, P.nx, P.ny, P.pitch)) return -1;
+  const int nl = P.n_thk * P.n_pdyn;
+  if (vuln && nl > 0) {
+    std::vector<uint32_t> bits((size_t)P.nx * P.ny);
+    if (download_plane(c, c->D.vuln, bits.data(), P.nx, P.ny, P.pitch)) return -1;
+    for (size_t cc = 0; cc < bits.size(); cc++)
+      for (int i = 0; i < nl; i++) vuln[cc * nl + i] = (bits[cc] >> i) & 1u;
+  }
+  return 0;
+}
+
+int imexsw_download_erosion(imexsw_ctx* c, double* erodible, double* deposit, double* erosion,
+                            double* B_cent) {
+  if (!c) return fail(c, "null argument");
+  const DevParams& P = c->P;
+  if (c->ns > 0) {
+    if (erodible && download_aos(c, c->D.erodible, c->ns, erodible)) return -1;
+    for (int i = 0; i < c->ns; i++) {
+      const size_t off = (size_t)i * P.nx * P.ny;
+      if (deposit && download_plane(c, c->D.deposit + i * c->plane, deposit + off, P.nx, P.ny, P.pitch)) return -1;
+      if (erosion && download_plane(c, c->D.erosion + i * c->plane, erosion + off, P.nx, P.ny, P.pitch)) return -1;
+    }
+  }
+  if (B_cent && download_plane(c, c->D.B, B_cent, P.nx, P.ny, P.pitch)) return -1;
+  return 0;
+}
+
+int imexsw_download_mask(imexsw_ctx* c, uint8_t* mask) {
+  if (!c || !mask) return fail(c, "null argument");
+  return download_plane(c, c->D.mask, mask, c->P.nx, c->P.ny, c->P.pitch);
+}
+
+int imexsw_get_dt(imexsw_ctx* c, double* dt) {
+  if (!c || !dt) return fail(c, "null argument");
+  double s[S_NSLOTS];
+  if (get_scalars(c, s)) return -1;
+  *dt = s[S_DT];
+  return 0;
+}
+
+int imexsw_stats_get(imexsw_ctx* c, imexsw_stats* st, int reset) {
+  if (!c || !st) return fail(c, "null argument");
+  unsigned long long v[C_NCOUNTERS];
+  HIPCHK(c, hipMemcpyAsync(v, c->D.counters, sizeof v, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  std::memset(st, 0, sizeof *st);
+  st->steps = (int64_t)v[C_STEPS];
+  st->cell_updates_active = (int64_t)v[C_ACTIVE_UPD];
+  st->cell_updates_grid = (int64_t)v[C_STEPS] * c->P.nx * c->P.ny;
+  st->newton_cells = (int64_t)v[C_NEWTON_CELLS];
+  st->newton_iters = (int64_t)v[C_NEWTON_ITERS];
+  st->newton_nonconverged = (int64_t)v[C_NONCONV];
+  st->nan_events = (int64_t)v[C_NAN];
+  st->neg_h_events = (int64_t)v[C_NEGH];
+  st->neg_solid_events = (int64_t)v[C_NEGS];
+  st->sum_solid_events = (int64_t)v[C_SUMS];
+  st->low_T_events = (int64_t)v[C_LOWT];
+  st->solve_cells = (int32_t)v[C_SOLVE_CELLS];
+  st->active_tiles = 0;
+  if (reset) {
+    unsigned long long keep = v[C_SOLVE_CELLS];
+    std::memset(v, 0, sizeof v);
+    v[C_SOLVE_CELLS] = keep;
+    HIPCHK(c, hipMemcpyAsync(c->D.counters, v, sizeof v, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+  }
+  return 0;
+}
+
+int imexsw_set_timing(imexsw_ctx* c, int enable) {
+  if (!c) return fail(c, "null argument");
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  collect_timing(c);
+  c->timing = enable != 0;
+  for (int i = 0; i < K_NK; i++) {
+    c->ms[i] = 0.0;
+    c->calls[i] = 0;
+  }
+  return 0;
+}
+
+int imexsw_kernel_times(imexsw_ctx* c, double* ms, int64_t* calls, int cap) {
+  if (!c) return fail(c, "null argument");
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  collect_timing(c);
+  const int n = std::min(cap, (int)K_NK);
+  for (int i = 0; i < n; i++) {
+    if (ms) ms[i] = c->ms[i];
+    if (calls) calls[i] = c->calls[i];
+  }
+  return n;
+}
+
+const char* imexsw_kernel_name(int index) {
+  return (index >= 0 && index < K_NK) ? kKernelNames[index] : "";
+}
+
+int imexsw_sync(imexsw_ctx* c) {
+  if (!c) return fail(c, "null argument");
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,1002 @@
+// imexsw_kernels.h -- gfx950 kernels of the per-timestep path.
+//
+// Device layout: every per-cell field is a structure-of-arrays plane of
+// `pitch * ny` doubles, x fastest (j), so a 64-lane wavefront reading 64
+// consecutive cells of a row issues fully coalesced 512-byte loads.  The
+// conservative state is NV such planes, the explicit residual D another NV.
+//
+// Kernels per time step (N_RK = 2):
+//   k_mask     check_solve: active mask (wet cells dilated N_RK times)
+//   k_stencil  one pass: reconstruction, wave speeds, KT fluxes, divergence,
+//              explicit terms -> D = divFlux - expl_terms, and the CFL dt
+//              (min over active cells, as an ordered-bits atomicMin)
+//   k_dtpolicy host-loop dt clipping / growth limiter on the device
+//   k_local    stage-2 Newton + assembly checks, erosion/deposition,
+//              post-step diagnostics, per-tile sums
+//   k_topo     topography_reconstruction of the slope fields (when the
+//              topography changes)
+#pragma once
+
+#include "imexsw_newton.h"
+
+namespace imexsw {
+
+constexpr int kWave = 64;
+constexpr int kStripW = kWave - 4;  // interior columns per stencil wavefront
+constexpr int kTileX = 64, kTileY = 4;
+
+// ---------------------------------------------------------------------------
+// pointers of one context (device addresses)
+// ---------------------------------------------------------------------------
+struct DevPtrs {
+  double* q;         // NV planes
+  double* D;         // NV planes
+  uint8_t* mask;     // 1 plane (bytes)
+  const int8_t* scell;   // source_cell plane (0/1/2) or null
+  const uint8_t* sbits;  // ring direction bits or null
+  const double* svect;   // 8 planes or null
+  double* B;         // B_cent
+  double* Bx;        // B_prime_x_geom
+  double* By;
+  double* Bxx;       // curvature only (else null)
+  double* Bxy;
+  double* Byy;
+  double* gcoef;     // slope correction only (else null)
+  double* dgdx;
+  double* gsx;       // (nx+1) x ny, pitch+1
+  double* gsy;       // nx x (ny+1)
+  const uint8_t* nodata;  // or null
+  double* erodible;  // NS planes
+  double* deposit;   // NS planes
+  double* erosion;   // NS planes
+  double* hmax;
+  double* pdynmax;
+  double* modvel;
+  uint32_t* vuln;    // bitmask plane or null
+  unsigned long long* dt_bits;  // CFL min (ordered bits of a positive double)
+  double* scal;      // device scalars, see ScalarSlot
+  unsigned long long* counters;  // see CounterSlot
+  double* tile_sums;  // 4 per local tile
+};
+
+enum ScalarSlot { S_T = 0, S_DT = 1, S_DT_OLD = 2, S_DT_OLD_OLD = 3, S_T_END = 4,
+                  S_T_OUT = 5, S_DONE = 6, S_DT_CFL = 7, S_NSLOTS = 8 };
+enum CounterSlot { C_SOLVE_CELLS = 0, C_NEWTON_CELLS, C_NEWTON_ITERS, C_NONCONV,
+                   C_NAN, C_NEGH, C_NEGS, C_SUMS, C_LOWT, C_STEPS, C_ACTIVE_UPD,
+                   C_NCOUNTERS };
+
+__device__ __forceinline__ size_t cidx(const DevParams& P, int j, int k) {
+  return (size_t)k * P.pitch + j;
+}
+__device__ __forceinline__ size_t plane(const DevParams& P) {
+  return (size_t)P.pitch * P.ny;
+}
+
+// x_stag / x_comp / y_stag / y_comp of init_grid (src/geometry_2d.f90:303-325)
+__device__ __forceinline__ double xstag(const DevParams& P, int j) { return P.x0 + (double)j * P.dx; }
+__device__ __forceinline__ double ystag(const DevParams& P, int k) { return P.y0 + (double)k * P.dy; }
+__device__ __forceinline__ double xcomp(const DevParams& P, int j) {
+  return 0.5 * (xstag(P, j) + xstag(P, j + 1));
+}
+__device__ __forceinline__ double ycomp(const DevParams& P, int k) {
+  return 0.5 * (ystag(P, k) + ystag(P, k + 1));
+}
+
+// ---------------------------------------------------------------------------
+// wave shuffles of doubles (ds_bpermute on 32-bit halves)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double shfl_d(double v, int src) {
+  int lo = __double2loint(v), hi = __double2hiint(v);
+  lo = __shfl(lo, src, kWave);
+  hi = __shfl(hi, src, kWave);
+  return __hiloint2double(hi, lo);
+}
+
+// ---------------------------------------------------------------------------
+// k_mask: check_solve (src/solver_2d.f90:666-857)
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_mask(DevParams P, DevPtrs D, int solve_all) {
+  const int j = blockIdx.x * kTileX + (threadIdx.x & 63);
+  const int k = blockIdx.y * kTileY + (threadIdx.x >> 6);
+  int active = 0;
+  if (j < P.nx && k < P.ny) {
+    const bool bnd = (j == 0 || j == P.nx - 1 || k == 0 || k == P.ny - 1);
+    if (bnd) {
+      active = 1;
+    } else {
+      const int R = P.nrk;
+      for (int dk = -R; dk <= R && !active; dk++) {
+        const int kk = k + dk;
+        if (kk < 0 || kk >= P.ny) continue;
+        const int rem = R - (dk < 0 ? -dk : dk);
+        for (int dj = -rem; dj <= rem; dj++) {
+          const int jj = j + dj;
+          if (jj < 0 || jj >= P.nx) continue;
+          const bool b2 = (jj == 0 || jj == P.nx - 1 || kk == 0 || kk == P.ny - 1);
+          bool base = b2 || solve_all;
+          if (!base) {
+            base = D.q[cidx(P, jj, kk)] > 0.0;
+            if (!base && P.radial && D.scell) base = D.scell[cidx(P, jj, kk)] == 2;
+          }
+          if (base) {
+            active = 1;
+            break;
+          }
+        }
+      }
+    }
+    if (P.radial && D.scell && D.scell[cidx(P, j, k)] == 1) active = 0;
+    D.mask[cidx(P, j, k)] = (uint8_t)active;
+  }
+  // count of active cells (solve_cells)
+  unsigned long long bal = __ballot(active);
+  if ((threadIdx.x & 63) == 0 && bal)
+    atomicAdd(&D.counters[C_SOLVE_CELLS], (unsigned long long)__popcll(bal));
+}
+
+// ---------------------------------------------------------------------------
+// reconstruction of one cell in one direction (src/solver_2d.f90:3152-3661)
+// ---------------------------------------------------------------------------
+struct SrcInfo {
+  bool src2;       // source_cell == 2 (ring) and RADIAL_SOURCE_FLAG
+  bool is_src;     // source_cell == 1 (source interior)
+  bool sE, sW, sS, sN;
+};
+
+__device__ __forceinline__ void qp2_uv(const double* qp, double& u, double& v) {
+  // qp_to_qp2 (src/constitutive_2d.f90:2155-2179), velocity components
+  if (qp[0] <= 0.0) {
+    u = 0.0;
+    v = 0.0;
+  } else {
+    u = qp[1] / qp[0];
+    v = qp[2] / qp[0];
+  }
+}
+
+// DIR 1: x (lo = west face value qrecW, hi = east qrecE); DIR 2: y (S, N).
+// c = own physical state, m / p = neighbours at index-1 / index+1 along DIR
+// (not read at a domain boundary), idx / n = position and extent along DIR,
+// other = position along the other direction, sb = source boundary state,
+// typo = the x_stencil(3) = y_stag(k+1) substitution of src/solver_2d.f90:
+// 3616 that reaches the x-slope of v.
+template <int NS, int DIR>
+__device__ __forceinline__ void recon(const DevParams& P, const double* c,
+                                      const double* m, const double* p, int idx,
+                                      int n, int other, const SrcInfo& S,
+                                      const double* sb, bool typo, double* lo,
+                                      double* hi) {
+  constexpr int NV = 4 + NS;
+  const int side_lo = (DIR == 1) ? IMEXSW_W : IMEXSW_S;
+  const int side_hi = (DIR == 1) ? IMEXSW_E : IMEXSW_N;
+  const double one_by = (DIR == 1) ? P.one_by_dx : P.one_by_dy;
+  const double d2 = (DIR == 1) ? P.dx2 : P.dy2;
+  const bool first = (idx == 0), last = (idx == n - 1);
+  const bool radial2 = P.radial && S.src2;
+  // DIR 1 honours the ring at the domain edge (:3185, :3226); DIR 2 does not
+  const bool srcb = (DIR == 1) && S.src2;
+  const bool ovr_lo = (DIR == 1) ? (radial2 && !S.sE && S.sW) : (radial2 && S.sS);
+  const bool ovr_hi = (DIR == 1) ? (radial2 && S.sE) : (radial2 && !S.sS && S.sN);
+  auto stag = [&](int i) { return (DIR == 1) ? xstag(P, i) : ystag(P, i); };
+  auto comp = [&](int i) { return (DIR == 1) ? xcomp(P, i) : ycomp(P, i); };
+  // stencil abscissae left by the loop over the conservative variables
+  double z0, z2;
+  const double z1 = comp(idx);
+  if (first) {
+    z0 = stag(0);
+    z2 = comp(idx + 1);
+  } else if (last) {
+    z0 = comp(idx - 1);
+    z2 = stag(n);
+  } else {
+    z0 = ovr_lo ? stag(idx) : comp(idx - 1);
+    z2 = ovr_hi ? stag(idx + 1) : comp(idx + 1);
+  }
+  (void)other;
+#pragma unroll
+  for (int i = 0; i < NV; i++) {
+    const double s1 = c[i];
+    const int lim = P.limiter[i];
+    double slope = 0.0;
+    if (first) {
+      const int f = P.bc_flag[side_lo][i];
+      const double bv = P.bc_value[side_lo][i];
+      if (srcb) slope = limit_(P, sb[i], s1, p[i], z0, z1, z2, lim);
+      else if (f == 0) slope = limit_(P, bv, s1, p[i], z0, z1, z2, lim);
+      else if (f == 1) slope = bv;
+      else if (f == 2) slope = (p[i] - s1) * one_by;
+    } else if (last) {
+      const int f = P.bc_flag[side_hi][i];
+      const double bv = P.bc_value[side_hi][i];
+      if (srcb) slope = limit_(P, m[i], s1, sb[i], z0, z1, z2, lim);
+      else if (f == 0) slope = limit_(P, m[i], s1, bv, z0, z1, z2, lim);
+      else if (f == 1) slope = bv;
+      else if (f == 2) slope = (s1 - m[i]) * one_by;
+    } else {
+      const double v0 = ovr_lo ? sb[i] : m[i];
+      const double v2 = ovr_hi ? sb[i] : p[i];
+      slope = limit_(P, v0, s1, v2, z0, z1, z2, lim);
+    }
+    const double dq = P.reconstr_coeff * d2 * slope;
+    lo[i] = s1 - dq;
+    hi[i] = s1 + dq;
+    if (first) {
+      const int f = P.bc_flag[side_lo][i];
+      if (srcb) lo[i] = sb[i];
+      else if (f == 0) lo[i] = P.bc_value[side_lo][i];
+      else if (i == DIR) lo[i] = fmin(lo[i], 0.0);
+    }
+    if (last) {
+      const int f = P.bc_flag[side_hi][i];
+      if (srcb) hi[i] = sb[i];
+      else if (f == 0) hi[i] = P.bc_value[side_hi][i];
+      else if (i == DIR) hi[i] = fmax(hi[i], 0.0);
+    }
+  }
+  // u and v with the limiters of hu and hv (:3496-3661)
+#pragma unroll
+  for (int i = NV; i < NV + 2; i++) {
+    const double s1 = c[i];
+    double v0, v2, zz2 = z2;
+    if (DIR == 1 && i == NV + 1 && typo) zz2 = (!first && !last && radial2 && S.sE) ? stag(idx + 1) : ystag(P, other + 1);
+    if (first) {
+      double u2, w2;
+      qp2_uv(lo, u2, w2);
+      v0 = (i == NV) ? u2 : w2;
+      v2 = p[i];
+    } else if (last) {
+      double u2, w2;
+      qp2_uv(hi, u2, w2);
+      v0 = m[i];
+      v2 = (i == NV) ? u2 : w2;
+    } else {
+      v0 = ovr_lo ? sb[i] : m[i];
+      v2 = ovr_hi ? sb[i] : p[i];
+    }
+    const double slope = limit_(P, v0, s1, v2, z0, z1, zz2, P.limiter[i]);
+    const double dq = P.reconstr_coeff * d2 * slope;
+    lo[i] = s1 - dq;
+    hi[i] = s1 + dq;
+    if (first) {
+      double u2, w2;
+      qp2_uv(lo, u2, w2);
+      lo[i] = (i == NV) ? u2 : w2;
+    } else if (last) {
+      double u2, w2;
+      qp2_uv(hi, u2, w2);
+      hi[i] = (i == NV) ? u2 : w2;
+    }
+  }
+  if (!first && !last) {
+    if (ovr_hi) {
+#pragma unroll
+      for (int i = 0; i < NV + 2; i++) hi[i] = sb[i];
+    } else if (ovr_lo) {
+#pragma unroll
+      for (int i = 0; i < NV + 2; i++) lo[i] = sb[i];
+    }
+  }
+}
+
+// One face side: what the KT flux needs from the state on that side.
+template <int NS>
+struct Side {
+  double f[4 + NS];
+  double qc[4 + NS];
+  double vmin, vmax, zv;
+};
+
+template <int NS, int DIR>
+__device__ __forceinline__ void make_side(const DevParams& P, const double* qp,
+                                          double gc, Side<NS>& s) {
+  qp2qc<NS>(P, qp, s.qc);
+  fluxes<NS, DIR>(P, s.qc, qp, gc, s.f);
+  speeds<NS, DIR>(P, qp, gc, s.vmin, s.vmax);
+  s.zv = (DIR == 1) ? qp[1] : s.qc[2];
+}
+
+template <int NS>
+__device__ __forceinline__ void zero_side(Side<NS>& s) {
+#pragma unroll
+  for (int i = 0; i < 4 + NS; i++) {
+    s.f[i] = 0.0;
+    s.qc[i] = 0.0;
+  }
+  s.vmin = s.vmax = s.zv = 0.0;
+}
+
+template <int NS>
+__device__ __forceinline__ void shfl_side(const Side<NS>& in, Side<NS>& out, int src) {
+#pragma unroll
+  for (int i = 0; i < 4 + NS; i++) {
+    out.f[i] = shfl_d(in.f[i], src);
+    out.qc[i] = shfl_d(in.qc[i], src);
+  }
+  out.vmin = shfl_d(in.vmin, src);
+  out.vmax = shfl_d(in.vmax, src);
+  out.zv = shfl_d(in.zv, src);
+}
+
+template <int NS, int DIR>
+__device__ __forceinline__ double face_flux(const Side<NS>& L, const Side<NS>& R,
+                                            double* H) {
+  const double aN = fmin(fmin(L.vmin, R.vmin), 0.0);
+  const double aP = fmax(fmax(L.vmax, R.vmax), 0.0);
+  kt_flux<NS, DIR>(L.f, R.f, L.qc, R.qc, aN, aP, L.zv, R.zv, H);
+  return fmax(aP, -aN);
+}
+
+// load the ring information of one cell
+__device__ __forceinline__ SrcInfo load_src(const DevParams& P, const DevPtrs& D,
+                                            size_t c) {
+  SrcInfo S{false, false, false, false, false, false};
+  if (P.radial && D.scell) {
+    const int sc = D.scell[c];
+    S.is_src = (sc == 1);
+    S.src2 = (sc == 2);
+    if (S.src2) {
+      const uint8_t b = D.sbits[c];
+      S.sE = b & IMEXSW_SRC_E;
+      S.sW = b & IMEXSW_SRC_W;
+      S.sS = b & IMEXSW_SRC_S;
+      S.sN = b & IMEXSW_SRC_N;
+    }
+  }
+  return S;
+}
+
+template <int NS>
+__device__ __forceinline__ void ring_state(const DevParams& P, const DevPtrs& D,
+                                           size_t c, const SrcInfo& S, double t,
+                                           double* sb) {
+  constexpr int NV = 4 + NS;
+#pragma unroll
+  for (int i = 0; i < NV + 2; i++) sb[i] = 0.0;
+  if (!(P.radial && S.src2)) return;
+  const size_t pl = plane(P);
+  int o = S.sE ? 0 : S.sW ? 2 : S.sS ? 4 : S.sN ? 6 : -1;
+  if (o < 0) return;
+  source_bdry<NS>(P, t, D.svect[o * pl + c], D.svect[(o + 1) * pl + c], sb);
+}
+
+// ---------------------------------------------------------------------------
+// k_stencil: the stage-1 stencil pass (timestep + eval_hyperbolic_terms +
+// eval_expl_terms at q^n).  One wavefront marches up a strip of rows keeping
+// a three-row window of physical states in registers; lane l owns column
+// j = 60*blockIdx.x - 2 + l.  Lanes 2..61 are the strip's cells, lanes 1 and
+// 62 reconstruct the halo sides of the strip's outer faces, lanes 0 and 63
+// only supply states.  x neighbours are exchanged with wave shuffles; rows
+// k0-2 .. k1+1 are read once each.
+// ---------------------------------------------------------------------------
+template <int NS>
+__device__ __forceinline__ void load_row(const DevParams& P, const DevPtrs& D,
+                                         int j, int k, double* qp) {
+  constexpr int NV = 4 + NS;
+  if (j >= 0 && j < P.nx && k >= 0 && k < P.ny) {
+    const size_t c = cidx(P, j, k), pl = plane(P);
+    double q[NV];
+#pragma unroll
+    for (int i = 0; i < NV; i++) q[i] = D.q[i * pl + c];
+    phys_or_dry<NS>(P, q, qp);
+  } else {
+#pragma unroll
+    for (int i = 0; i < NV + 2; i++) qp[i] = 0.0;
+    qp[3] = P.T_ambient;
+  }
+}
+
+// the two sides of the y face between rows k-1 (B, owner Sb) and k (T,
+// owner St), with the ring overrides of src/solver_2d.f90:3879-3889
+template <int NS>
+__device__ __forceinline__ void yface_sides(const DevParams& P, const double* hib,
+                                            const double* lot, const SrcInfo& Sb,
+                                            const SrcInfo& St, int k, double g,
+                                            Side<NS>& Bs, Side<NS>& Ts) {
+  if (Sb.is_src) {
+    if (P.radial && St.src2 && St.sS && k < P.ny - 1) make_side<NS, 2>(P, lot, g, Bs);
+    else zero_side<NS>(Bs);
+  } else {
+    make_side<NS, 2>(P, hib, g, Bs);
+  }
+  if (St.is_src) {
+    if (P.radial && Sb.src2 && !Sb.sS && Sb.sN && k - 1 > 0) make_side<NS, 2>(P, hib, g, Ts);
+    else zero_side<NS>(Ts);
+  } else {
+    make_side<NS, 2>(P, lot, g, Ts);
+  }
+}
+
+template <int NS, bool HASY>
+__global__ void __launch_bounds__(64) k_stencil(DevParams P, DevPtrs D, int strip_h,
+                                                int compute_dt) {
+  constexpr int NV = 4 + NS;
+  const int lane = threadIdx.x;
+  const int j = blockIdx.x * kStripW - 2 + lane;
+  const int k0 = blockIdx.y * strip_h;
+  const int k1 = min(k0 + strip_h, P.ny);
+  const bool injx = (j >= 0 && j < P.nx);
+  const bool own = (lane >= 2 && lane < 2 + kStripW && injx);
+  if (D.scal[S_DONE] != 0.0) return;
+  const double t = D.scal[S_T];
+
+  // skip strips without an active cell
+  {
+    int any = 0;
+    if (own)
+      for (int k = k0; k < k1; k++) any |= D.mask[cidx(P, j, k)];
+    if (!__any(any)) return;
+  }
+  const size_t pl = plane(P);
+  double dtmin = P.max_dt;
+
+  double qm[NV + 2], qc[NV + 2], qn[NV + 2];  // rows k-1, k, k+1
+  double hiy[NV + 2];                          // N value of row k
+  double HS[NV];                               // y flux at the S face of row k
+  double amaxS = 0.0;
+  SrcInfo S = injx ? load_src(P, D, cidx(P, j, k0)) : SrcInfo{};
+  if (HASY) {
+    double qmm[NV + 2], sb[NV + 2], lo[NV + 2];
+    load_row<NS>(P, D, j, k0 - 2, qmm);
+    load_row<NS>(P, D, j, k0 - 1, qm);
+    load_row<NS>(P, D, j, k0, qc);
+    load_row<NS>(P, D, j, k0 + 1, qn);
+    Side<NS> Bs, Ts;
+    if (injx) {
+      const size_t c = cidx(P, j, k0);
+      ring_state<NS>(P, D, c, S, t, sb);
+      recon<NS, 2>(P, qc, qm, qn, k0, P.ny, j, S, sb, false, lo, hiy);
+      const double g = D.gsy ? D.gsy[c] : 1.0;
+      if (k0 == 0) {
+        // boundary face: both sides carry the S value of row 0
+        if (S.is_src) zero_side<NS>(Ts); else make_side<NS, 2>(P, lo, g, Ts);
+        Bs = Ts;
+      } else {
+        const size_t cm = cidx(P, j, k0 - 1);
+        SrcInfo Sm = load_src(P, D, cm);
+        double sbm[NV + 2], lom[NV + 2], him[NV + 2];
+        ring_state<NS>(P, D, cm, Sm, t, sbm);
+        recon<NS, 2>(P, qm, qmm, qc, k0 - 1, P.ny, j, Sm, sbm, false, lom, him);
+        yface_sides<NS>(P, him, lo, Sm, S, k0, g, Bs, Ts);
+      }
+    } else {
+      zero_side<NS>(Bs);
+      zero_side<NS>(Ts);
+#pragma unroll
+      for (int i = 0; i < NV + 2; i++) hiy[i] = 0.0;
+    }
+    amaxS = face_flux<NS, 2>(Bs, Ts, HS);
+  } else {
+    load_row<NS>(P, D, j, k0, qc);
+  }
+
+  for (int k = k0; k < k1; k++) {
+    const size_t c = injx ? cidx(P, j, k) : 0;
+    double sb[NV + 2];
+    if (injx) ring_state<NS>(P, D, c, S, t, sb);
+    else {
+#pragma unroll
+      for (int i = 0; i < NV + 2; i++) sb[i] = 0.0;
+    }
+    // ---------------- y: N face of row k ----------------
+    double HN[NV];
+    double amaxN = 0.0;
+    SrcInfo Sn{};
+    if (HASY) {
+      double qnn[NV + 2], hin[NV + 2];
+      load_row<NS>(P, D, j, k + 2, qnn);
+      Side<NS> Bs, Ts;
+      if (injx) {
+        if (k == P.ny - 1) {
+          const double g = D.gsy ? D.gsy[cidx(P, j, P.ny)] : 1.0;
+          if (S.is_src) zero_side<NS>(Bs); else make_side<NS, 2>(P, hiy, g, Bs);
+          Ts = Bs;
+#pragma unroll
+          for (int i = 0; i < NV + 2; i++) hin[i] = 0.0;
+        } else {
+          const size_t cn = cidx(P, j, k + 1);
+          Sn = load_src(P, D, cn);
+          double sbn[NV + 2], lon[NV + 2];
+          ring_state<NS>(P, D, cn, Sn, t, sbn);
+          recon<NS, 2>(P, qn, qc, qnn, k + 1, P.ny, j, Sn, sbn, false, lon, hin);
+          const double g = D.gsy ? D.gsy[cn] : 1.0;
+          yface_sides<NS>(P, hiy, lon, S, Sn, k + 1, g, Bs, Ts);
+        }
+      } else {
+        zero_side<NS>(Bs);
+        zero_side<NS>(Ts);
+#pragma unroll
+        for (int i = 0; i < NV + 2; i++) hin[i] = 0.0;
+      }
+      amaxN = face_flux<NS, 2>(Bs, Ts, HN);
+#pragma unroll
+      for (int i = 0; i < NV + 2; i++) {
+        qm[i] = qc[i];
+        qc[i] = qn[i];
+        qn[i] = qnn[i];
+        hiy[i] = hin[i];
+      }
+    }
+    // physical state of row k: qm after the shift (2-D) or qc (1-D)
+    const double* qk = HASY ? qm : qc;
+    // ---------------- x: W face on every lane, E face by shuffle ----------
+    double qkm[NV + 2], qkp[NV + 2];
+#pragma unroll
+    for (int i = 0; i < NV + 2; i++) {
+      qkm[i] = shfl_d(qk[i], (lane + 63) & 63);
+      qkp[i] = shfl_d(qk[i], (lane + 1) & 63);
+    }
+    // after a north ring cell the x-slope of v uses y_stag(k+1) as its
+    // third abscissa (src/solver_2d.f90:3616)
+    const bool typo = HASY && injx && P.radial && S.src2 && k > 0 && k < P.ny - 1 &&
+                      !S.sS && S.sN;
+    Side<NS> Wself, Eself;
+    if (injx) {
+      double lo[NV + 2], hi[NV + 2];
+      recon<NS, 1>(P, qk, qkm, qkp, j, P.nx, k, S, sb, typo, lo, hi);
+      const double gW = D.gsx ? D.gsx[(size_t)k * (P.pitch + 1) + j] : 1.0;
+      const double gE = D.gsx ? D.gsx[(size_t)k * (P.pitch + 1) + j + 1] : 1.0;
+      if (S.is_src) {
+        zero_side<NS>(Wself);
+        zero_side<NS>(Eself);
+      } else {
+        make_side<NS, 1>(P, lo, gW, Wself);
+        make_side<NS, 1>(P, hi, gE, Eself);
+      }
+    } else {
+      zero_side<NS>(Wself);
+      zero_side<NS>(Eself);
+    }
+    Side<NS> Eleft;
+    shfl_side<NS>(Eself, Eleft, (lane + 63) & 63);
+    const int flags = (S.is_src ? 1 : 0) | (S.src2 ? 2 : 0) | (S.sE ? 4 : 0) |
+                      (S.sW ? 8 : 0);
+    const int flags_left = __shfl(flags, (lane + 63) & 63, kWave);
+    double HW[NV];
+    double amaxW;
+    {
+      Side<NS> Ls, Rs;
+      if (j == 0) {
+        Ls = Wself;
+        Rs = Wself;
+      } else {
+        const bool interior_me = (j > 0 && j < P.nx - 1);
+        const bool interior_left = (j - 1 > 0 && j - 1 < P.nx - 1);
+        if (flags_left & 1) {
+          if (P.radial && S.src2 && !S.sE && S.sW && interior_me) Ls = Wself;
+          else zero_side<NS>(Ls);
+        } else {
+          Ls = Eleft;
+        }
+        if (S.is_src) {
+          if (P.radial && (flags_left & 2) && (flags_left & 4) && interior_left) Rs = Eleft;
+          else zero_side<NS>(Rs);
+        } else {
+          Rs = Wself;
+        }
+      }
+      amaxW = face_flux<NS, 1>(Ls, Rs, HW);
+    }
+    double HE[NV];
+    double amaxE;
+    {
+      double HEs[NV];
+#pragma unroll
+      for (int i = 0; i < NV; i++) HEs[i] = shfl_d(HW[i], (lane + 1) & 63);
+      double amaxEs = shfl_d(amaxW, (lane + 1) & 63);
+      if (j == P.nx - 1) {
+        amaxE = face_flux<NS, 1>(Eself, Eself, HE);
+      } else {
+#pragma unroll
+        for (int i = 0; i < NV; i++) HE[i] = HEs[i];
+        amaxE = amaxEs;
+      }
+    }
+    // ---------------- divergence, explicit terms, D ----------------
+    if (own && D.mask[c]) {
+      double ex[NV];
+      const double bx = D.Bx[c], by = D.By[c];
+      const double bxx = D.Bxx ? D.Bxx[c] : 0.0, bxy = D.Bxy ? D.Bxy[c] : 0.0,
+                   byy = D.Byy ? D.Byy[c] : 0.0;
+      const double gc = D.gcoef ? D.gcoef[c] : 1.0;
+      const double dg = D.dgdx ? D.dgdx[c] : 0.0;
+      expl_terms<NS>(P, bx, by, bxx, bxy, byy, gc, dg, qk, ex);
+#pragma unroll
+      for (int i = 0; i < NV; i++) {
+        double dv = 0.0;
+        dv = dv + (HE[i] - HW[i]) * P.one_by_dx;
+        if (HASY) dv = dv + (HN[i] - HS[i]) * P.one_by_dy;
+        D.D[i * pl + c] = (0.0 + (dv - ex[i]) * 1.0) + (0.0 - 0.0) * 0.0;
+      }
+      if (compute_dt) {
+        double ma = fmax(amaxW, amaxE);
+        const double dtx = (ma > 0.0) ? P.cfl * P.dx / ma : P.max_dt;
+        double dty = P.max_dt;
+        if (HASY) {
+          ma = fmax(amaxS, amaxN);
+          dty = (ma > 0.0) ? P.cfl * P.dy / ma : P.max_dt;
+        }
+        dtmin = fmin(dtmin, fmin(dtx, dty));
+      }
+    }
+    if (HASY) {
+#pragma unroll
+      for (int i = 0; i < NV; i++) HS[i] = HN[i];
+      amaxS = amaxN;
+      S = Sn;
+    }
+  }
+  if (compute_dt) {
+    // wave minimum, one atomic per wave (dt > 0: IEEE order == integer order)
+    for (int off = 32; off > 0; off >>= 1) dtmin = fmin(dtmin, shfl_d(dtmin, (lane + off) & 63));
+    if (lane == 0) atomicMin(D.dt_bits, (unsigned long long)__double_as_longlong(dtmin));
+  }
+}
+
+// ---------------------------------------------------------------------------
+// k_dtpolicy: src/IMEX_SfloW2D.f90:357-381 on the device (one thread)
+// ---------------------------------------------------------------------------
+__global__ void k_dtpolicy(DevParams P, DevPtrs D) {
+  double* s = D.scal;
+  if (s[S_DONE] != 0.0) return;
+  double t = s[S_T], t_end = s[S_T_END], t_out = s[S_T_OUT];
+  if (!(t < t_end)) {
+    s[S_DONE] = 1.0;
+    return;
+  }
+  double dt = (P.cfl != -1.0) ? __longlong_as_double((long long)*D.dt_bits) : P.max_dt;
+  s[S_DT_CFL] = dt;
+  if (t_end - t_out < 1.0e-7) t_out = t_end;
+  if (t + dt > t_end) dt = t_end - t;
+  if (t + dt > t_out) dt = t_out - t;
+  dt = fmin(dt, 1.1 * 0.5 * (s[S_DT_OLD] + s[S_DT_OLD_OLD]));
+  s[S_DT_OLD_OLD] = s[S_DT_OLD];
+  s[S_DT_OLD] = dt;
+  s[S_DT] = dt;
+  s[S_T_OUT] = t_out;
+}
+
+// t = t + dt, reset of the CFL accumulator and the solve-cells counter
+__global__ void k_advance_time(DevPtrs D) {
+  double* s = D.scal;
+  if (s[S_DONE] != 0.0) return;
+  s[S_T] = s[S_T] + s[S_DT];
+  D.counters[C_STEPS] += 1;
+  D.counters[C_ACTIVE_UPD] += D.counters[C_SOLVE_CELLS];
+}
+
+__global__ void k_reset_step(DevParams P, DevPtrs D) {
+  *D.dt_bits = (unsigned long long)__double_as_longlong(P.max_dt);
+  D.counters[C_SOLVE_CELLS] = 0;
+}
+
+// ---------------------------------------------------------------------------
+// k_local: stage 2 + assembly (imex_RK_solver :1105-1626), erosion /
+// deposition (update_erosion_deposition_cell :2380-2595) and the post-step
+// loop (src/IMEX_SfloW2D.f90:391-447), fused per cell.
+// ---------------------------------------------------------------------------
+enum LocalOps { OP_RK = 1, OP_EROSION = 2, OP_POST = 4, OP_SUMS = 8 };
+
+template <int NS>
+__global__ void __launch_bounds__(256) k_local(DevParams P, DevPtrs D, int ops) {
+  constexpr int NV = 4 + NS;
+  const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+  const int j = blockIdx.x * kTileX + tx;
+  const int k = blockIdx.y * kTileY + ty;
+  const bool inb = (j < P.nx && k < P.ny);
+  const size_t c = inb ? cidx(P, j, k) : 0;
+  const bool act = inb && D.mask[c] && (D.scal[S_DONE] == 0.0);
+  __shared__ double red[4][4];
+  __shared__ unsigned long long cnt[4][8];
+  double s_mass = 0.0, s_vol = 0.0, s_area = 0.0, s_sol = 0.0;
+  unsigned long long e[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  if (act) {
+    const size_t pl = plane(P);
+    const double dt = D.scal[S_DT];
+    double q[NV], qp[NV + 2];
+#pragma unroll
+    for (int i = 0; i < NV; i++) q[i] = D.q[i * pl + c];
+    if (ops & OP_RK) {
+      double q0[NV], Dr[NV];
+#pragma unroll
+      for (int i = 0; i < NV; i++) {
+        q0[i] = q[i];
+        Dr[i] = D.D[i * pl + c];
+      }
+      const double bx = D.Bx[c], by = D.By[c];
+      const double bxx = D.Bxx ? D.Bxx[c] : 0.0, bxy = D.Bxy ? D.Bxy[c] : 0.0,
+                   byy = D.Byy ? D.Byy[c] : 0.0;
+      const double gc = D.gcoef ? D.gcoef[c] : 1.0;
+      int nonconv = 0;
+      int it = stage2<NS>(P, q0, Dr, dt, bx, by, bxx, bxy, byy, gc, q, nonconv);
+      if (it > 0) {
+        e[0] += 1;
+        e[1] += it;
+        e[2] += nonconv;
+      }
+      // assembly checks (:1379-1624)
+      bool isn = false;
+#pragma unroll
+      for (int i = 0; i < NV; i++) isn |= isnan(q[i]);
+      e[3] += isn;
+      if (q[0] < kEps) {
+        if (q[0] > -1.0e-7) {
+#pragma unroll
+          for (int i = 0; i < NV; i++) q[i] = 0.0;
+        } else {
+          e[4] += 1;
+        }
+      }
+      {
+        bool anyneg = false, anybig = false;
+#pragma unroll
+        for (int i = 0; i < NS; i++) {
+          anyneg |= q[4 + i] < 0.0;
+          anybig |= q[4 + i] <= -1.0e-7;
+        }
+        if (anyneg) {
+          if (anybig) e[5] += 1;
+          else {
+#pragma unroll
+            for (int i = 0; i < NS; i++)
+              if (q[4 + i] > -1.0e-7) q[4 + i] = 0.0;
+          }
+        }
+      }
+      phys<NS>(P, q, qp);
+      if (qp[3] < 273.0) e[7] += 1;
+      double ssol = 0.0;
+#pragma unroll
+      for (int i = 0; i < NS; i++) ssol += q[4 + i];
+      if (ssol > q[0]) {
+        if (((ssol - q[0]) / q[0] < 1.0e-10) || (q[0] < kEps)) {
+#pragma unroll
+          for (int i = 0; i < NS; i++) q[4 + i] = q[4 + i] / ssol * q[0];
+        } else {
+          e[6] += 1;
+        }
+      }
+      if (D.nodata && D.nodata[c]) {
+#pragma unroll
+        for (int i = 0; i < NV; i++) q[i] = 0.0;
+      }
+    }
+    if ((ops & OP_EROSION) &&
+        !((P.erosion_coeff == 0.0) && !P.settling && !P.entrain)) {
+      phys_or_dry<NS>(P, q, qp);
+      double er[NS > 0 ? NS : 1], dep[NS > 0 ? NS : 1], eq[NV], topo;
+      double erod[NS > 0 ? NS : 1];
+#pragma unroll
+      for (int i = 0; i < NS; i++) erod[i] = D.erodible[i * pl + c];
+      mass_exchange<NS>(P, qp, D.Bx[c], D.By[c], erod, dt, er, dep, eq, topo);
+#pragma unroll
+      for (int i = 0; i < NV; i++) q[i] = q[i] + dt * eq[i];
+#pragma unroll
+      for (int i = 0; i < NS; i++) q[4 + i] = fmax(0.0, q[4 + i]);
+#pragma unroll
+      for (int i = 0; i < NS; i++) {
+        D.deposit[i * pl + c] = D.deposit[i * pl + c] + dt * dep[i];
+        D.erosion[i * pl + c] = D.erosion[i * pl + c] + dt * er[i];
+        double ev = erod[i] - dt * er[i];
+        if (P.erod_dep) ev = ev + dt * dep[i];
+        D.erodible[i * pl + c] = ev;
+      }
+      if (P.topo) D.B[c] = D.B[c] + dt * topo;
+      if (q[0] <= 0.0 && q[0] > -1.0e-10) {
+#pragma unroll
+        for (int i = 0; i < NV; i++) q[i] = 0.0;
+      }
+      double ssol = 0.0;
+#pragma unroll
+      for (int i = 0; i < NS; i++) ssol += q[4 + i];
+      if (ssol > q[0] && q[0] < 1.0e-10) {
+#pragma unroll
+        for (int i = 0; i < NS; i++) q[4 + i] = q[4 + i] / ssol * q[0];
+      }
+      double rg = 0.0;
+      if (q[0] > 0.0) {
+        phys<NS>(P, q, qp);
+        rg = mixt<NS, false>(P, qp).rg;
+      }
+      if (rg <= 0.0) {
+#pragma unroll
+        for (int i = 0; i < NV; i++) q[i] = 0.0;
+      }
+    }
+    if (ops & (OP_RK | OP_EROSION)) {
+#pragma unroll
+      for (int i = 0; i < NV; i++) D.q[i * pl + c] = q[i];
+    }
+    if (ops & OP_POST) {
+      double pd = phys_or_dry<NS>(P, q, qp);
+      if (q[0] > 0.0) {
+        D.hmax[c] = fmax(D.hmax[c], qp[0]);
+        const double mv = sqrt(qp[NV] * qp[NV] + qp[NV + 1] * qp[NV + 1]);
+        if (qp[0] > 0.001) {
+          D.pdynmax[c] = fmax(D.pdynmax[c], pd);
+          D.modvel[c] = fmax(D.modvel[c], mv);
+        }
+        if (D.vuln) {
+          uint32_t bits = D.vuln[c];
+          int it = 0;
+          for (int a = 0; a < P.n_thk; a++) {
+            const bool th = qp[0] >= P.thickness_levels[a];
+            for (int b = 0; b < P.n_pdyn; b++) {
+              const bool pdb = pd >= P.dyn_pres_levels[b];
+              if (th && pdb) bits |= (1u << it);
+              it++;
+            }
+          }
+          D.vuln[c] = bits;
+        }
+      }
+      if (ops & OP_SUMS) {
+        s_mass = q[0];
+        s_vol = qp[0];
+        s_area = (q[0] > 1.0e-7) ? 1.0 : 0.0;
+#pragma unroll
+        for (int i = 0; i < NS; i++) s_sol += q[4 + i];
+      }
+    }
+  }
+  // counters: wave reduction, one atomic per wave and counter
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+  for (int m = 0; m < 8; m++) {
+    unsigned long long v = e[m];
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    if (lane == 0) cnt[w][m] = v;
+  }
+  if (ops & OP_SUMS) {
+    double v4[4] = {s_mass, s_vol, s_area, s_sol};
+#pragma unroll
+    for (int m = 0; m < 4; m++) {
+      double v = v4[m];
+      for (int off = 32; off > 0; off >>= 1) v += shfl_d(v, lane ^ off);
+      if (lane == 0) red[w][m] = v;
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int m = 0; m < 8; m++) {
+      unsigned long long v = cnt[0][m] + cnt[1][m] + cnt[2][m] + cnt[3][m];
+      if (v) atomicAdd(&D.counters[C_NEWTON_CELLS + m], v);
+    }
+    if (ops & OP_SUMS) {
+      const size_t tile = (size_t)blockIdx.y * gridDim.x + blockIdx.x;
+      for (int m = 0; m < 4; m++)
+        D.tile_sums[4 * tile + m] = ((red[0][m] + red[1][m]) + red[2][m]) + red[3][m];
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// k_topo: topography_reconstruction (src/geometry_2d.f90:688-856) for the
+// fields the hot path reads.  Border cells take the value of the nearest
+// cell of the 5-point interior (:737-796).  Only tiles within one tile of an
+// active cell are recomputed when `sparse` (the topography changes only
+// where the flow is).
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_topo(DevParams P, DevPtrs D, int sparse) {
+  const int j = blockIdx.x * kTileX + (threadIdx.x & 63);
+  const int k = blockIdx.y * kTileY + (threadIdx.x >> 6);
+  if (sparse) {
+    // any active cell within 4 cells (5-point stencil around a clamped
+    // centre)
+    int any = 0;
+    if (j < P.nx && k < P.ny) {
+      for (int dk = -4; dk <= 4 && !any; dk += 4)
+        for (int dj = -4; dj <= 4; dj += 4) {
+          int jj = min(max(j + dj, 0), P.nx - 1), kk = min(max(k + dk, 0), P.ny - 1);
+          if (D.mask[cidx(P, jj, kk)]) {
+            any = 1;
+            break;
+          }
+        }
+    }
+    if (!__syncthreads_or(any)) return;
+  }
+  if (j >= P.nx || k >= P.ny) return;
+  const size_t c = cidx(P, j, k);
+  double bx = 0.0, by = 0.0, bxx = 0.0, byy = 0.0, bxy = 0.0;
+  if (P.nx >= 5 && P.ny >= 5) {
+    const int jj = min(max(j, 2), P.nx - 3), kk = min(max(k, 2), P.ny - 3);
+    const double c1[5] = {-2.0, -1.0, 0.0, 1.0, 2.0};
+    const double c2[5] = {2.0, -1.0, -2.0, -1.0, 2.0};
+    double px = 0.0, py = 0.0, sxx = 0.0, syy = 0.0;
+#pragma unroll
+    for (int t = 0; t < 5; t++) {
+      const double b = D.B[cidx(P, jj + t - 2, kk)];
+      px = px + c1[t] * b;
+      sxx = sxx + c2[t] * b;
+    }
+#pragma unroll
+    for (int t = 0; t < 5; t++) {
+      const double b = D.B[cidx(P, jj, kk + t - 2)];
+      py = py + c1[t] * b;
+      syy = syy + c2[t] * b;
+    }
+    bx = px / (10.0 * P.dx);
+    by = py / (10.0 * P.dy);
+    bxx = sxx / (7.0 * (P.dx * P.dx));
+    byy = syy / (7.0 * (P.dy * P.dy));
+    if (D.Bxy) {
+      double ws = 0.0;
+      for (int a = 0; a < 5; a++)
+        for (int b = 0; b < 5; b++)
+          ws = ws + c1[b] * c1[a] * D.B[cidx(P, jj + b - 2, kk + a - 2)];
+      bxy = ws / ((10.0 * P.dx) * (10.0 * P.dy));
+    }
+  }
+  D.Bx[c] = bx;
+  D.By[c] = by;
+  if (D.Bxx) {
+    D.Bxx[c] = bxx;
+    D.Byy[c] = byy;
+    D.Bxy[c] = bxy;
+  }
+  if (D.gcoef) {
+    const double g = 1.0 / (1.0 + bx * bx + by * by);
+    D.gcoef[c] = g;
+    D.dgdx[c] = -(2.0 * (g * g) * (bx * bxx + by * bxy));
+  }
+}
+
+// staggered grav_coeff (slope correction only), src/geometry_2d.f90:838-848
+__global__ void k_gstag(DevParams P, DevPtrs D) {
+  const int j = blockIdx.x * kTileX + (threadIdx.x & 63);
+  const int k = blockIdx.y * kTileY + (threadIdx.x >> 6);
+  if (k < P.ny && j <= P.nx) {
+    double g;
+    if (j == 0) g = D.gcoef[cidx(P, 0, k)];
+    else if (j == P.nx) g = D.gcoef[cidx(P, P.nx - 1, k)];
+    else g = 0.5 * (D.gcoef[cidx(P, j - 1, k)] + D.gcoef[cidx(P, j, k)]);
+    D.gsx[(size_t)k * (P.pitch + 1) + j] = g;
+  }
+  if (k <= P.ny && j < P.nx) {
+    double g;
+    if (k == 0) g = D.gcoef[cidx(P, j, 0)];
+    else if (k == P.ny) g = D.gcoef[cidx(P, j, P.ny - 1)];
+    else g = 0.5 * (D.gcoef[cidx(P, j, k - 1)] + D.gcoef[cidx(P, j, k)]);
+    D.gsy[(size_t)k * P.pitch + j] = g;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// layout conversion: host Fortran (var, j, k) <-> device planes
+// ---------------------------------------------------------------------------
+__global__ void k_aos_to_planes(DevParams P, const double* __restrict__ src, int nvar,
+                                double* __restrict__ dst) {
+  const int j = blockIdx.x * kTileX + (threadIdx.x & 63);
+  const int k = blockIdx.y * kTileY + (threadIdx.x >> 6);
+  if (j >= P.nx || k >= P.ny) return;
+  const size_t pl = plane(P), s = ((size_t)k * P.nx + j) * nvar;
+  for (int i = 0; i < nvar; i++) dst[i * pl + cidx(P, j, k)] = src[s + i];
+}
+
+__global__ void k_planes_to_aos(DevParams P, const double* __restrict__ src, int nvar,
+                                double* __restrict__ dst) {
+  const int j = blockIdx.x * kTileX + (threadIdx.x & 63);
+  const int k = blockIdx.y * kTileY + (threadIdx.x >> 6);
+  if (j >= P.nx || k >= P.ny) return;
+  const size_t pl = plane(P), s = ((size_t)k * P.nx + j) * nvar;
+  for (int i = 0; i < nvar; i++) dst[s + i] = src[i * pl + cidx(P, j, k)];
+}
+
+// qp(n_vars+2, j, k) of the current state (the module array qp after the
+// post-step loop: src/IMEX_SfloW2D.f90:395-445)
+template <int NS>
+__global__ void k_qp_out(DevParams P, const double* __restrict__ q, double* __restrict__ dst) {
+  constexpr int NV = 4 + NS;
+  const int j = blockIdx.x * kTileX + (threadIdx.x & 63);
+  const int k = blockIdx.y * kTileY + (threadIdx.x >> 6);
+  if (j >= P.nx || k >= P.ny) return;
+  const size_t pl = plane(P), c = cidx(P, j, k), s = ((size_t)k * P.nx + j) * (NV + 2);
+  double qq[NV], qp[NV + 2];
+#pragma unroll
+  for (int i = 0; i < NV; i++) qq[i] = q[i * pl + c];
+  phys_or_dry<NS>(P, qq, qp);
+#pragma unroll
+  for (int i = 0; i < NV + 2; i++) dst[s + i] = qp[i];
+}
+
+}  // namespace imexsw

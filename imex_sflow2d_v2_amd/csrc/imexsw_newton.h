@@ -1,0 +1,424 @@
+// imexsw_newton.h -- the per-cell implicit Runge-Kutta stage (one lane per
+// cell, everything in registers): semi-implicit Coulomb predictor, Newton
+// with complex-step Jacobian and backtracking line search for the Voellmy
+// drag, velocity realignment, and the assembly checks of imex_RK_solver.
+#pragma once
+
+#include "imexsw_physics.h"
+
+namespace imexsw {
+
+// eval_f (src/solver_2d.f90:2243-2282)
+template <int NS>
+__device__ __forceinline__ void eval_f(const DevParams& P, const double* qj,
+                                       const double* qold, const double* Rj,
+                                       double a_diag, double dt, double Bx,
+                                       double By, double* f, double& scal) {
+  constexpr int NV = 4 + NS;
+  cpx cq[NV];
+#pragma unroll
+  for (int i = 0; i < NV; i++) cq[i] = mk(qj[i], 0.0);
+  cpx n1, n2;
+  implicit_c<NS>(P, cq, Bx, By, n1, n2);
+  double s = 0.0;
+#pragma unroll
+  for (int i = 0; i < NV; i++) {
+    double nh = (i == 1) ? n1.re : (i == 2) ? n2.re : 0.0;
+    double R = Rj[i] - a_diag * nh;
+    f[i] = 1.0 * (qj[i] - qold[i] + dt * R);
+  }
+#pragma unroll
+  for (int i = 0; i < NV; i++) s += f[i] * f[i];
+  scal = 0.5 * s;
+}
+
+// gfortran NORM2 (scaled sum of squares)
+template <int NV>
+__device__ __forceinline__ double norm2_(const double* x) {
+  double scale = 0.0, res = 1.0;
+#pragma unroll
+  for (int i = 0; i < NV; i++) {
+    if (x[i] != 0.0) {
+      double ax = fabs(x[i]);
+      if (scale < ax) {
+        double r = scale / ax;
+        res = 1.0 + res * (r * r);
+        scale = ax;
+      } else {
+        double r = ax / scale;
+        res = res + r * r;
+      }
+    }
+  }
+  return scale * sqrt(res);
+}
+
+struct NewtonOut {
+  int iters;
+  int conv;
+};
+
+// solve_rk_step with eval_jacobian and lnsrch (src/solver_2d.f90:1660-2365).
+// The implicit unknowns are the momenta (implicit_flag = F,T,T,F,...,
+// src/constitutive_2d.f90:478-480): the Jacobian has the 2x2 momentum block
+// plus the diagonal scaling of the other unknowns, which is what the
+// reference's packed partitioned solve reduces to.
+template <int NS>
+__device__ __forceinline__ NewtonOut solve_rk_step(const DevParams& P, double* qj,
+                                                  const double* qold,
+                                                  const double* Rj, double a_diag,
+                                                  double dt, double Bx, double By) {
+  constexpr int NV = 4 + NS;
+  const double TOLF = 1.0e-10;
+  const double TOLX = kEps;
+  double org[NV], rel[NV], rt[NV], old[NV], desc[NV];
+  double scal, scal_old;
+#pragma unroll
+  for (int i = 0; i < NV; i++) org[i] = fmax(fabs(qj[i]), 1.0e-3);
+#pragma unroll
+  for (int i = 0; i < NV; i++) rel[i] = qj[i] / org[i];
+  NewtonOut out{0, 0};
+  int it;
+  for (it = 1; it <= 100; it++) {
+    eval_f<NS>(P, qj, qold, Rj, a_diag, dt, Bx, By, rt, scal);
+    double mf = 0.0;
+#pragma unroll
+    for (int i = 0; i < NV; i++) mf = fmax(mf, fabs(rt[i]));
+    if (mf < TOLF) {
+      out.conv = 1;
+      break;
+    }
+    // --- Jacobian: complex step on the two momentum columns
+    const double dta = dt * a_diag;
+    double J1[3], J2[3];  // d nh_{1,2} / d rel_{1}, d nh_{1,2} / d rel_{2}
+    {
+      cpx cq[NV];
+#pragma unroll
+      for (int m = 0; m < NV; m++) cq[m] = mk(rel[m], 0.0);
+      cq[1] = mk(rel[1], P.h_cs);
+#pragma unroll
+      for (int m = 0; m < NV; m++) cq[m] = cmulr(cq[m], org[m]);
+      cpx n1, n2;
+      implicit_c<NS>(P, cq, Bx, By, n1, n2);
+      J1[1] = 1.0 * n1.im * P.one_by_h;
+      J1[2] = 1.0 * n2.im * P.one_by_h;
+    }
+    {
+      cpx cq[NV];
+#pragma unroll
+      for (int m = 0; m < NV; m++) cq[m] = mk(rel[m], 0.0);
+      cq[2] = mk(rel[2], P.h_cs);
+#pragma unroll
+      for (int m = 0; m < NV; m++) cq[m] = cmulr(cq[m], org[m]);
+      cpx n1, n2;
+      implicit_c<NS>(P, cq, Bx, By, n1, n2);
+      J2[1] = 1.0 * n1.im * P.one_by_h;
+      J2[2] = 1.0 * n2.im * P.one_by_h;
+    }
+    // left_matrix entries (column 1 and 2 of the momentum block); rows of
+    // the other equations of those columns are 0 - dta*(1.0*0*one_by_h) = 0
+    const double zJ = 1.0 * 0.0 * P.one_by_h;
+    const double L11 = 1.0 * org[1] - dta * J1[1];
+    const double L21 = 0.0 - dta * J1[2];
+    const double L12 = 0.0 - dta * J2[1];
+    const double L22 = 1.0 * org[2] - dta * J2[2];
+    const double Lz = 0.0 - dta * zJ;  // off-block rows of the implicit columns
+    // partitioned solve (:1846-1925)
+    double d1[NV];
+#pragma unroll
+    for (int i = 0; i < NV; i++) d1[i] = (i == 1 || i == 2) ? 0.0 : rt[i] / (1.0 * org[i]);
+    double s = 0.0;
+#pragma unroll
+    for (int i = 0; i < NV; i++)
+      if (i != 1 && i != 2) s += d1[i] * 0.0;
+    double b1 = rt[1] - s, b2 = rt[2] - s;
+    {
+      // the same partial sum for both implicit rows (MATMUL over zeros)
+      double inv_det = 1.0 / (L11 * L22 - L21 * L12);
+      double x1 = (b1 * L22 - b2 * L12) * inv_det;
+      double x2 = (L11 * b2 - L21 * b1) * inv_det;
+#pragma unroll
+      for (int i = 0; i < NV; i++) {
+        if (i == 1) desc[i] = -x1 + 0.0;
+        else if (i == 2) desc[i] = -x2 + 0.0;
+        else desc[i] = 0.0 + -d1[i];
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < NV; i++) old[i] = rel[i];
+    scal_old = scal;
+    if (it > 1) {
+      double dd = 0.0;
+#pragma unroll
+      for (int i = 0; i < NV; i++) dd += rel[i] * rel[i];
+      double stpmax = 100.0 * fmax(sqrt(dd), (double)NV);
+      double grad[NV];
+#pragma unroll
+      for (int i = 0; i < NV; i++) {
+        double g = 0.0;
+#pragma unroll
+        for (int r = 0; r < NV; r++) {
+          double Lri;
+          if (i == 1) Lri = (r == 1) ? L11 : (r == 2) ? L21 : Lz;
+          else if (i == 2) Lri = (r == 1) ? L12 : (r == 2) ? L22 : Lz;
+          else Lri = (r == i) ? 1.0 * org[i] : 0.0;
+          g += rt[r] * Lri;
+        }
+        grad[i] = g;
+      }
+      // ---- lnsrch (:2019-2224)
+      double dabs = norm2_<NV>(desc);
+      if (dabs > stpmax) {
+#pragma unroll
+        for (int i = 0; i < NV; i++) desc[i] = desc[i] * stpmax / dabs;
+      }
+      double slope = 0.0;
+#pragma unroll
+      for (int i = 0; i < NV; i++) slope += grad[i] * desc[i];
+      double mx = -INFINITY;
+#pragma unroll
+      for (int i = 0; i < NV; i++) {
+        double v = fabs(desc[i]) / fmax(fabs(old[i]), 1.0);
+        if (v > mx) mx = v;
+      }
+      double alamin = TOLX / mx;
+      if (alamin == 0.0) {
+#pragma unroll
+        for (int i = 0; i < NV; i++) rel[i] = old[i];
+      } else {
+        double alam = 1.0, alam2 = 0.0, f2 = 0.0, tmplam = 0.0;
+        // alam at least halves every pass after the first, so it falls
+        // below alamin (>= ~1e-16 relative) within a few dozen passes; the
+        // bound only guards against NaN states
+        for (int ls = 0; ls < 256; ls++) {
+          double q[NV];
+#pragma unroll
+          for (int i = 0; i < NV; i++) rel[i] = old[i] + alam * desc[i];
+#pragma unroll
+          for (int i = 0; i < NV; i++) q[i] = rel[i] * org[i];
+          eval_f<NS>(P, q, qold, Rj, a_diag, dt, Bx, By, rt, scal);
+          if (scal <= 0.9 * scal_old) break;
+          if (alam < alamin) {
+#pragma unroll
+            for (int i = 0; i < NV; i++) rel[i] = old[i];
+            scal = scal_old;
+            break;
+          }
+          if (alam == 1.0) {
+            tmplam = -(slope / (2.0 * (scal - scal_old - slope)));
+          } else {
+            double rhs1 = scal - scal_old - alam * slope;
+            double rhs2 = f2 - scal_old - alam2 * slope;
+            double a = (rhs1 / (alam * alam) - rhs2 / (alam2 * alam2)) / (alam - alam2);
+            double b = (-(alam2 * rhs1 / (alam * alam)) + alam * rhs2 / (alam2 * alam2)) /
+                       (alam - alam2);
+            if (a == 0.0) {
+              tmplam = -(slope / (2.0 * b));
+            } else {
+              double disc = b * b - 3.0 * a * slope;
+              if (disc < 0.0) tmplam = 0.5 * alam;
+              else if (b <= 0.0) tmplam = (-b + sqrt(disc)) / (3.0 * a);
+              else tmplam = -(slope / (b + sqrt(disc)));
+            }
+            if (tmplam > 0.5 * alam) tmplam = 0.5 * alam;
+          }
+          alam2 = alam;
+          f2 = scal;
+          alam = fmax(tmplam, 0.5 * alam);
+        }
+      }
+    } else {
+#pragma unroll
+      for (int i = 0; i < NV; i++) rel[i] = old[i] + desc[i];
+#pragma unroll
+      for (int i = 0; i < NV; i++) qj[i] = rel[i] * org[i];
+      eval_f<NS>(P, qj, qold, Rj, a_diag, dt, Bx, By, rt, scal);
+    }
+#pragma unroll
+    for (int i = 0; i < NV; i++) qj[i] = rel[i] * org[i];
+    mf = 0.0;
+#pragma unroll
+    for (int i = 0; i < NV; i++) mf = fmax(mf, fabs(rt[i]));
+    if (mf < TOLF) {
+      out.conv = 1;
+      break;
+    }
+    double mx = 0.0;
+#pragma unroll
+    for (int i = 0; i < NV; i++) mx = fmax(mx, fabs(rel[i] - old[i]) / fmax(fabs(rel[i]), 1.0));
+    if (mx < TOLX) {
+      out.conv = 1;
+      break;
+    }
+  }
+  out.iters = it > 100 ? 100 : it;
+  return out;
+}
+
+// Stage 2 of the N_RK = 2 IMEX scheme for one cell (src/solver_2d.f90:
+// 1105-1280), from q0 = q^n and D = divFlux - expl_terms of stage 1.  The
+// implicit term refresh of :1234-1236 is not evaluated: for the stiffly
+// accurate N_RK = 2 tableau it is overwritten at :1271-1277 and never enters
+// q.  Returns the Newton iteration count (0: no Newton).
+template <int NS>
+__device__ __forceinline__ int stage2(const DevParams& P, const double* q0,
+                                      const double* D, double dt, double Bx,
+                                      double By, double Bxx, double Bxy,
+                                      double Byy, double gc, double* q,
+                                      int& nonconv) {
+  constexpr int NV = 4 + NS;
+  double q_fv[NV], qp[NV + 2], q_si[NV];
+#pragma unroll
+  for (int i = 0; i < NV; i++) q_fv[i] = q0[i] - dt * (D[i] - 0.0);
+  phys<NS>(P, q_fv, qp);
+  nonconv = 0;
+  if (!(q_fv[0] > 0.0)) {
+#pragma unroll
+    for (int i = 0; i < NV; i++) q[i] = q_fv[i];
+    return 0;
+  }
+  double s1, s2;
+  semi_impl<NS>(P, Bx, By, Bxx, Bxy, Byy, gc, qp, s1, s2);
+#pragma unroll
+  for (int i = 0; i < NV; i++) {
+    double si = (i == 1) ? s1 : (i == 2) ? s2 : 0.0;
+    q_si[i] = q_fv[i] + dt * 1.0 * si;
+  }
+  const double f2 = q_fv[1], f3 = q_fv[2];
+  if ((f2 * f2 + f3 * f3) == 0.0) {
+    q_si[1] = 0.0;
+    q_si[2] = 0.0;
+  } else if ((q_si[1] * f2 < 0.0) || (q_si[2] * f3 < 0.0)) {
+    q_si[1] = 0.0;
+    q_si[2] = 0.0;
+  } else {
+    double m = sqrt(q_si[1] * q_si[1] + q_si[2] * q_si[2]);
+    double den = sqrt(f2 * f2 + f3 * f3);
+    q_si[1] = m * f2 / den;
+    q_si[2] = m * f3 / den;
+  }
+  double Rj[NV];
+#pragma unroll
+  for (int i = 0; i < NV; i++) {
+    double si = (q_si[i] - q_fv[i]) / (dt * 1.0);
+    Rj[i] = (D[i] - 0.0) - 1.0 * si;
+    q[i] = q_si[i];
+  }
+  NewtonOut no = solve_rk_step<NS>(P, q, q0, Rj, 1.0, dt, Bx, By);
+  nonconv = !no.conv;
+  if (P.ny == 1) q[2] = 0.0;
+  if (P.nx == 1) q[1] = 0.0;
+  if (q_si[1] * q_si[1] + q_si[2] * q_si[2] == 0.0) {
+    q[1] = 0.0;
+    q[2] = 0.0;
+  } else if ((q[1] * q_si[1] <= 0.0) && (q[2] * q_si[2] <= 0.0)) {
+    q[1] = 0.0;
+    q[2] = 0.0;
+  } else {
+    double m = sqrt(q[1] * q[1] + q[2] * q[2]);
+    double den = sqrt(q_si[1] * q_si[1] + q_si[2] * q_si[2]);
+    q[1] = m * q_si[1] / den;
+    q[2] = m * q_si[2] / den;
+  }
+  return no.iters;
+}
+
+// eval_mass_exchange_terms, supported branches (src/constitutive_2d.f90:
+// 4424-5003).  Returns false when every term is zero.
+template <int NS>
+__device__ __forceinline__ void mass_exchange(const DevParams& P, const double* qp,
+                                              double Bx, double By,
+                                              const double* erodible, double dt,
+                                              double* er, double* dep, double* eq,
+                                              double& topo) {
+  constexpr int NV = 4 + NS;
+#pragma unroll
+  for (int i = 0; i < NS; i++) {
+    er[i] = 0.0;
+    dep[i] = 0.0;
+  }
+#pragma unroll
+  for (int i = 0; i < NV; i++) eq[i] = 0.0;
+  topo = 0.0;
+  if (qp[0] <= kEps) return;
+  const double alpha_max = 0.6, hind_exp = 4.65;
+  double r_h = qp[0], r_u = qp[NV], r_v = qp[NV + 1];
+  double al[NS > 0 ? NS : 1], at = 0.0;
+#pragma unroll
+  for (int i = 0; i < NS; i++) al[i] = P.alpha ? qp[4 + i] : qp[4 + i] / qp[0];
+#pragma unroll
+  for (int i = 0; i < NS; i++) at += al[i];
+  double r_w = P.slope ? r_u * Bx + r_v * By : 0.0;
+  double mv2 = r_u * r_u + r_v * r_v + r_w * r_w;
+  double mv = sqrt(mv2);
+  if (P.erosion_coeff > 0.0) {
+    double te = P.erosion_coeff * mv * r_h * (1.0 - at);
+    double tse = te * (1.0 - P.erodible_porosity);
+#pragma unroll
+    for (int i = 0; i < NS; i++) er[i] = P.erodible_fract[i] * tse;
+  }
+#pragma unroll
+  for (int i = 0; i < NS; i++) er[i] = fmax(0.0, fmin(er[i], erodible[i] / dt));
+  double tse = 0.0;
+#pragma unroll
+  for (int i = 0; i < NS; i++) tse += er[i];
+  double te = tse / (1.0 - P.erodible_porosity);
+  double cont_erosion = te * P.erodible_porosity;
+  if (at <= P.alphastot_min) return;
+  double r_T = qp[3];
+  Mixt m = mixt<NS, true>(P, qp);
+  double inv_kin_visc = 1.0 / P.kin_visc_c;
+#pragma unroll
+  for (int i = 0; i < NS; i++) {
+    if ((al[i] > 0.0) && P.settling) {
+      double sv = settling_velocity(P, P.diam_s[i], P.rho_s[i], m.rho_c, inv_kin_visc);
+      double d = al[i] * sv * 1.0;
+      d = d * pow(1.0 - fmin(1.0, at / alpha_max), hind_exp);
+      dep[i] = fmin(d, r_h * al[i] / dt);
+    }
+  }
+  double sdep = 0.0;
+#pragma unroll
+  for (int i = 0; i < NS; i++) sdep += dep[i];
+  double cl = 0.0 + P.coeff_porosity * sdep;
+  double sal = 0.0;
+#pragma unroll
+  for (int i = 0; i < NS; i++) sal += al[i];
+  cl = fmin(cl, r_h * fmax(0.0, P.max_packing - sal) / dt);
+  cl = fmin(cl, (r_h * (1.0 - sal) - P.coeff_porosity * (r_h * sal - dt * sdep)) / dt);
+  double T_erodible = P.erod_dep ? r_T : P.T_erodible;
+  double air = 0.0;
+  if (P.entrain && (r_h > 0.0) && (m.ri > 0.0)) {
+    double ec = 0.075 / sqrt(1.0 + 718.0 * pow(m.ri, 2.4));
+    air = ec * sqrt(mv2);
+  }
+  double dep_tot = 0.0, ers_tot = 0.0, rdep = 0.0, rers = 0.0, sdc = 0.0, sec = 0.0;
+#pragma unroll
+  for (int i = 0; i < NS; i++) dep_tot += dep[i];
+#pragma unroll
+  for (int i = 0; i < NS; i++) ers_tot += er[i];
+#pragma unroll
+  for (int i = 0; i < NS; i++) rdep += P.rho_s[i] * dep[i];
+#pragma unroll
+  for (int i = 0; i < NS; i++) rers += P.rho_s[i] * er[i];
+  eq[0] = P.rho_a_amb * air + rers - rdep + P.rho_c_sub * cont_erosion - m.rho_c * cl;
+  eq[1] = -(r_u * (rdep + m.rho_c * cl));
+  eq[2] = -(r_v * (rdep + m.rho_c * cl));
+#pragma unroll
+  for (int i = 0; i < NS; i++) sdc += P.rho_s[i] * P.sp_heat_s[i] * dep[i];
+#pragma unroll
+  for (int i = 0; i < NS; i++) sec += P.rho_s[i] * P.sp_heat_s[i] * er[i];
+  if (P.energy)
+    eq[3] = -(r_T * (sdc + m.rho_c * m.shc * cl)) - 0.5 * mv2 * (rdep + m.rho_c * cl) +
+            T_erodible * (sec + P.rho_c_sub * m.shc * cont_erosion) +
+            P.T_ambient * P.sp_heat_a * P.rho_a_amb * air;
+  else
+    eq[3] = -(r_T * (sdc + m.rho_c * m.shc * cl)) +
+            T_erodible * (sec + P.rho_c_sub * m.shc * cont_erosion) +
+            P.T_ambient * P.sp_heat_a * P.rho_a_amb * air;
+#pragma unroll
+  for (int i = 0; i < NS; i++) eq[4 + i] = P.rho_s[i] * (er[i] - dep[i]);
+  topo = (dep_tot - ers_tot) / (1.0 - P.erodible_porosity);
+}
+
+}  // namespace imexsw

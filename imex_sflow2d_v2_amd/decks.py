@@ -1,0 +1,385 @@
+"""Synthetic input decks for the benchmark and test configurations.
+
+The reference reads its inputs from NAMELIST decks, ESRI DEMs and ``.q_2d``
+restart files (src/inpout_2d.f90); these builders produce the same module
+state directly: an ``ImexswParams``, the cell-centred topography ``B_cent``,
+the conservative state ``q(n_vars, nx, ny)`` (Fortran order) and, for a
+radial source, the ring arrays that ``init_source`` derives
+(src/geometry_2d.f90:872-1102).
+
+Host-side set-up only (runs once per run); nothing here is on the per-step
+path.
+"""
+import math
+
+import numpy as np
+
+from .params import make_params, E, N, S, W
+
+
+# --------------------------------------------------------------------------
+# host restatements of the reference's one-time set-up arithmetic
+# --------------------------------------------------------------------------
+
+def qp_to_qc_np(p, qp):
+    """Vectorised physical -> conservative conversion for initial states.
+
+    Follows ``qp_to_qc`` (src/constitutive_2d.f90:1709-2140, non-profile,
+    gas-carrier branch) with the reference's evaluation order.  ``qp`` has
+    shape (n_vars+2, ...) with rows h, hu, hv, T, (h*)alphas..., u, v.
+    """
+    nv = p.n_vars
+    ns = p.n_solid
+    qp = np.asarray(qp, dtype=np.float64)
+    h = qp[0]
+    qc = np.zeros((nv,) + h.shape)
+    wet = h > np.finfo(np.float64).eps
+    with np.errstate(divide="ignore", invalid="ignore"):
+        T = qp[3]
+        al = [qp[4 + i] if p.alpha_flag else qp[4 + i] / qp[0] for i in range(ns)]
+        at = np.zeros_like(h)
+        for a in al:
+            at = at + a
+        ars = [al[i] * p.rho_s[i] for i in range(ns)]
+        rho_a = p.pres / (p.sp_gas_const_a * T)
+        rho_c = ((1.0 - 0.0 - at - 0.0) * rho_a + 0.0) / (1.0 - 0.0 - at)
+        alphac = 1.0 - at
+        sar = np.zeros_like(h)
+        for a in ars:
+            sar = sar + a
+        rho_m = alphac * rho_c + sar
+        inv_rhom = 1.0 / rho_m
+        xs = [a * inv_rhom for a in ars]
+        xc = alphac * rho_c * inv_rhom
+        shc = np.where(xc > np.finfo(np.float64).eps,
+                       ((xc - 0.0) * p.sp_heat_a + 0.0) / xc, p.sp_heat_a)
+        dsh = np.zeros_like(h)
+        for i in range(ns):
+            dsh = dsh + xs[i] * p.sp_heat_s[i]
+        shm = dsh + xc * shc
+        q1 = rho_m * h
+        u, v = qp[nv], qp[nv + 1]
+        qc[0] = q1
+        qc[1] = rho_m * qp[1]
+        qc[2] = rho_m * qp[2]
+        if p.energy_flag:
+            qc[3] = np.where(h > 0.0, h * rho_m * (shm * T + 0.5 * (u * u + v * v)), 0.0)
+        else:
+            qc[3] = h * rho_m * shm * T
+        for i in range(ns):
+            qc[4 + i] = xs[i] * q1
+    qc[:, ~wet] = 0.0
+    return qc
+
+
+def mixt_var_np(p, qp):
+    """(rho_m, rho_c, red_grav) of one physical state (src/constitutive_2d.f90:1333-1483)."""
+    h = qp[0]
+    if h <= np.finfo(np.float64).eps:
+        rho_a_amb = p.pres / (p.sp_gas_const_a * p.T_ambient)
+        return rho_a_amb, rho_a_amb, 0.0
+    T = qp[3]
+    al = [qp[4 + i] if p.alpha_flag else qp[4 + i] / qp[0] for i in range(p.n_solid)]
+    at = 0.0
+    for a in al:
+        at += a
+    rho_a = p.pres / (p.sp_gas_const_a * T)
+    rho_c = ((1.0 - 0.0 - at - 0.0) * rho_a + 0.0) / (1.0 - 0.0 - at)
+    d = 0.0
+    for i in range(p.n_solid):
+        d += al[i] * p.rho_s[i]
+    rho_m = (1.0 - at) * rho_c + d
+    rho_a_amb = p.pres / (p.sp_gas_const_a * p.T_ambient)
+    rg = (rho_m - rho_a_amb) / rho_m * p.grav
+    return rho_m, rho_c, rg
+
+
+def h_source_from_mfr(p, mfr, vel, T_source, alphas_source, r_source):
+    """MFR_SOURCE + VEL_SOURCE -> H_SOURCE (src/inpout_2d.f90:3930-3945, 4466-4480)."""
+    source_length = math.pi * (r_source + r_source) * 1.0
+    nv = p.n_vars
+    qp = np.zeros(nv + 2)
+    qp[0] = 1.0
+    qp[1] = 1.0 * vel
+    qp[3] = T_source
+    for i in range(p.n_solid):
+        qp[4 + i] = alphas_source[i] if p.alpha_flag else alphas_source[i] * 1.0
+    qp[nv] = vel
+    rho_m, _, _ = mixt_var_np(p, qp)
+    return mfr / (source_length * rho_m * vel * 1.0)
+
+
+def cell_centres(p):
+    dx, dy = p.dx, p.dy
+    x_stag = p.x0 + np.arange(p.nx + 1) * dx
+    y_stag = p.y0 + np.arange(p.ny + 1) * dy
+    x_comp = 0.5 * (x_stag[:-1] + x_stag[1:])
+    y_comp = 0.5 * (y_stag[:-1] + y_stag[1:])
+    return x_comp, y_comp, x_stag, y_stag
+
+
+def init_source_np(p):
+    """Radial source ring (init_source, src/geometry_2d.f90:1005-1100).
+
+    Returns (source_cell[int32], dir_bits[uint8], vect[8, nx, ny]) in
+    Fortran (j, k) order.
+    """
+    nx, ny = p.nx, p.ny
+    sc = np.zeros((nx, ny), dtype=np.int32)
+    bits = np.zeros((nx, ny), dtype=np.uint8)
+    vect = np.zeros((8, nx, ny))
+    if not p.radial_source_flag:
+        return sc, bits, vect
+    xc, yc, xs_, ys_ = cell_centres(p)
+    xs, ys, rs = p.x_source, p.y_source, p.r_source
+    r2 = rs * rs
+
+    def sq(a):
+        return a * a
+
+    for k in range(1, ny - 1):
+        for j in range(1, nx - 1):
+            if sq(xc[j] - xs) + sq(yc[k] - ys) <= r2:
+                sc[j, k] = 1
+                if sq(xc[j - 1] - xs) + sq(yc[k] - ys) >= r2:
+                    sc[j - 1, k] = 2
+                    bits[j - 1, k] |= 1
+                    d = math.sqrt(sq(xs_[j] - xs) + sq(yc[k] - ys))
+                    vect[0, j - 1, k] = (xs_[j] - xs) * rs / sq(d)
+                    vect[1, j - 1, k] = (yc[k] - ys) * rs / sq(d)
+                elif sq(xc[j + 1] - xs) + sq(yc[k] - ys) >= r2:
+                    sc[j + 1, k] = 2
+                    bits[j + 1, k] |= 2
+                    d = math.sqrt(sq(xs_[j + 1] - xs) + sq(yc[k] - ys))
+                    vect[2, j + 1, k] = (xs_[j + 1] - xs) * rs / sq(d)
+                    vect[3, j + 1, k] = (yc[k] - ys) * rs / sq(d)
+                if sq(xc[j] - xs) + sq(yc[k - 1] - ys) >= r2:
+                    sc[j, k - 1] = 2
+                    bits[j, k - 1] |= 8
+                    d = math.sqrt(sq(xc[j] - xs) + sq(ys_[k] - ys))
+                    vect[6, j, k - 1] = (xc[j] - xs) * rs / sq(d)
+                    vect[7, j, k - 1] = (ys_[k] - ys) * rs / sq(d)
+                elif sq(xc[j] - xs) + sq(yc[k + 1] - ys) >= r2:
+                    sc[j, k + 1] = 2
+                    bits[j, k + 1] |= 4
+                    d = math.sqrt(sq(xc[j] - xs) + sq(ys_[k + 1] - ys))
+                    vect[4, j, k + 1] = (xc[j] - xs) * rs / sq(d)
+                    vect[5, j, k + 1] = (ys_[k + 1] - ys) * rs / sq(d)
+    return sc, bits, vect
+
+
+class Deck:
+    """One run's inputs: params, topography, initial q, erodible layer."""
+
+    def __init__(self, name, params, B_cent, q, erodible=None, t_end=None,
+                 steps=None, dt0=1e-4, description=""):
+        self.name = name
+        self.params = params
+        self.B_cent = np.asfortranarray(B_cent, dtype=np.float64)
+        self.q = np.asfortranarray(q, dtype=np.float64)
+        nx, ny = params.nx, params.ny
+        if erodible is None:
+            erodible = np.zeros((params.n_solid, nx, ny))
+        self.erodible = np.asfortranarray(erodible, dtype=np.float64)
+        self.t_end = t_end
+        self.steps = steps
+        self.dt0 = dt0
+        self.description = description
+
+    @property
+    def n_cells(self):
+        return self.params.nx * self.params.ny
+
+
+def _state_from_h(p, h, T, alphas, u=None, v=None):
+    nv = p.n_vars
+    qp = np.zeros((nv + 2,) + h.shape)
+    qp[0] = h
+    u = np.zeros_like(h) if u is None else u
+    v = np.zeros_like(h) if v is None else v
+    qp[1] = h * u
+    qp[2] = h * v
+    qp[3] = T
+    for i, a in enumerate(alphas):
+        qp[4 + i] = a if p.alpha_flag else a * h
+    qp[nv] = u
+    qp[nv + 1] = v
+    return qp_to_qc_np(p, qp)
+
+
+# --------------------------------------------------------------------------
+# the five benchmark configurations
+# --------------------------------------------------------------------------
+
+def dambreak_1d(nx=400, energy=True, t_end=200.0, n_rk=2, cfl=0.4):
+    """1D dam break, 400 x 1 cells, dx = 1 m, flat bed: h = 2 m for x < 200 m
+    and 1 m elsewhere, at rest; one solid class (2500 kg/m3, 1 mm, 0.01);
+    ideal-gas carrier at 300 K; Voellmy mu = 0.1, xi = 1000 m/s2; N_RK = 2,
+    CFL 0.4, 200 s."""
+    p = make_params(nx, 1, 1.0, 1, n_rk=n_rk, energy_flag=energy, cfl=cfl,
+                    max_dt=1.0, mu=0.1, xi=1000.0, rho_s=(2500.0,),
+                    diam_s=(1.0e-3,), sp_heat_s=(1617.0,), T_ambient=300.0)
+    xc, _, _, _ = cell_centres(p)
+    h = np.where(xc < 0.5 * nx * p.dx, 2.0, 1.0)[:, None]
+    q = _state_from_h(p, h, 300.0, [0.01])
+    return Deck("dambreak_1d", p, np.zeros((nx, 1)), q, t_end=t_end,
+                dt0=1e-3, description="1D dam break 400x1 dx=1")
+
+
+def incline_2d(n=512, dx=10.0, steps=2000, energy=True, settling=True):
+    """2D collapse on an incline: n x n cells (512 x 512), dx = 10 m, planar
+    10 degree slope; Gaussian column h = 20 exp(-r^2/(2 200^2)) m at rest
+    centred at (L/4, L/2); two solid classes (0.5 mm and 2 mm, volume
+    fractions 0.02 and 0.01); carrier at 600 K; sedimentation on, erosion off;
+    2000 time steps; CFL 0.24; Voellmy 0.1 / 1000."""
+    p = make_params(n, n, dx, 2, energy_flag=energy, cfl=0.24, max_dt=1.0,
+                    rho_s=(2500.0, 2500.0), diam_s=(0.5e-3, 2.0e-3),
+                    sp_heat_s=(1617.0, 1617.0), settling=settling,
+                    T_ambient=300.0)
+    xc, yc, _, _ = cell_centres(p)
+    L = n * dx
+    X, Y = np.meshgrid(xc, yc, indexing="ij")
+    B = (L - X) * math.tan(math.radians(10.0))
+    r2 = (X - 0.25 * L) ** 2 + (Y - 0.5 * L) ** 2
+    h = 20.0 * np.exp(-r2 / (2.0 * 200.0 ** 2))
+    q = _state_from_h(p, h, 600.0, [0.02, 0.01])
+    return Deck("incline_2d", p, B, q, steps=steps, dt0=1e-3,
+                description="2D Gaussian column on 10 deg incline %dx%d dx=%g" % (n, n, dx))
+
+
+def volcano_dem(n, dx, seed=3, cone_h=1500.0, cone_r=3000.0, n_bumps=32,
+                noise_rms=20.0):
+    """Cone + seeded Gaussian bumps (amplitude U[10,80] m, sigma U[50,300] m)
+    + fractal (1/f^2 spectrum) noise rescaled to noise_rms, clamped at zero as
+    the reference clamps its DEM (src/geometry_2d.f90:322)."""
+    rng = np.random.default_rng(seed)
+    L = n * dx
+    xc = (np.arange(n) + 0.5) * dx
+    X, Y = np.meshgrid(xc, xc, indexing="ij")
+    r = np.hypot(X - 0.5 * L, Y - 0.5 * L)
+    B = cone_h * np.clip(1.0 - r / cone_r, 0.0, None)
+    amp = rng.uniform(10.0, 80.0, n_bumps)
+    sig = rng.uniform(50.0, 300.0, n_bumps)
+    cx = rng.uniform(0.0, L, n_bumps)
+    cy = rng.uniform(0.0, L, n_bumps)
+    for a, s, x0, y0 in zip(amp, sig, cx, cy):
+        B += a * np.exp(-((X - x0) ** 2 + (Y - y0) ** 2) / (2.0 * s * s))
+    kx = np.fft.fftfreq(n)
+    K2 = kx[:, None] ** 2 + kx[None, :] ** 2
+    K2[0, 0] = 1.0
+    spec = (rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n))) / K2
+    spec[0, 0] = 0.0
+    noise = np.real(np.fft.ifft2(spec))
+    noise *= noise_rms / max(noise.std(), 1e-30)
+    B = np.maximum(B + noise, 0.0)
+    return B
+
+
+def volcano_2048(n=2048, dx=5.0, steps=1000, energy=True, mfr=1.0e6,
+                 vel_source=10.0, erosion_coeff=1.0e-5, erodible=1.0,
+                 topo_change=True, seed=3):
+    """Large synthetic volcanic DEM: n x n (2048 x 2048) cells, dx = 5 m;
+    cone 1500 m high, 3 km radius, 32 bumps, 20 m RMS fractal noise; radial
+    source of 1e6 kg/s over a 50 m radius at the summit (VEL_SOURCE 10 m/s,
+    T_SOURCE 800 K); three solid classes (0.1, 1, 4 mm); erosion (coefficient
+    1e-5, 1 m erodible layer, porosity 0.3) and deposition on, topography
+    change on; 1000 time steps, CFL 0.24."""
+    L = n * dx
+    alphas_src = [0.02, 0.02, 0.01]
+    p = make_params(n, n, dx, 3, energy_flag=energy, cfl=0.24, max_dt=1.0,
+                    rho_s=(2500.0, 2500.0, 2500.0),
+                    diam_s=(0.1e-3, 1.0e-3, 4.0e-3),
+                    sp_heat_s=(1617.0, 1617.0, 1617.0), settling=True,
+                    erosion_coeff=erosion_coeff, erodible_porosity=0.3,
+                    erodible_fract=(0.4, 0.4, 0.2), T_erodible=300.0,
+                    topo_change=topo_change, radial_source=True,
+                    T_ambient=300.0)
+    r_src = 50.0
+    hs = h_source_from_mfr(p, mfr, vel_source, 800.0, alphas_src, r_src)
+    p2 = make_params(n, n, dx, 3, energy_flag=energy, cfl=0.24, max_dt=1.0,
+                     rho_s=(2500.0, 2500.0, 2500.0),
+                     diam_s=(0.1e-3, 1.0e-3, 4.0e-3),
+                     sp_heat_s=(1617.0, 1617.0, 1617.0), settling=True,
+                     erosion_coeff=erosion_coeff, erodible_porosity=0.3,
+                     erodible_fract=(0.4, 0.4, 0.2), T_erodible=300.0,
+                     topo_change=topo_change, radial_source=True,
+                     T_ambient=300.0,
+                     source=dict(x=0.5 * L, y=0.5 * L, r=r_src, h=hs,
+                                 vel=vel_source, T=800.0, alphas=alphas_src,
+                                 time_param=(1e9, 1e9, 0.0, 1e9)))
+    B = volcano_dem(n, dx, seed=seed)
+    q = np.zeros((p2.n_vars, n, n))
+    er = np.full((3, n, n), erodible)
+    return Deck("volcano_%d" % n, p2, B, q, erodible=er, steps=steps, dt0=1e-3,
+                description="volcanic DEM %dx%d dx=%g, radial source" % (n, n, dx))
+
+
+def wet_2048(n=2048, dx=5.0, steps=1000, energy=True):
+    """Fully wet 2048^2 variant (the incline geometry scaled) used for the
+    bandwidth roofline: every cell is active from the first step."""
+    p = make_params(n, n, dx, 3, energy_flag=energy, cfl=0.24, max_dt=1.0,
+                    rho_s=(2500.0, 2500.0, 2500.0),
+                    diam_s=(0.1e-3, 1.0e-3, 4.0e-3),
+                    sp_heat_s=(1617.0, 1617.0, 1617.0), settling=True,
+                    T_ambient=300.0)
+    xc, yc, _, _ = cell_centres(p)
+    L = n * dx
+    X, Y = np.meshgrid(xc, yc, indexing="ij")
+    B = (L - X) * math.tan(math.radians(10.0))
+    r2 = (X - 0.25 * L) ** 2 + (Y - 0.5 * L) ** 2
+    h = 1.0 + 20.0 * np.exp(-r2 / (2.0 * (0.05 * L) ** 2))
+    q = _state_from_h(p, h, 600.0, [0.02, 0.01, 0.005])
+    return Deck("wet_%d" % n, p, B, q, steps=steps, dt0=1e-3,
+                description="fully wet %dx%d dx=%g incline" % (n, n, dx))
+
+
+def small_2d(n=48, ny=None, dx=10.0, seed=0, energy=True, alpha=False,
+             settling=True, erosion=False, source=False, topo_change=False,
+             slope_correction=False, curvature=False, n_solid=2, rheology=True,
+             bc=None, limiter=1, n_rk=2, wet_floor=0.0, entrainment=True):
+    """Small seeded 2D decks for the parity tests (seconds on the oracle)."""
+    ny = n if ny is None else ny
+    rng = np.random.default_rng(seed)
+    rho = (2500.0, 2200.0, 2700.0)[:n_solid]
+    dia = (0.5e-3, 2.0e-3, 0.1e-3)[:n_solid]
+    shs = (1617.0, 1500.0, 1400.0)[:n_solid]
+    L = n * dx
+    src = None
+    if source:
+        src = dict(x=0.5 * L + 0.3 * dx, y=0.5 * ny * dx - 0.2 * dx, r=2.6 * dx,
+                   h=2.0, vel=5.0, T=700.0, alphas=[0.02, 0.01, 0.01][:n_solid],
+                   time_param=(1e9, 1e9, 0.0, 1e9))
+    p = make_params(n, ny, dx, n_solid, energy_flag=energy, alpha_flag=alpha,
+                    cfl=0.24, max_dt=1.0, rho_s=rho, diam_s=dia, sp_heat_s=shs,
+                    settling=settling, erosion_coeff=(1e-3 if erosion else 0.0),
+                    erodible_porosity=0.3, T_erodible=300.0,
+                    topo_change=topo_change, radial_source=source,
+                    slope_correction=slope_correction, curvature=curvature,
+                    rheology=rheology, bc=bc, limiter=limiter, n_rk=n_rk,
+                    source=src, entrainment=entrainment,
+                    thickness_levels=(0.1, 1.0), dyn_pres_levels=(10.0, 1000.0))
+    xc, yc, _, _ = cell_centres(p)
+    X, Y = np.meshgrid(xc, yc, indexing="ij")
+    B = 0.15 * (L - X) + 3.0 * np.sin(X / (7 * dx)) * np.cos(Y / (5 * dx)) + 5.0
+    B = np.maximum(B, 0.0)
+    r2 = (X - 0.35 * L) ** 2 + (Y - 0.45 * ny * dx) ** 2
+    h = 4.0 * np.exp(-r2 / (2.0 * (0.12 * L) ** 2))
+    h = np.where(h > 1e-3, h, wet_floor)
+    u = 0.5 * rng.standard_normal(h.shape) * (h > 0)
+    v = 0.5 * rng.standard_normal(h.shape) * (h > 0)
+    T = 400.0 + 50.0 * rng.random(h.shape)
+    al = [0.02 * (1 + 0.2 * rng.random(h.shape)) for _ in range(n_solid)]
+    if alpha:
+        pass
+    q = _state_from_h(p, h, T, al, u, v)
+    er = np.full((n_solid, n, ny), 0.5) if erosion else None
+    return Deck("small2d_%d" % seed, p, B, q, erodible=er, steps=20, dt0=1e-3,
+                description="small 2D test deck")
+
+
+CONFIGS = {
+    "dambreak_1d": dambreak_1d,
+    "incline_2d": incline_2d,
+    "volcano_2048": volcano_2048,
+    "wet_2048": wet_2048,
+}

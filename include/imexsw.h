@@ -1,0 +1,274 @@
+/*
+ * imexsw.h -- C-ABI of the MI355X (gfx950) implementation of the IMEX_SfloW2D
+ * per-timestep hot path: limited reconstruction + central-upwind (Kurganov-
+ * Tadmor) fluxes, IMEX Runge-Kutta stages, the per-cell implicit Newton solve
+ * (complex-step Jacobian, line search) for Voellmy friction, and the explicit
+ * erosion/deposition and post-step diagnostics epilogues.
+ *
+ * The boundary replaces the public procedures and public state of the
+ * reference Fortran module `solver_2d` (src/solver_2d.f90:12-247), which the
+ * unchanged main program calls once per timestep (src/IMEX_SfloW2D.f90:337-531).
+ * Every entry point below names the reference routine it stands in for.
+ *
+ * Conventions
+ *  - All arrays crossing the ABI are host arrays in the reference's Fortran
+ *    column-major layout: q(var, j, k) with the variable index fastest, then
+ *    x (j = 1..nx), then y (k = 1..ny) (src/solver_2d.f90:276-282).  Static
+ *    per-cell fields are (j, k); staggered x-fields are (nx+1, ny), y-fields
+ *    (nx, ny+1).  Device-side layout is private to the library.
+ *  - Everything is FP64 (wp = dp, src/parameters_2d.f90:15).
+ *  - Every call returns int: 0 = OK, < 0 = error (message via
+ *    imexsw_last_error), > 0 = count of numerical events (NaN / negative
+ *    thickness / non-converged Newton cells) that the reference would have
+ *    reported by printing and blocking on stdin (src/solver_2d.f90:1379-1471).
+ *    Nothing crosses the ABI as an exception and no call ever blocks on stdin.
+ *  - One host thread per context; calls on one context are not re-entrant.
+ *    Many contexts per process are allowed.
+ */
+#ifndef IMEXSW_H
+#define IMEXSW_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define IMEXSW_MAX_SOLID 6
+#define IMEXSW_MAX_VARS (4 + IMEXSW_MAX_SOLID)
+#define IMEXSW_MAX_LEVELS 8
+
+/* Side order used by bc_flag / bc_value: west, east, south, north. */
+enum { IMEXSW_W = 0, IMEXSW_E = 1, IMEXSW_S = 2, IMEXSW_N = 3 };
+
+/* Source-ring direction bits (geometry_2d sourceE/W/S/N,
+ * src/geometry_2d.f90:872-1102). */
+enum { IMEXSW_SRC_E = 1, IMEXSW_SRC_W = 2, IMEXSW_SRC_S = 4, IMEXSW_SRC_N = 8 };
+
+/*
+ * Plain-old-data mirror of the module variables the hot path reads
+ * (parameters_2d, constitutive_2d, geometry_2d).  Filled by the host from the
+ * reference's namelists (read_param, src/inpout_2d.f90:957-4886); derived
+ * quantities (rho_a_amb, inv_pres, eps_sing, one_by_dx, ...) are computed by
+ * the library exactly as read_param / init_grid compute them.
+ *
+ * Supported model subset: gas carrier (GAS_FLAG=T), no liquid, no additional
+ * gas components, no vertical profiles, no stochastic or pore-pressure
+ * variables, rheology model 1 (Voellmy-Salm) or RHEOLOGY_FLAG=F, solver
+ * scheme "KT".  n_vars = 4 + n_solid (src/inpout_2d.f90:1429, 1852).
+ */
+typedef struct imexsw_params {
+  /* grid: geometry_2d comp_cells_x/y, x0, y0, cell_size */
+  int32_t nx, ny;
+  double x0, y0, cell_size;
+
+  /* layout / scheme: parameters_2d */
+  int32_t n_solid;          /* number of solid classes, 0..IMEXSW_MAX_SOLID */
+  int32_t n_rk;             /* IMEX RK stages N_RK (1..4) */
+
+  /* flags (LOGICALs as 0/1) */
+  int32_t energy_flag;      /* q4 = total energy (T) or h*rho*cp*T (F) */
+  int32_t alpha_flag;       /* qp carries alphas (T) or h*alphas (F) */
+  int32_t slope_correction_flag;
+  int32_t curvature_term_flag;
+  int32_t topo_change_flag;
+  int32_t radial_source_flag;
+  int32_t rheology_flag;
+  int32_t rheology_model;   /* 1 = Voellmy-Salm */
+  int32_t settling_flag;
+  int32_t entrainment_flag;
+  int32_t erodible_deposit_flag;
+
+  /* numerics: NUMERIC_PARAMETERS */
+  double cfl;               /* CFL (-1 => fixed max_dt) */
+  double max_dt;
+  double reconstr_coeff;
+  double theta;             /* generalized minmod parameter */
+  int32_t limiter[IMEXSW_MAX_VARS + 2];   /* limiter(1:n_vars+2) */
+
+  /* boundary conditions bc{W,E,S,N}(1:n_vars) */
+  int32_t bc_flag[4][IMEXSW_MAX_VARS];
+  double bc_value[4][IMEXSW_MAX_VARS];
+
+  /* physics: EXPL_TERMS / GAS_TRANSPORT / SOLID_TRANSPORT / RHEOLOGY */
+  double grav;
+  double pres;
+  double T_ambient;
+  double sp_heat_a;
+  double sp_gas_const_a;
+  double kin_visc_c;        /* carrier kinematic viscosity (= KIN_VISC_A) */
+  double rho_s[IMEXSW_MAX_SOLID];
+  double diam_s[IMEXSW_MAX_SOLID];
+  double sp_heat_s[IMEXSW_MAX_SOLID];
+  double mu;
+  double xi;
+
+  /* erosion / deposition */
+  double erosion_coeff;
+  double erodible_porosity;
+  double erodible_fract[IMEXSW_MAX_SOLID];
+  double T_erodible;
+  double alphastot_min;
+  double maximum_solid_packing;
+
+  /* radial source (RADIAL_SOURCE_PARAMETERS, h_source already derived from
+   * MFR_SOURCE by read_param, src/inpout_2d.f90:4379-4482) */
+  double x_source, y_source, r_source;
+  double h_source, vel_source, T_source;
+  double alphas_source[IMEXSW_MAX_SOLID];
+  double time_param[4];
+
+  /* hazard tables (THICKNESS_LEVELS0 / DYN_PRES_LEVELS0); n = 0 disables */
+  int32_t n_thickness_levels;
+  int32_t n_dyn_pres_levels;
+  double thickness_levels[IMEXSW_MAX_LEVELS];
+  double dyn_pres_levels[IMEXSW_MAX_LEVELS];
+} imexsw_params;
+
+/* Per-context counters (cumulative since create or the last reset). */
+typedef struct imexsw_stats {
+  int64_t steps;
+  int64_t cell_updates_active;  /* sum over steps of solve_cells */
+  int64_t cell_updates_grid;    /* nx*ny per step */
+  int64_t newton_cells;         /* cells that entered the Newton solve */
+  int64_t newton_iters;         /* total Newton iterations */
+  int64_t newton_nonconverged;  /* cells that hit max_nl_iter */
+  int64_t nan_events;           /* NaN trap, src/solver_2d.f90:1379-1394 */
+  int64_t neg_h_events;         /* h < -1e-7, src/solver_2d.f90:1405-1433 */
+  int64_t neg_solid_events;     /* solids <= -1e-7, src/solver_2d.f90:1439 */
+  int64_t sum_solid_events;     /* sum(solids) > h, src/solver_2d.f90:1569 */
+  int64_t low_T_events;         /* T < 273 K, src/solver_2d.f90:1538 */
+  int32_t solve_cells;          /* active cells of the last check_solve */
+  int32_t active_tiles;         /* active device tiles of the last check_solve */
+} imexsw_stats;
+
+typedef struct imexsw_ctx imexsw_ctx;
+
+/* Library version string. */
+const char *imexsw_version(void);
+
+/* Last error message of a context (or of the last failed create when ctx is
+ * NULL). */
+const char *imexsw_last_error(const imexsw_ctx *ctx);
+
+/* Allocate device state for one grid.  Replaces allocate_solver_variables
+ * (src/solver_2d.f90:263-529) together with the derived-constant setup of
+ * read_param / init_grid / init_problem_param (src/inpout_2d.f90:1330, 1353,
+ * 1852-1886; src/geometry_2d.f90:282-290; src/constitutive_2d.f90:469-518). */
+int imexsw_create(const imexsw_params *params, imexsw_ctx **ctx);
+
+/* Release everything.  Replaces deallocate_solver_variables
+ * (src/solver_2d.f90:543-657). */
+void imexsw_destroy(imexsw_ctx *ctx);
+
+/* Upload the topography B_cent(nx,ny) and B_nodata(nx,ny) (may be NULL) and
+ * derive every per-cell static field on the device the way
+ * topography_reconstruction does (src/geometry_2d.f90:535-860). */
+int imexsw_set_topography(imexsw_ctx *ctx, const double *B_cent,
+                          const uint8_t *B_nodata);
+
+/* Alternative to imexsw_set_topography for a host that already holds the
+ * geometry_2d fields: B_cent, B_prime_{x,y}_geom, B_second_{xx,xy,yy}_geom,
+ * grav_coeff, d_grav_coeff_dx (nx,ny); grav_coeff_stag_x (nx+1,ny);
+ * grav_coeff_stag_y (nx,ny+1); B_nodata (may be NULL).  Any double pointer
+ * except B_cent may be NULL (zero, or one for the grav_coeff family). */
+int imexsw_set_geometry(imexsw_ctx *ctx, const double *B_cent,
+                        const double *Bx_geom, const double *By_geom,
+                        const double *Bxx_geom, const double *Bxy_geom,
+                        const double *Byy_geom, const double *grav_coeff,
+                        const double *grav_stag_x, const double *grav_stag_y,
+                        const double *dgrav_dx, const uint8_t *B_nodata);
+
+/* Radial source ring: source_cell(nx,ny) in {0,1,2}, dir_bits(nx,ny) of
+ * IMEXSW_SRC_*, vect(8,nx,ny) = (E_x,E_y,W_x,W_y,S_x,S_y,N_x,N_y) as produced
+ * by init_source (src/geometry_2d.f90:872-1102). */
+int imexsw_set_source(imexsw_ctx *ctx, const int32_t *source_cell,
+                      const uint8_t *dir_bits, const double *vect);
+
+/* Upload the conservative state q(n_vars,nx,ny) (read_solution /
+ * init_2d write the module array q: src/inpout_2d.f90:5013,
+ * src/init_2d.f90:36, 89). */
+int imexsw_upload_q(imexsw_ctx *ctx, const double *q);
+
+/* Upload erodible(n_solid,nx,ny) and optionally deposit/erosion
+ * (nx,ny,n_solid) accumulators (may be NULL = keep). */
+int imexsw_upload_erosion(imexsw_ctx *ctx, const double *erodible,
+                          const double *deposit, const double *erosion);
+
+/* Active-cell mask: wet cells, source ring, N_RK-fold 4-neighbour dilation,
+ * source interior excluded.  Replaces check_solve(solve_all)
+ * (src/solver_2d.f90:666-857).  solve_cells (may be NULL) receives the count
+ * (copied back to the host: this call synchronises). */
+int imexsw_check_solve(imexsw_ctx *ctx, int solve_all, double t,
+                       int32_t *solve_cells);
+
+/* CFL time step over the active cells.  Replaces timestep
+ * (src/solver_2d.f90:872-1005).  Synchronises and returns dt on the host. */
+int imexsw_timestep(imexsw_ctx *ctx, double *dt);
+
+/* One IMEX Runge-Kutta step of size dt at time t.  Replaces imex_RK_solver
+ * (src/solver_2d.f90:1021-1632). */
+int imexsw_rk_step(imexsw_ctx *ctx, double t, double dt);
+
+/* Operator-split erosion/deposition/entrainment update.  Replaces
+ * update_erosion_deposition_cell (src/solver_2d.f90:2380-2595). */
+int imexsw_erosion_deposition(imexsw_ctx *ctx, double dt);
+
+/* Per-step topography update derivatives.  Replaces the per-step call of
+ * topography_reconstruction (src/IMEX_SfloW2D.f90:387,
+ * src/geometry_2d.f90:535-860). */
+int imexsw_topography_reconstruction(imexsw_ctx *ctx);
+
+/* Post-step diagnostics: qc->qp, running maxima of h, dynamic pressure and
+ * speed, hazard tables, and the per-step sums printed by the host
+ * (src/IMEX_SfloW2D.f90:391-483).  sums (may be NULL) receives
+ * {sum q1, sum qp1, count(q1 > 1e-7), sum of solid masses}, each still to be
+ * multiplied by dx*dy as the host prints them (synchronises when non-NULL). */
+int imexsw_post_step(imexsw_ctx *ctx, double sums[4]);
+
+/* Run nsteps of the reference's main loop entirely on the device, including
+ * the dt clipping and growth limiter of src/IMEX_SfloW2D.f90:357-381 (dt_old
+ * and dt_old_old are carried in the context, initialised from dt0 by the
+ * first call), with no host round trip per step.  t_io = current time
+ * (updated), t_end and t_output as in the host loop.  Stops early when t
+ * reaches t_end. */
+int imexsw_advance(imexsw_ctx *ctx, int32_t nsteps, double dt0, double *t_io,
+                   double t_end, double t_output, int32_t *steps_done);
+
+/* Download state.  Any pointer may be NULL.  q(n_vars,nx,ny),
+ * qp(n_vars+2,nx,ny), hmax/pdynmax/mod_vel_max(nx,ny),
+ * vuln(n_thk*n_pdyn,nx,ny) as 0/1 bytes. */
+int imexsw_download(imexsw_ctx *ctx, double *q, double *qp, double *hmax,
+                    double *pdynmax, double *mod_vel_max, uint8_t *vuln);
+
+/* Download erosion state: erodible(n_solid,nx,ny), deposit(nx,ny,n_solid),
+ * erosion(nx,ny,n_solid), B_cent(nx,ny).  Any pointer may be NULL. */
+int imexsw_download_erosion(imexsw_ctx *ctx, double *erodible, double *deposit,
+                            double *erosion, double *B_cent);
+
+/* Download solve_mask(nx,ny) as 0/1 bytes (src/solver_2d.f90:159). */
+int imexsw_download_mask(imexsw_ctx *ctx, uint8_t *mask);
+
+/* Last device dt (after clipping when produced by imexsw_advance). */
+int imexsw_get_dt(imexsw_ctx *ctx, double *dt);
+
+/* Counters.  reset != 0 zeroes them after reading. */
+int imexsw_stats_get(imexsw_ctx *ctx, imexsw_stats *stats, int reset);
+
+/* Per-kernel device time, measured with HIP events recorded on the
+ * context's stream around every launch.  names: mask, cfl, flux, local,
+ * topo, dtpolicy, post.  ms[i] accumulates milliseconds, calls[i] launches.
+ * Returns the number of kernels (<= cap).  Timing is enabled by
+ * imexsw_set_timing(ctx, 1). */
+int imexsw_set_timing(imexsw_ctx *ctx, int enable);
+int imexsw_kernel_times(imexsw_ctx *ctx, double *ms, int64_t *calls, int cap);
+const char *imexsw_kernel_name(int index);
+
+/* Wait for all work queued on the context's stream. */
+int imexsw_sync(imexsw_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* IMEXSW_H */

@@ -1,0 +1,219 @@
+"""ctypes wrapper of the C oracle (oracle/imexsw_oracle.c) -- TEST
+INFRASTRUCTURE ONLY: used as the checker in tests/ and smoke(), and timed as
+the CPU baseline in bench.py.  Never used by the product path."""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+from imex_sflow2d_v2_amd.params import ImexswParams
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "_build", "liboracle.so")
+_lib = None
+
+_D = C.POINTER(C.c_double)
+_U8 = C.POINTER(C.c_uint8)
+_I32 = C.POINTER(C.c_int32)
+_I64 = C.POINTER(C.c_int64)
+
+
+def build_oracle(force=False):
+    if force or not os.path.exists(LIB):
+        subprocess.check_call(["make", "-s", "-C", HERE] + (["-B"] if force else []))
+    return LIB
+
+
+def load_oracle():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB):
+        build_oracle()
+    lib = C.CDLL(LIB)
+    P = C.POINTER(ImexswParams)
+    sig = {
+        "orc_create": (C.c_void_p, [P]),
+        "orc_destroy": (None, [C.c_void_p]),
+        "orc_set_fresh": (None, [C.c_void_p, C.c_int]),
+        "orc_set_threads": (None, [C.c_int]),
+        "orc_n_vars": (C.c_int, [C.c_void_p]),
+        "orc_set_topography": (None, [C.c_void_p, _D, _U8]),
+        "orc_init_source": (None, [C.c_void_p]),
+        "orc_get_source": (None, [C.c_void_p, _I32, _U8, _D]),
+        "orc_get_geometry": (None, [C.c_void_p, _D, _D, _D, _D, _D, _D]),
+        "orc_set_q": (None, [C.c_void_p, _D]),
+        "orc_get_q": (None, [C.c_void_p, _D]),
+        "orc_get_qp": (None, [C.c_void_p, _D]),
+        "orc_set_erodible": (None, [C.c_void_p, _D]),
+        "orc_get_erosion": (None, [C.c_void_p, _D, _D, _D, _D]),
+        "orc_get_post": (None, [C.c_void_p, _D, _D, _D, _U8]),
+        "orc_get_mask": (None, [C.c_void_p, _U8]),
+        "orc_check_solve": (C.c_int, [C.c_void_p, C.c_int]),
+        "orc_timestep": (C.c_double, [C.c_void_p]),
+        "orc_imex_rk": (None, [C.c_void_p, C.c_double, C.c_double]),
+        "orc_erosion_deposition": (None, [C.c_void_p, C.c_double]),
+        "orc_topography_reconstruction": (None, [C.c_void_p]),
+        "orc_post_step": (None, [C.c_void_p, _D]),
+        "orc_init_post": (None, [C.c_void_p]),
+        "orc_run": (C.c_int, [C.c_void_p, C.c_int, C.c_double, _D, C.c_double,
+                              C.c_double, _D]),
+        "orc_counters": (None, [C.c_void_p, _I64]),
+        "orc_qc_to_qp": (None, [C.c_void_p, _D, _D, _D]),
+        "orc_qp_to_qc": (None, [C.c_void_p, _D, _D]),
+        "orc_mixt_var": (None, [C.c_void_p, _D, _D]),
+        "orc_limit": (C.c_double, [C.c_void_p, _D, _D, C.c_int]),
+        "orc_settling_velocity": (C.c_double, [C.c_void_p, C.c_double, C.c_double,
+                                               C.c_double, C.c_double]),
+        "orc_eval_fluxes": (None, [C.c_void_p, _D, _D, C.c_double, C.c_int, _D]),
+        "orc_local_speeds": (None, [C.c_void_p, _D, C.c_double, C.c_int, _D]),
+        "orc_implicit_terms": (None, [C.c_void_p, _D, _D]),
+        "orc_implicit_terms_cstep": (None, [C.c_void_p, _D, C.c_int, C.c_double, _D, _D]),
+        "orc_semi_impl_terms": (None, [C.c_void_p, _D, _D, C.c_double, _D]),
+        "orc_local_update": (C.c_int, [C.c_void_p, _D, _D, C.c_double, _D]),
+        "orc_source_bdry": (None, [C.c_void_p, C.c_double, C.c_double, C.c_double, _D]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = lib
+    return lib
+
+
+def _dp(a):
+    return a.ctypes.data_as(_D)
+
+
+def _f(a, dtype=np.float64):
+    return np.asfortranarray(a, dtype=dtype)
+
+
+class Oracle:
+    """One oracle context (mirrors the reference's module state)."""
+
+    def __init__(self, params, fresh=True, threads=0):
+        self.lib = load_oracle()
+        self.params = params.copy()
+        if threads:
+            self.lib.orc_set_threads(int(threads))
+        self.h = self.lib.orc_create(C.byref(self.params))
+        self.lib.orc_set_fresh(self.h, int(fresh))
+        self.nv = self.lib.orc_n_vars(self.h)
+        self.nx, self.ny = self.params.nx, self.params.ny
+        self.t = 0.0
+        self.dt_last = np.array([-1.0])
+
+    def close(self):
+        if self.h:
+            self.lib.orc_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---- set-up
+    def load_deck(self, deck, init=True):
+        B = _f(deck.B_cent)
+        self.lib.orc_set_topography(self.h, _dp(B), None)
+        self.lib.orc_init_source(self.h)
+        q = _f(deck.q)
+        self.lib.orc_set_q(self.h, _dp(q))
+        if deck.params.n_solid:
+            e = _f(deck.erodible)
+            self.lib.orc_set_erodible(self.h, _dp(e))
+        if init:
+            # src/IMEX_SfloW2D.f90:219-311
+            self.lib.orc_check_solve(self.h, 1)
+            self.lib.orc_topography_reconstruction(self.h)
+            self.lib.orc_init_post(self.h)
+        return self
+
+    # ---- accessors
+    def q(self):
+        a = np.zeros((self.nv, self.nx, self.ny), order="F")
+        self.lib.orc_get_q(self.h, _dp(a))
+        return a
+
+    def qp(self):
+        a = np.zeros((self.nv + 2, self.nx, self.ny), order="F")
+        self.lib.orc_get_qp(self.h, _dp(a))
+        return a
+
+    def set_q(self, q):
+        q = _f(q)
+        self.lib.orc_set_q(self.h, _dp(q))
+
+    def mask(self):
+        a = np.zeros((self.nx, self.ny), dtype=np.uint8, order="F")
+        self.lib.orc_get_mask(self.h, a.ctypes.data_as(_U8))
+        return a
+
+    def erosion_state(self):
+        ns = self.params.n_solid
+        er = np.zeros((ns, self.nx, self.ny), order="F")
+        dep = np.zeros((self.nx, self.ny, ns), order="F")
+        ers = np.zeros((self.nx, self.ny, ns), order="F")
+        B = np.zeros((self.nx, self.ny), order="F")
+        self.lib.orc_get_erosion(self.h, _dp(er), _dp(dep), _dp(ers), _dp(B))
+        return er, dep, ers, B
+
+    def post(self):
+        hm = np.zeros((self.nx, self.ny), order="F")
+        pd = np.zeros((self.nx, self.ny), order="F")
+        mv = np.zeros((self.nx, self.ny), order="F")
+        nl = self.params.n_thickness_levels * self.params.n_dyn_pres_levels
+        vu = np.zeros((max(nl, 1), self.nx, self.ny), dtype=np.uint8, order="F")
+        self.lib.orc_get_post(self.h, _dp(hm), _dp(pd), _dp(mv), vu.ctypes.data_as(_U8))
+        return hm, pd, mv, vu[:nl]
+
+    def geometry(self):
+        out = [np.zeros((self.nx, self.ny), order="F") for _ in range(6)]
+        self.lib.orc_get_geometry(self.h, *[_dp(a) for a in out])
+        return out
+
+    def source(self):
+        sc = np.zeros((self.nx, self.ny), dtype=np.int32, order="F")
+        bits = np.zeros((self.nx, self.ny), dtype=np.uint8, order="F")
+        v = np.zeros((8, self.nx, self.ny), order="F")
+        self.lib.orc_get_source(self.h, sc.ctypes.data_as(_I32),
+                                bits.ctypes.data_as(_U8), _dp(v))
+        return sc, bits, v
+
+    def counters(self):
+        c = np.zeros(10, dtype=np.int64)
+        self.lib.orc_counters(self.h, c.ctypes.data_as(_I64))
+        return dict(zip(["newton_cells", "newton_iters", "nonconverged",
+                         "stale_sides", "nan", "neg_h", "neg_solid",
+                         "sum_solid", "low_T", "active_cell_updates"], c.tolist()))
+
+    # ---- solver_2d procedures
+    def check_solve(self, solve_all=False):
+        return self.lib.orc_check_solve(self.h, int(solve_all))
+
+    def timestep(self):
+        return self.lib.orc_timestep(self.h)
+
+    def imex_rk(self, t, dt):
+        self.lib.orc_imex_rk(self.h, t, dt)
+
+    def erosion_deposition(self, dt):
+        self.lib.orc_erosion_deposition(self.h, dt)
+
+    def post_step(self):
+        s = np.zeros(4)
+        self.lib.orc_post_step(self.h, _dp(s))
+        return s
+
+    def run(self, nsteps, dt0=1e-4, t_end=1e30, t_output=None):
+        t = np.array([self.t])
+        if t_output is None:
+            t_output = t_end
+        n = self.lib.orc_run(self.h, int(nsteps), dt0, _dp(t), t_end, t_output,
+                             _dp(self.dt_last))
+        self.t = float(t[0])
+        return n
