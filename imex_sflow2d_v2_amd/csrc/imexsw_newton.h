@@ -373,7 +373,7 @@ __device__ __forceinline__ void mass_exchange(const DevParams& P, const double* 
     if ((al[i] > 0.0) && P.settling) {
       double sv = settling_velocity(P, P.diam_s[i], P.rho_s[i], m.rho_c, inv_kin_visc);
       double d = al[i] * sv * 1.0;
-      d = d * pow(1.0 - fmin(1.0, at / alpha_max), hind_exp);
+      d = d * det_pow(1.0 - fmin(1.0, at / alpha_max), hind_exp);
       dep[i] = fmin(d, r_h * al[i] / dt);
     }
   }
@@ -389,7 +389,7 @@ __device__ __forceinline__ void mass_exchange(const DevParams& P, const double* 
   double T_erodible = P.erod_dep ? r_T : P.T_erodible;
   double air = 0.0;
   if (P.entrain && (r_h > 0.0) && (m.ri > 0.0)) {
-    double ec = 0.075 / sqrt(1.0 + 718.0 * pow(m.ri, 2.4));
+    double ec = 0.075 / sqrt(1.0 + 718.0 * det_pow(m.ri, 2.4));
     air = ec * sqrt(mv2);
   }
   double dep_tot = 0.0, ers_tot = 0.0, rdep = 0.0, rers = 0.0, sdc = 0.0, sec = 0.0;

@@ -14,6 +14,7 @@
 #include <stdint.h>
 
 #include "../../include/imexsw.h"
+#include "imexsw_math.h"
 
 namespace imexsw {
 
@@ -88,7 +89,7 @@ __device__ __forceinline__ cpx csqrt_(cpx x) {
                    : 0.5 * sqrt(2.0 * fabs(x.im));
     return mk(r, copysign(r, x.im));
   }
-  double d = hypot(x.re, x.im), r, s;
+  double d = det_hypot(x.re, x.im), r, s;
   if (x.re > 0.0) {
     r = sqrt(0.5 * (d + x.re));
     s = 0.5 * (x.im / r);
@@ -556,7 +557,7 @@ __device__ __forceinline__ double settling_velocity(const DevParams& P, double d
   if (Rey <= 1000.0) {
     for (int i = 1; i <= 20; i++) {
       double old = sv;
-      double isc = sqrt(Rey / (24.0 * (1.0 + 0.15 * pow(Rey, 0.687))));
+      double isc = sqrt(Rey / (24.0 * (1.0 + 0.15 * det_pow(Rey, 0.687))));
       sv = cp * isc;
       if (fabs(old - sv) / old < 1.0e-6) return sv;
       Rey = diam * sv * inv_kin_visc;
@@ -583,21 +584,21 @@ __device__ __forceinline__ void source_bdry(const DevParams& P, double time,
     if (t_rem <= tp[1]) tc = 1.0;
   } else {
     if (t_rem < tp[2])
-      tc = 0.5 * (1.0 - cos(P.pi_g * t_rem / tp[2]));
+      tc = 0.5 * (1.0 - det_cos(P.pi_g * t_rem / tp[2]));
     else if (t_rem <= (tp[1] - tp[2]))
       tc = 1.0;
     else if (t_rem <= tp[1])
-      tc = 0.5 * (1.0 + cos(P.pi_g * ((t_rem - tp[1]) / tp[2] + 1.0)));
+      tc = 0.5 * (1.0 + det_cos(P.pi_g * ((t_rem - tp[1]) / tp[2] + 1.0)));
   }
   sb[0] = tc * P.h_source;
-  sb[1] = pow(tc, 1.5) * P.h_source * P.vel_source * vx;
-  sb[2] = pow(tc, 1.5) * P.h_source * P.vel_source * vy;
+  sb[1] = det_pow(tc, 1.5) * P.h_source * P.vel_source * vx;
+  sb[2] = det_pow(tc, 1.5) * P.h_source * P.vel_source * vy;
   sb[3] = P.T_source;
 #pragma unroll
   for (int i = 0; i < NS; i++)
     sb[4 + i] = P.alpha ? P.alphas_source[i] : tc * P.h_source * P.alphas_source[i];
-  sb[NV] = pow(tc, 0.5) * P.vel_source * vx;
-  sb[NV + 1] = pow(tc, 0.5) * P.vel_source * vy;
+  sb[NV] = det_pow(tc, 0.5) * P.vel_source * vx;
+  sb[NV + 1] = det_pow(tc, 0.5) * P.vel_source * vy;
 }
 
 }  // namespace imexsw

@@ -32,6 +32,128 @@
 #define MAXS IMEXSW_MAX_SOLID
 #define EPS_WP 2.220446049250313e-16 /* EPSILON(1.0_wp) */
 
+
+/* ------------------------------------------------------------------------ */
+/* elementary functions with a fixed operation sequence                     */
+/* The reference calls libm pow, cos and (inside csqrt) hypot.  Vendor      */
+/* libraries differ in the last bits between CPU and GPU; both the oracle   */
+/* and the device implement these with the same IEEE operation sequence so */
+/* that device and oracle can agree bit for bit.  Methods: Cody & Waite     */
+/* (1980) argument reduction (ln2 and pi/2 split), log(m) = 2 atanh(s),     */
+/* s = (m-1)/(m+1), degree-14 Taylor exp, Taylor sin/cos on |r| <= pi/4.    */
+/* They are within a few ulp of the correctly rounded values over the       */
+/* ranges used, i.e. 1e-15 relative to libm.                                */
+/* ------------------------------------------------------------------------ */
+static inline int64_t d2i(double x) { int64_t i; memcpy(&i, &x, 8); return i; }
+static inline double i2d(int64_t i) { double x; memcpy(&x, &i, 8); return x; }
+
+static double det_log(double x) {
+  int64_t bits = d2i(x);
+  int e = (int)((bits >> 52) & 0x7ff);
+  if (e == 0) {
+    x = x * 18014398509481984.0;
+    bits = d2i(x);
+    e = (int)((bits >> 52) & 0x7ff) - 54;
+  }
+  e -= 1022;
+  double m = i2d((bits & 0x000fffffffffffffLL) | 0x3fe0000000000000LL);
+  if (m < 0.70710678118654752440) {
+    m = m * 2.0;
+    e -= 1;
+  }
+  const double s = (m - 1.0) / (m + 1.0);
+  const double s2 = s * s;
+  double p = 1.0 / 23.0;
+  p = p * s2 + 1.0 / 21.0;
+  p = p * s2 + 1.0 / 19.0;
+  p = p * s2 + 1.0 / 17.0;
+  p = p * s2 + 1.0 / 15.0;
+  p = p * s2 + 1.0 / 13.0;
+  p = p * s2 + 1.0 / 11.0;
+  p = p * s2 + 1.0 / 9.0;
+  p = p * s2 + 1.0 / 7.0;
+  p = p * s2 + 1.0 / 5.0;
+  p = p * s2 + 1.0 / 3.0;
+  const double logm = 2.0 * s + 2.0 * s * (s2 * p);
+  const double de = (double)e;
+  return (de * 6.93147180369123816490e-01 + logm) + de * 1.90821492927058770002e-10;
+}
+
+static double det_exp(double y) {
+  if (y > 709.0) return INFINITY;
+  if (y < -745.0) return 0.0;
+  const double n = rint(y * 1.44269504088896338700e+00);
+  const double r = (y - n * 6.93147180369123816490e-01) - n * 1.90821492927058770002e-10;
+  double p = 1.0 / 87178291200.0;
+  p = p * r + 1.0 / 6227020800.0;
+  p = p * r + 1.0 / 479001600.0;
+  p = p * r + 1.0 / 39916800.0;
+  p = p * r + 1.0 / 3628800.0;
+  p = p * r + 1.0 / 362880.0;
+  p = p * r + 1.0 / 40320.0;
+  p = p * r + 1.0 / 5040.0;
+  p = p * r + 1.0 / 720.0;
+  p = p * r + 1.0 / 120.0;
+  p = p * r + 1.0 / 24.0;
+  p = p * r + 1.0 / 6.0;
+  p = p * r + 0.5;
+  p = p * r + 1.0;
+  p = p * r + 1.0;
+  const int ni = (int)n;
+  const int n1 = ni / 2, n2 = ni - n1;
+  const double s1 = i2d((int64_t)(1023 + n1) << 52);
+  const double s2 = i2d((int64_t)(1023 + n2) << 52);
+  return (p * s1) * s2;
+}
+
+static double det_pow(double x, double y) {
+  if (x == 0.0) return (y > 0.0) ? 0.0 : (y == 0.0 ? 1.0 : INFINITY);
+  if (x == 1.0 || y == 0.0) return 1.0;
+  if (x != x || y != y) return x + y;
+  return det_exp(y * det_log(x));
+}
+
+static double det_cos(double x) {
+  const double P1 = 1.57079632673412561417e+00, P2 = 6.07710050650619224932e-11,
+               P3 = 2.02226624879595063154e-21;
+  const double k = rint(x * 6.36619772367581382433e-01);
+  const double r = ((x - k * P1) - k * P2) - k * P3;
+  const double r2 = r * r;
+  double c = -1.0 / 87178291200.0;
+  c = c * r2 + 1.0 / 479001600.0;
+  c = c * r2 - 1.0 / 3628800.0;
+  c = c * r2 + 1.0 / 40320.0;
+  c = c * r2 - 1.0 / 720.0;
+  c = c * r2 + 1.0 / 24.0;
+  c = c * r2 - 0.5;
+  c = c * r2 + 1.0;
+  double s = -1.0 / 1307674368000.0;
+  s = s * r2 + 1.0 / 6227020800.0;
+  s = s * r2 - 1.0 / 39916800.0;
+  s = s * r2 + 1.0 / 362880.0;
+  s = s * r2 - 1.0 / 5040.0;
+  s = s * r2 + 1.0 / 120.0;
+  s = s * r2 - 1.0 / 6.0;
+  s = r + r * (r2 * s);
+  const int q = ((int)k) & 3;
+  return q == 0 ? c : q == 1 ? -s : q == 2 ? -c : s;
+}
+
+static double det_hypot(double x, double y) {
+  double a = fabs(x), b = fabs(y);
+  if (a < b) {
+    double t = a;
+    a = b;
+    b = t;
+  }
+  if (a == 0.0)
+    return 0.0;
+  const double r = b / a;
+  return a * sqrt(1.0 + r * r);
+}
+double orc_det_pow(double x, double y) { return det_pow(x, y); }
+double orc_det_cos(double x) { return det_cos(x); }
+
 /* ------------------------------------------------------------------------ */
 /* complex numbers with gfortran semantics                                  */
 /* ------------------------------------------------------------------------ */
@@ -83,7 +205,7 @@ static inline cx csqrt_f(cx x) {
       r = 0.5 * sqrt(2.0 * fabs(x.im));
     return C(r, copysign(r, x.im));
   }
-  double d = hypot(x.re, x.im), r, s;
+  double d = det_hypot(x.re, x.im), r, s;
   if (x.re > 0.0) {
     r = sqrt(0.5 * (d + x.re));
     s = 0.5 * (x.im / r);
@@ -724,7 +846,7 @@ static double settling_velocity(const orc_t *o, double diam, double rhos,
   if (Rey <= 1000.0) {
     for (int i = 1; i <= 20; i++) {
       double old = sv;
-      inv_sqrt_C_D = sqrt(Rey / (24.0 * (1.0 + 0.15 * pow(Rey, 0.687))));
+      inv_sqrt_C_D = sqrt(Rey / (24.0 * (1.0 + 0.15 * det_pow(Rey, 0.687))));
       sv = const_part * inv_sqrt_C_D;
       if (fabs(old - sv) / old < 1.0e-6)
         return sv;
@@ -799,7 +921,7 @@ static void mass_exchange_terms(orc_t *o, const double *qpj, double Bx,
       deposition_term[i] = r_alphas[i] * sv * 1.0;
       deposition_term[i] =
           deposition_term[i] *
-          pow(1.0 - fmin(1.0, alphas_tot / alpha_max), hind_exp);
+          det_pow(1.0 - fmin(1.0, alphas_tot / alpha_max), hind_exp);
       deposition_term[i] = fmin(deposition_term[i], r_h * r_alphas[i] / dt);
     }
   }
@@ -821,7 +943,7 @@ static void mass_exchange_terms(orc_t *o, const double *qpj, double Bx,
     T_erodible = r_T;
   double air_entr;
   if (P->entrainment_flag && (r_h > 0.0) && (ri > 0.0)) {
-    double entr_coeff = 0.075 / sqrt(1.0 + 718.0 * pow(ri, 2.4));
+    double entr_coeff = 0.075 / sqrt(1.0 + 718.0 * det_pow(ri, 2.4));
     air_entr = entr_coeff * sqrt(mod_vel2);
   } else {
     air_entr = 0.0;
@@ -884,21 +1006,21 @@ static void source_bdry(const orc_t *o, double time, double vx, double vy,
       t_coeff = 1.0;
   } else {
     if (t_rem < tp[2])
-      t_coeff = 0.5 * (1.0 - cos(o->pi_g * t_rem / tp[2]));
+      t_coeff = 0.5 * (1.0 - det_cos(o->pi_g * t_rem / tp[2]));
     else if (t_rem <= (tp[1] - tp[2]))
       t_coeff = 1.0;
     else if (t_rem <= tp[1])
-      t_coeff = 0.5 * (1.0 + cos(o->pi_g * ((t_rem - tp[1]) / tp[2] + 1.0)));
+      t_coeff = 0.5 * (1.0 + det_cos(o->pi_g * ((t_rem - tp[1]) / tp[2] + 1.0)));
   }
   sb[0] = t_coeff * P->h_source;
-  sb[1] = pow(t_coeff, 1.5) * P->h_source * P->vel_source * vx;
-  sb[2] = pow(t_coeff, 1.5) * P->h_source * P->vel_source * vy;
+  sb[1] = det_pow(t_coeff, 1.5) * P->h_source * P->vel_source * vx;
+  sb[2] = det_pow(t_coeff, 1.5) * P->h_source * P->vel_source * vy;
   sb[3] = P->T_source;
   for (int i = 0; i < ns; i++)
     sb[4 + i] = P->alpha_flag ? P->alphas_source[i]
                               : t_coeff * P->h_source * P->alphas_source[i];
-  sb[o->idx_u - 1] = pow(t_coeff, 0.5) * P->vel_source * vx;
-  sb[o->idx_v - 1] = pow(t_coeff, 0.5) * P->vel_source * vy;
+  sb[o->idx_u - 1] = det_pow(t_coeff, 0.5) * P->vel_source * vx;
+  sb[o->idx_v - 1] = det_pow(t_coeff, 0.5) * P->vel_source * vy;
 }
 void orc_source_bdry(const orc_t *o, double time, double vx, double vy,
                      double *out) {

@@ -93,6 +93,8 @@ void orc_semi_impl_terms(const orc_t *o, const double *qc, const double *qp,
  * residual D = divFlux - expl_terms: returns Newton iterations */
 int orc_local_update(const orc_t *o, const double *q0, const double *D,
                      double dt, double *q_out);
+double orc_det_pow(double x, double y);
+double orc_det_cos(double x);
 void orc_source_bdry(const orc_t *o, double time, double vx, double vy,
                      double *out);
 
