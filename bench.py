@@ -1,0 +1,251 @@
+"""Throughput of the IMEX_SfloW2D per-timestep path on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload volcano]
+
+One step = one full timestep of the 2048 x 2048 grid (check_solve mask,
+stencil pass with the CFL reduction, dt policy, stage-2 Newton and assembly,
+erosion/deposition, post-step diagnostics, topography update) with the state
+resident in HBM.  The default workload is the synthetic volcano: 2048^2
+cells, dx = 5 m, cone + 32 bumps + fractal noise, radial source 1e6 kg/s,
+three solid classes, erosion/deposition with topography change, 1000
+timesteps from t = 0.  ``--workload wet`` runs the fully wet 2048^2 incline
+variant (every cell active) that exposes the stencil and Newton kernels.
+
+Printed: one JSON line.  ``value`` is grid cell-updates/s (nx*ny*steps /
+wall time of the timed steps); the active variant (sum of solve_cells /
+time) is in ``config``.  ``roofline`` prices the dominant kernel's
+algorithmic HBM bytes per launch against 8 TB/s using the kernel's average
+duration measured with HIP events on the library's stream; ``traffic`` comes
+from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary of that kernel
+when one exists.  ``cpu_baseline`` times the C oracle (the reference
+algorithm restated, OpenMP over cells, reference interface semantics) on the
+host cores over the first steps of the same workload.
+"""
+import argparse
+import json
+import os
+import re
+import sys
+import time
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+
+HBM_PEAK_GBS = 8000.0
+
+# Algorithmic HBM bytes per processed cell of each kernel (doubles unless
+# stated): what the kernel has to read and write at least once.
+#   stencil: q (NV) + B'x, B'y + mask byte in, D (NV) out, plus the slope
+#            correction / curvature fields when they are on
+#   local:   q, D, B'x, B'y, mask in; q out; erodible/deposit/erosion
+#            (read+write, NS each) when erosion or settling is on; B_cent
+#            read+write when the topography changes; hmax, pdynmax,
+#            mod_vel_max, vulnerability word read+write
+#   mask:    h plane in, mask byte out (every grid cell)
+#   topo:    B_cent in, B'x, B'y out (cells near the flow)
+
+
+def kernel_bytes(p, name):
+    ns = p.n_solid
+    nv = 4 + ns
+    if name == "stencil":
+        b = 8 * nv + 16 + 1 + 8 * nv
+        if p.slope_correction_flag:
+            b += 8 * 4
+        if p.curvature_term_flag:
+            b += 8 * 3
+        return b
+    if name == "local":
+        b = 8 * nv * 2 + 16 + 1 + 8 * nv
+        if p.erosion_coeff > 0.0 or p.settling_flag:
+            b += 3 * ns * 16
+        if p.topo_change_flag:
+            b += 16
+        b += 3 * 16 + 8
+        return b
+    if name == "mask":
+        return 9
+    if name == "topo":
+        return 24
+    return 0
+
+
+def step_bytes(p):
+    """Bytes per active cell-update of the whole step (stencil + local)."""
+    return kernel_bytes(p, "stencil") + kernel_bytes(p, "local")
+
+
+def parse_traffic(kernel, workload):
+    """HBM bytes per dispatch of `kernel` from the committed rocprofv3
+    counter summary (FETCH_SIZE and WRITE_SIZE in KiB per dispatch, scaled by
+    the copy-kernel calibration factor recorded in the same file)."""
+    path = os.path.join(HERE, "profiles", "%s_%s_traffic.txt" % (kernel, workload))
+    if not os.path.exists(path):
+        return None
+    vals = {}
+    with open(path) as f:
+        for line in f:
+            m = re.match(r"\s*(fetch_kib_per_dispatch|write_kib_per_dispatch|"
+                         r"calibration_factor)\s*[:=]\s*([0-9.eE+-]+)", line)
+            if m:
+                vals[m.group(1)] = float(m.group(2))
+    if "fetch_kib_per_dispatch" not in vals or "write_kib_per_dispatch" not in vals:
+        return None
+    k = vals.get("calibration_factor", 1.0)
+    return (vals["fetch_kib_per_dispatch"] + vals["write_kib_per_dispatch"]) * 1024.0 * k
+
+
+def make_deck(workload):
+    from imex_sflow2d_v2_amd import decks
+    if workload == "volcano":
+        return decks.volcano_2048()
+    if workload == "wet":
+        return decks.wet_2048()
+    raise SystemExit("unknown workload %r" % workload)
+
+
+def cpu_baseline(workload, steps, threads):
+    """Time the oracle (reference interface semantics) on the host cores
+    over the first `steps` timesteps of the workload."""
+    from oracle.pyoracle import Oracle
+    d = make_deck(workload)
+    o = Oracle(d.params, fresh=False, threads=threads).load_deck(d)
+    t0 = time.perf_counter()
+    n = o.run(steps, dt0=d.dt0)
+    el = time.perf_counter() - t0
+    c = o.counters()
+    o.close()
+    cells = d.params.nx * d.params.ny
+    return {
+        "value": cells * n / el,
+        "unit": "cell-updates/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": "first %d timesteps of the %s workload (%dx%d) from t=0 on %d "
+                  "host threads, %.2f s wall; active cell-updates/s %.4g"
+                  % (n, workload, d.params.nx, d.params.ny, threads, el,
+                     c["active_cell_updates"] / el),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--workload", default="volcano", choices=["volcano", "wet"])
+    ap.add_argument("--cpu-steps", type=int, default=30)
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="run one device-to-device copy of 512 MiB and exit: the "
+                         "reference kernel for scaling rocprofv3 FETCH_SIZE / "
+                         "WRITE_SIZE to bytes")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build runs on one GPU; --gpus %d is not supported "
+              "(the multi-GPU decomposition is a design only)" % args.gpus,
+              file=sys.stderr)
+        return 2
+
+    import torch
+    from imex_sflow2d_v2_amd.solver import Solver
+
+    if not torch.cuda.is_available():
+        print("bench.py: no GPU visible", file=sys.stderr)
+        return 1
+    torch.cuda.init()
+    if args.copy_probe:
+        n = 1 << 26  # 2^26 doubles = 512 MiB read + 512 MiB written
+        a = torch.ones(n, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes_read": 8 * n, "copy_probe_bytes_written": 8 * n}))
+        return 0
+
+    d = make_deck(args.workload)
+    p = d.params
+    cells = p.nx * p.ny
+
+    # warm-up on its own context (code objects, caches), then a fresh
+    # context timed from t = 0
+    if args.warmup > 0:
+        w = Solver(p).load_deck(d)
+        w.advance(args.warmup, dt0=d.dt0)
+        w.sync()
+        w.close()
+    g = Solver(p).load_deck(d)
+    g.sync()
+    g.set_timing(True)
+    g.stats(reset=True)
+    torch.cuda.synchronize()
+    g.sync()
+    t0 = time.perf_counter()
+    n = g.advance(args.steps, dt0=d.dt0)
+    g.sync()
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    st = g.stats()
+    kt = g.kernel_times()
+    g.set_timing(False)
+
+    steps = max(n, 1)
+    active = st["cell_updates_active"]
+    dom = max(((k, v) for k, v in kt.items() if v[1] > 0), key=lambda kv: kv[1][0])
+    dname, (dms, dcalls) = dom
+    avg_s = dms / dcalls * 1e-3
+    units = active / steps if dname in ("stencil", "local") else cells
+    alg_bytes = kernel_bytes(p, dname) * units
+    achieved = alg_bytes / avg_s / 1e9
+    traffic = parse_traffic(dname, args.workload)
+
+    out = {
+        "metric": "FP64 cell-updates/s at 2048^2",
+        "value": cells * steps / el,
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": args.warmup,
+        "ms_per_step": el / steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": "%s_%dx%d" % (args.workload, p.nx, p.ny),
+            "nx": p.nx, "ny": p.ny, "n_solid": p.n_solid, "n_rk": p.n_rk,
+            "dx": p.cell_size,
+            "active_cell_updates_per_s": active / el,
+            "active_fraction": active / float(cells * steps),
+            "newton_iters_per_cell": st["newton_iters"] / max(1, st["newton_cells"]),
+            "simulated_time_s": g.t,
+            "kernel_ms_per_step": {k: v[0] / steps for k, v in kt.items() if v[1]},
+            "step_bytes_per_active_cell": step_bytes(p),
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": dname,
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": traffic,
+            "bytes_per_unit": kernel_bytes(p, dname),
+            "units_per_launch": units,
+            "avg_launch_ms": avg_s * 1e3,
+        },
+        "cpu_baseline": None,
+    }
+    g.close()
+    if not args.no_cpu:
+        threads = args.cpu_threads or min(16, len(os.sched_getaffinity(0)))
+        out["cpu_baseline"] = cpu_baseline(args.workload, args.cpu_steps, threads)
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

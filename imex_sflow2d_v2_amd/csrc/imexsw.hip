@@ -37,8 +37,10 @@ struct imexsw_ctx {
   size_t plane = 0;
   double* stage = nullptr;  // layout-conversion staging
   size_t stage_bytes = 0;
-  dim3 grid_tiles, grid_strips, grid_stag;
-  int strip_h = 32;
+  dim3 grid_tiles, grid_mask, grid_strips, grid_stag;
+  int strip_h = 32;         // tallest stencil strip (rows)
+  int row_unit = 4;         // stencil grid granularity (rows)
+  int target_strips = 2048; // strips wanted in flight
   bool d_valid = false;   // D holds the residual of the current q
   bool dt_hist = false;   // dt_old / dt_old_old initialised
   bool timing = false;
@@ -133,10 +135,10 @@ template <int NS>
 void launch_stencil(imexsw_ctx* c, int compute_dt) {
   if (c->P.ny > 1)
     hipLaunchKernelGGL((k_stencil<NS, true>), c->grid_strips, dim3(64), 0, c->stream, c->P,
-                       c->D, c->strip_h, compute_dt);
+                       c->D, c->row_unit, c->strip_h, c->target_strips, compute_dt);
   else
     hipLaunchKernelGGL((k_stencil<NS, false>), c->grid_strips, dim3(64), 0, c->stream, c->P,
-                       c->D, c->strip_h, compute_dt);
+                       c->D, c->row_unit, c->strip_h, c->target_strips, compute_dt);
 }
 template <int NS>
 void launch_local(imexsw_ctx* c, int ops) {
@@ -182,7 +184,7 @@ int run_local(imexsw_ctx* c, int ops, int kid) {
 int run_mask(imexsw_ctx* c, int solve_all) {
   {
     Timed tm(c, K_MASK);
-    hipLaunchKernelGGL(k_mask, c->grid_tiles, dim3(256), 0, c->stream, c->P, c->D, solve_all);
+    hipLaunchKernelGGL(k_mask, c->grid_mask, dim3(256), 0, c->stream, c->P, c->D, solve_all);
   }
   return check_launch(c, "k_mask");
 }
@@ -331,6 +333,8 @@ int imexsw_create(const imexsw_params* params, imexsw_ctx** out) {
       P.bc_flag[s][i] = p.bc_flag[s][i];
       P.bc_value[s][i] = p.bc_value[s][i];
     }
+  for (int s = 0; s < 4; s++)
+    P.bc_quiet[s] = (p.bc_flag[s][0] == 2 || p.bc_value[s][0] == 0.0) ? 1 : 0;
   P.grav = p.grav;
   P.pres = p.pres;
   P.inv_pres = 1.0 / p.pres;  // src/inpout_2d.f90:1330
@@ -408,12 +412,14 @@ int imexsw_create(const imexsw_params* params, imexsw_ctx** out) {
   if (P.n_thk * P.n_pdyn > 0) rc |= dalloc(c, &D.vuln, pl);
   rc |= dalloc(c, &D.dt_bits, 1);
   rc |= dalloc(c, &D.scal, S_NSLOTS);
-  rc |= dalloc(c, &D.counters, C_NCOUNTERS);
+  rc |= dalloc(c, &D.counters, C_NCOUNTERS * kSlots);
   c->grid_tiles = dim3((P.nx + kTileX - 1) / kTileX, (P.ny + kTileY - 1) / kTileY);
+  c->grid_mask = dim3((P.nx + kMaskTX - 1) / kMaskTX, (P.ny + kMaskTY - 1) / kMaskTY);
   c->grid_stag = dim3((P.nx + 1 + kTileX - 1) / kTileX, (P.ny + 1 + kTileY - 1) / kTileY);
   rc |= dalloc(c, &D.tile_sums, 4 * (size_t)c->grid_tiles.x * c->grid_tiles.y);
   c->strip_h = (P.ny > 1) ? 32 : 1;
-  c->grid_strips = dim3((P.nx + kStripW - 1) / kStripW, (P.ny + c->strip_h - 1) / c->strip_h);
+  c->row_unit = (P.ny > 1) ? 4 : 1;
+  c->grid_strips = dim3((P.nx + kStripW - 1) / kStripW, (P.ny + c->row_unit - 1) / c->row_unit);
   if (rc) {
     g_create_error = c->err;
     imexsw_destroy(c);
@@ -541,13 +547,16 @@ int imexsw_check_solve(imexsw_ctx* c, int solve_all, double t, int32_t* solve_ce
   if (!c) return fail(c, "null argument");
   if (set_scalar(c, S_T, t)) return -1;
   if (set_scalar(c, S_DONE, 0.0)) return -1;
-  HIPCHK(c, hipMemsetAsync(c->D.counters + C_SOLVE_CELLS, 0, sizeof(unsigned long long), c->stream));
+  HIPCHK(c, hipMemsetAsync(c->D.counters + C_SOLVE_CELLS * kSlots, 0,
+                           kSlots * sizeof(unsigned long long), c->stream));
   if (run_mask(c, solve_all)) return -1;
   c->d_valid = false;
   if (solve_cells) {
-    unsigned long long n = 0;
-    HIPCHK(c, hipMemcpyAsync(&n, c->D.counters + C_SOLVE_CELLS, sizeof n, hipMemcpyDeviceToHost, c->stream));
+    unsigned long long v[kSlots], n = 0;
+    HIPCHK(c, hipMemcpyAsync(v, c->D.counters + C_SOLVE_CELLS * kSlots, sizeof v,
+                             hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (int i = 0; i < kSlots; i++) n += v[i];
     *solve_cells = (int32_t)n;
   }
   return 0;
@@ -630,7 +639,7 @@ int imexsw_advance(imexsw_ctx* c, int32_t nsteps, double dt0, double* t_io, doub
   }
   HIPCHK(c, hipMemcpyAsync(c->D.scal, s, sizeof s, hipMemcpyHostToDevice, c->stream));
   unsigned long long steps0 = 0;
-  HIPCHK(c, hipMemcpyAsync(&steps0, c->D.counters + C_STEPS, sizeof steps0, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipMemcpyAsync(&steps0, c->D.counters + C_STEPS * kSlots, sizeof steps0, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   const int ops = OP_RK | OP_EROSION | OP_POST;
   for (int n = 0; n < nsteps; n++) {
@@ -650,7 +659,7 @@ int imexsw_advance(imexsw_ctx* c, int32_t nsteps, double dt0, double* t_io, doub
   if (get_scalars(c, s)) return -1;
   *t_io = s[S_T];
   unsigned long long steps1 = 0;
-  HIPCHK(c, hipMemcpyAsync(&steps1, c->D.counters + C_STEPS, sizeof steps1, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipMemcpyAsync(&steps1, c->D.counters + C_STEPS * kSlots, sizeof steps1, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   if (steps_done) *steps_done = (int32_t)(steps1 - steps0);
   return 0;
@@ -702,7 +711,10 @@ int imexsw_download_erosion(imexsw_ctx* c, double* erodible, double* deposit, do
 
 int imexsw_download_mask(imexsw_ctx* c, uint8_t* mask) {
   if (!c || !mask) return fail(c, "null argument");
-  return download_plane(c, c->D.mask, mask, c->P.nx, c->P.ny, c->P.pitch);
+  if (download_plane(c, c->D.mask, mask, c->P.nx, c->P.ny, c->P.pitch)) return -1;
+  const size_t n = (size_t)c->P.nx * c->P.ny;
+  for (size_t i = 0; i < n; i++) mask[i] &= M_ACTIVE;  // solve_mask only
+  return 0;
 }
 
 int imexsw_get_dt(imexsw_ctx* c, double* dt) {
@@ -715,9 +727,15 @@ int imexsw_get_dt(imexsw_ctx* c, double* dt) {
 
 int imexsw_stats_get(imexsw_ctx* c, imexsw_stats* st, int reset) {
   if (!c || !st) return fail(c, "null argument");
-  unsigned long long v[C_NCOUNTERS];
-  HIPCHK(c, hipMemcpyAsync(v, c->D.counters, sizeof v, hipMemcpyDeviceToHost, c->stream));
+  std::vector<unsigned long long> raw((size_t)C_NCOUNTERS * kSlots);
+  HIPCHK(c, hipMemcpyAsync(raw.data(), c->D.counters, raw.size() * sizeof(unsigned long long),
+                           hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
+  unsigned long long v[C_NCOUNTERS];
+  for (int m = 0; m < C_NCOUNTERS; m++) {
+    v[m] = 0;
+    for (int i = 0; i < kSlots; i++) v[m] += raw[(size_t)m * kSlots + i];
+  }
   std::memset(st, 0, sizeof *st);
   st->steps = (int64_t)v[C_STEPS];
   st->cell_updates_active = (int64_t)v[C_ACTIVE_UPD];
@@ -733,10 +751,11 @@ int imexsw_stats_get(imexsw_ctx* c, imexsw_stats* st, int reset) {
   st->solve_cells = (int32_t)v[C_SOLVE_CELLS];
   st->active_tiles = 0;
   if (reset) {
-    unsigned long long keep = v[C_SOLVE_CELLS];
-    std::memset(v, 0, sizeof v);
-    v[C_SOLVE_CELLS] = keep;
-    HIPCHK(c, hipMemcpyAsync(c->D.counters, v, sizeof v, hipMemcpyHostToDevice, c->stream));
+    // keep the current solve_cells slots, clear everything else
+    for (size_t i = 0; i < raw.size(); i++)
+      if (i / kSlots != C_SOLVE_CELLS) raw[i] = 0;
+    HIPCHK(c, hipMemcpyAsync(c->D.counters, raw.data(), raw.size() * sizeof(unsigned long long),
+                             hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
   }
   return 0;
@@ -768,6 +787,17 @@ int imexsw_kernel_times(imexsw_ctx* c, double* ms, int64_t* calls, int cap) {
 
 const char* imexsw_kernel_name(int index) {
   return (index >= 0 && index < K_NK) ? kKernelNames[index] : "";
+}
+
+int imexsw_set_tuning(imexsw_ctx* c, int max_strip_rows, int target_strips) {
+  if (!c) return fail(c, "null argument");
+  const int m = max_strip_rows;
+  if (!(m == 4 || m == 8 || m == 16 || m == 32 || m == 64))
+    return fail(c, "max_strip_rows must be 4, 8, 16, 32 or 64 (got %d)", m);
+  if (target_strips < 1) return fail(c, "target_strips must be >= 1 (got %d)", target_strips);
+  if (c->P.ny > 1) c->strip_h = m;
+  c->target_strips = target_strips;
+  return 0;
 }
 
 int imexsw_sync(imexsw_ctx* c) {

@@ -64,6 +64,23 @@ enum ScalarSlot { S_T = 0, S_DT = 1, S_DT_OLD = 2, S_DT_OLD_OLD = 3, S_T_END = 4
 enum CounterSlot { C_SOLVE_CELLS = 0, C_NEWTON_CELLS, C_NEWTON_ITERS, C_NONCONV,
                    C_NAN, C_NEGH, C_NEGS, C_SUMS, C_LOWT, C_STEPS, C_ACTIVE_UPD,
                    C_NCOUNTERS };
+// Every counter has kSlots copies, counters[m * kSlots + slot]; workgroups
+// add into slot (workgroup id mod kSlots) so that thousands of workgroups do
+// not serialise on one L2 line.  Totals are the sum over the slots.
+constexpr int kSlots = 64;
+__device__ __forceinline__ unsigned long long* ctr(const DevPtrs& D, int m, int slot) {
+  return D.counters + m * kSlots + (slot & (kSlots - 1));
+}
+
+// mask byte: bit 0 = check_solve's active flag (solve_mask), bit 1 = the
+// cell's update has to be computed.  An active cell whose update is known
+// to leave it unchanged is "quiet": it and every cell of its reconstruction
+// stencil (two cells along x and along y) hold h == 0, no source-ring cell
+// is in that stencil, its own state is all zeros and no boundary condition
+// within reach can feed mass in.  Its faces then carry dry states (zero
+// flux, zero wave speed) and the stage update, erosion/deposition and
+// diagnostics return the state and accumulators as they are.
+enum MaskBits { M_ACTIVE = 1, M_COMPUTE = 2 };
 
 __device__ __forceinline__ size_t cidx(const DevParams& P, int j, int k) {
   return (size_t)k * P.pitch + j;
@@ -93,45 +110,83 @@ __device__ __forceinline__ double shfl_d(double v, int src) {
 }
 
 // ---------------------------------------------------------------------------
-// k_mask: check_solve (src/solver_2d.f90:666-857)
+// k_mask: check_solve (src/solver_2d.f90:666-857).  One workgroup per
+// 64 x 16 tile; the wet/base flags of the tile and a halo of two cells are
+// staged in LDS, then the N_RK-fold 4-neighbour dilation (an L1 diamond of
+// radius N_RK) and the quiet test read LDS only.
 // ---------------------------------------------------------------------------
+constexpr int kMaskTX = 64, kMaskTY = 16, kMaskH = 2;
+
 __global__ void __launch_bounds__(256) k_mask(DevParams P, DevPtrs D, int solve_all) {
-  const int j = blockIdx.x * kTileX + (threadIdx.x & 63);
-  const int k = blockIdx.y * kTileY + (threadIdx.x >> 6);
-  int active = 0;
-  if (j < P.nx && k < P.ny) {
+  constexpr int LW = kMaskTX + 2 * kMaskH, LH = kMaskTY + 2 * kMaskH;
+  __shared__ uint8_t base[LH][LW];  // seeds of the dilation
+  __shared__ uint8_t wet[LH][LW];   // h != 0 or a source cell
+  const int j0 = blockIdx.x * kMaskTX, k0 = blockIdx.y * kMaskTY;
+  for (int e = threadIdx.x; e < LW * LH; e += 256) {
+    const int r = e / LW, cc = e - r * LW;
+    const int jj = j0 - kMaskH + cc, kk = k0 - kMaskH + r;
+    uint8_t b = 0, w = 0;
+    if (jj >= 0 && jj < P.nx && kk >= 0 && kk < P.ny) {
+      const size_t c = cidx(P, jj, kk);
+      const bool bnd = (jj == 0 || jj == P.nx - 1 || kk == 0 || kk == P.ny - 1);
+      const double h = D.q[c];
+      const int sc = (P.radial && D.scell) ? D.scell[c] : 0;
+      b = (bnd || solve_all || h > 0.0 || sc == 2) ? 1 : 0;
+      w = (!(h == 0.0) || sc != 0) ? 1 : 0;
+    }
+    base[r][cc] = b;
+    wet[r][cc] = w;
+  }
+  __syncthreads();
+  const int tx = threadIdx.x & 63;
+  const int R = P.nrk;
+  const size_t pl = plane(P);
+  int count = 0;
+  for (int ty = threadIdx.x >> 6; ty < kMaskTY; ty += 4) {
+    const int j = j0 + tx, k = k0 + ty;
+    if (j >= P.nx || k >= P.ny) continue;
+    const size_t c = cidx(P, j, k);
+    const int lr = ty + kMaskH, lc = tx + kMaskH;
     const bool bnd = (j == 0 || j == P.nx - 1 || k == 0 || k == P.ny - 1);
-    if (bnd) {
-      active = 1;
-    } else {
-      const int R = P.nrk;
+    int active = bnd ? 1 : 0;
+    if (!active) {
       for (int dk = -R; dk <= R && !active; dk++) {
-        const int kk = k + dk;
-        if (kk < 0 || kk >= P.ny) continue;
         const int rem = R - (dk < 0 ? -dk : dk);
-        for (int dj = -rem; dj <= rem; dj++) {
-          const int jj = j + dj;
-          if (jj < 0 || jj >= P.nx) continue;
-          const bool b2 = (jj == 0 || jj == P.nx - 1 || kk == 0 || kk == P.ny - 1);
-          bool base = b2 || solve_all;
-          if (!base) {
-            base = D.q[cidx(P, jj, kk)] > 0.0;
-            if (!base && P.radial && D.scell) base = D.scell[cidx(P, jj, kk)] == 2;
-          }
-          if (base) {
+        for (int dj = -rem; dj <= rem; dj++)
+          if (base[lr + dk][lc + dj]) {
             active = 1;
             break;
           }
-        }
       }
     }
-    if (P.radial && D.scell && D.scell[cidx(P, j, k)] == 1) active = 0;
-    D.mask[cidx(P, j, k)] = (uint8_t)active;
+    if (P.radial && D.scell && D.scell[c] == 1) active = 0;
+    int compute = 0;
+    if (active) {
+      bool quiet = true;
+#pragma unroll
+      for (int d = -2; d <= 2; d++) quiet = quiet && !wet[lr][lc + d];
+      if (P.ny > 1) {
+#pragma unroll
+        for (int d = -2; d <= 2; d++) quiet = quiet && !wet[lr + d][lc];
+      }
+      if (j <= 1) quiet = quiet && P.bc_quiet[IMEXSW_W];
+      if (j >= P.nx - 2) quiet = quiet && P.bc_quiet[IMEXSW_E];
+      if (P.ny > 1) {
+        if (k <= 1) quiet = quiet && P.bc_quiet[IMEXSW_S];
+        if (k >= P.ny - 2) quiet = quiet && P.bc_quiet[IMEXSW_N];
+      }
+      if (quiet)
+        for (int i = 1; i < P.nv; i++) quiet = quiet && (D.q[i * pl + c] == 0.0);
+      compute = quiet ? 0 : 1;
+    }
+    D.mask[c] = (uint8_t)((active ? M_ACTIVE : 0) | (compute ? M_COMPUTE : 0));
+    count += active;
   }
-  // count of active cells (solve_cells)
-  unsigned long long bal = __ballot(active);
-  if ((threadIdx.x & 63) == 0 && bal)
-    atomicAdd(&D.counters[C_SOLVE_CELLS], (unsigned long long)__popcll(bal));
+  // count of active cells (solve_cells): wave sum, one atomic per wave
+  for (int off = 32; off > 0; off >>= 1) count += __shfl_xor(count, off, 64);
+  if ((threadIdx.x & 63) == 0 && count)
+    atomicAdd(ctr(D, C_SOLVE_CELLS, blockIdx.y * gridDim.x + blockIdx.x),
+              (unsigned long long)count);
 }
 
 // ---------------------------------------------------------------------------
@@ -407,23 +462,39 @@ __device__ __forceinline__ void yface_sides(const DevParams& P, const double* hi
 }
 
 template <int NS, bool HASY>
-__global__ void __launch_bounds__(64) k_stencil(DevParams P, DevPtrs D, int strip_h,
+__global__ void __launch_bounds__(64) k_stencil(DevParams P, DevPtrs D, int unit,
+                                                int max_strip, int target_strips,
                                                 int compute_dt) {
   constexpr int NV = 4 + NS;
   const int lane = threadIdx.x;
   const int j = blockIdx.x * kStripW - 2 + lane;
-  const int k0 = blockIdx.y * strip_h;
+  // Strip height: the grid has one workgroup per `unit` rows; the height
+  // actually used is the largest unit * 2^m <= max_strip that still leaves
+  // about `target_strips` strips for this step's active cells (short strips
+  // when few cells are active, so the sequential march stays short; tall
+  // strips on a wet grid, so the halo rows of each strip are amortised).
+  // Workgroups whose first row is not a strip start exit.
+  int strip_h = unit;
+  if (unit > 1) {
+    unsigned long long w = *ctr(D, C_SOLVE_CELLS, lane);
+    for (int off = 32; off > 0; off >>= 1) w += __shfl_xor(w, off, 64);
+    while (2 * strip_h <= max_strip &&
+           w / (unsigned long long)(kStripW * 2 * strip_h) >= (unsigned long long)target_strips)
+      strip_h *= 2;
+  }
+  const int k0 = blockIdx.y * unit;
+  if (k0 % strip_h) return;
   const int k1 = min(k0 + strip_h, P.ny);
   const bool injx = (j >= 0 && j < P.nx);
   const bool own = (lane >= 2 && lane < 2 + kStripW && injx);
   if (D.scal[S_DONE] != 0.0) return;
   const double t = D.scal[S_T];
 
-  // skip strips without an active cell
+  // skip strips without a cell to compute
   {
     int any = 0;
     if (own)
-      for (int k = k0; k < k1; k++) any |= D.mask[cidx(P, j, k)];
+      for (int k = k0; k < k1; k++) any |= D.mask[cidx(P, j, k)] & M_COMPUTE;
     if (!__any(any)) return;
   }
   const size_t pl = plane(P);
@@ -592,7 +663,7 @@ __global__ void __launch_bounds__(64) k_stencil(DevParams P, DevPtrs D, int stri
       }
     }
     // ---------------- divergence, explicit terms, D ----------------
-    if (own && D.mask[c]) {
+    if (own && (D.mask[c] & M_COMPUTE)) {
       double ex[NV];
       const double bx = D.Bx[c], by = D.By[c];
       const double bxx = D.Bxx ? D.Bxx[c] : 0.0, bxy = D.Bxy ? D.Bxy[c] : 0.0,
@@ -628,7 +699,8 @@ __global__ void __launch_bounds__(64) k_stencil(DevParams P, DevPtrs D, int stri
   if (compute_dt) {
     // wave minimum, one atomic per wave (dt > 0: IEEE order == integer order)
     for (int off = 32; off > 0; off >>= 1) dtmin = fmin(dtmin, shfl_d(dtmin, (lane + off) & 63));
-    if (lane == 0) atomicMin(D.dt_bits, (unsigned long long)__double_as_longlong(dtmin));
+    if (lane == 0 && dtmin < P.max_dt)
+      atomicMin(D.dt_bits, (unsigned long long)__double_as_longlong(dtmin));
   }
 }
 
@@ -660,13 +732,15 @@ __global__ void k_advance_time(DevPtrs D) {
   double* s = D.scal;
   if (s[S_DONE] != 0.0) return;
   s[S_T] = s[S_T] + s[S_DT];
-  D.counters[C_STEPS] += 1;
-  D.counters[C_ACTIVE_UPD] += D.counters[C_SOLVE_CELLS];
+  unsigned long long n = 0;
+  for (int i = 0; i < kSlots; i++) n += *ctr(D, C_SOLVE_CELLS, i);
+  *ctr(D, C_STEPS, 0) += 1;
+  *ctr(D, C_ACTIVE_UPD, 0) += n;
 }
 
 __global__ void k_reset_step(DevParams P, DevPtrs D) {
   *D.dt_bits = (unsigned long long)__double_as_longlong(P.max_dt);
-  D.counters[C_SOLVE_CELLS] = 0;
+  for (int i = 0; i < kSlots; i++) *ctr(D, C_SOLVE_CELLS, i) = 0;
 }
 
 // ---------------------------------------------------------------------------
@@ -684,7 +758,7 @@ __global__ void __launch_bounds__(256) k_local(DevParams P, DevPtrs D, int ops) 
   const int k = blockIdx.y * kTileY + ty;
   const bool inb = (j < P.nx && k < P.ny);
   const size_t c = inb ? cidx(P, j, k) : 0;
-  const bool act = inb && D.mask[c] && (D.scal[S_DONE] == 0.0);
+  const bool act = inb && (D.mask[c] & M_COMPUTE) && (D.scal[S_DONE] == 0.0);
   __shared__ double red[4][4];
   __shared__ unsigned long long cnt[4][8];
   double s_mass = 0.0, s_vol = 0.0, s_area = 0.0, s_sol = 0.0;
@@ -857,9 +931,10 @@ __global__ void __launch_bounds__(256) k_local(DevParams P, DevPtrs D, int ops) 
   }
   __syncthreads();
   if (threadIdx.x == 0) {
+    const int slot = blockIdx.y * gridDim.x + blockIdx.x;
     for (int m = 0; m < 8; m++) {
       unsigned long long v = cnt[0][m] + cnt[1][m] + cnt[2][m] + cnt[3][m];
-      if (v) atomicAdd(&D.counters[C_NEWTON_CELLS + m], v);
+      if (v) atomicAdd(ctr(D, C_NEWTON_CELLS + m, slot), v);
     }
     if (ops & OP_SUMS) {
       const size_t tile = (size_t)blockIdx.y * gridDim.x + blockIdx.x;
@@ -872,9 +947,9 @@ __global__ void __launch_bounds__(256) k_local(DevParams P, DevPtrs D, int ops) 
 // ---------------------------------------------------------------------------
 // k_topo: topography_reconstruction (src/geometry_2d.f90:688-856) for the
 // fields the hot path reads.  Border cells take the value of the nearest
-// cell of the 5-point interior (:737-796).  Only tiles within one tile of an
-// active cell are recomputed when `sparse` (the topography changes only
-// where the flow is).
+// cell of the 5-point interior (:737-796).  When `sparse`, only tiles within
+// four cells of a computed cell are recomputed: B_cent changes only where
+// the stage update runs.
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(256) k_topo(DevParams P, DevPtrs D, int sparse) {
   const int j = blockIdx.x * kTileX + (threadIdx.x & 63);
@@ -887,7 +962,7 @@ __global__ void __launch_bounds__(256) k_topo(DevParams P, DevPtrs D, int sparse
       for (int dk = -4; dk <= 4 && !any; dk += 4)
         for (int dj = -4; dj <= 4; dj += 4) {
           int jj = min(max(j + dj, 0), P.nx - 1), kk = min(max(k + dk, 0), P.ny - 1);
-          if (D.mask[cidx(P, jj, kk)]) {
+          if (D.mask[cidx(P, jj, kk)] & M_COMPUTE) {
             any = 1;
             break;
           }

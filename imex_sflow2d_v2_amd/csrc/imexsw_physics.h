@@ -33,6 +33,9 @@ struct DevParams {
   int limiter[IMEXSW_MAX_VARS + 2];
   int bc_flag[4][IMEXSW_MAX_VARS];
   double bc_value[4][IMEXSW_MAX_VARS];
+  // the side's condition on h keeps a dry neighbourhood dry: zero-gradient
+  // extrapolation, or a prescribed value / slope of exactly zero
+  int bc_quiet[4];
   double grav, pres, inv_pres, T_ambient, sp_heat_a, sp_gas_const_a,
       rho_a_amb, kin_visc_c;
   double rho_s[IMEXSW_MAX_SOLID], inv_rho_s[IMEXSW_MAX_SOLID],

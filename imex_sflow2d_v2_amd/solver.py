@@ -60,6 +60,7 @@ ABI = {
     "imexsw_set_timing": (C.c_int, [C.c_void_p, C.c_int]),
     "imexsw_kernel_times": (C.c_int, [C.c_void_p, _D, _I64, C.c_int]),
     "imexsw_kernel_name": (C.c_char_p, [C.c_int]),
+    "imexsw_set_tuning": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
     "imexsw_sync": (C.c_int, [C.c_void_p]),
 }
 
@@ -209,6 +210,9 @@ class Solver:
                                           C.byref(n)))
         self.t = float(t[0])
         return n.value
+
+    def set_tuning(self, max_strip_rows=32, target_strips=2048):
+        self._chk(self.lib.imexsw_set_tuning(self.h, int(max_strip_rows), int(target_strips)))
 
     def sync(self):
         self._chk(self.lib.imexsw_sync(self.h))

@@ -256,13 +256,21 @@ int imexsw_get_dt(imexsw_ctx *ctx, double *dt);
 int imexsw_stats_get(imexsw_ctx *ctx, imexsw_stats *stats, int reset);
 
 /* Per-kernel device time, measured with HIP events recorded on the
- * context's stream around every launch.  names: mask, cfl, flux, local,
+ * context's stream around every launch.  names: mask, stencil, local,
  * topo, dtpolicy, post.  ms[i] accumulates milliseconds, calls[i] launches.
  * Returns the number of kernels (<= cap).  Timing is enabled by
  * imexsw_set_timing(ctx, 1). */
 int imexsw_set_timing(imexsw_ctx *ctx, int enable);
 int imexsw_kernel_times(imexsw_ctx *ctx, double *ms, int64_t *calls, int cap);
 const char *imexsw_kernel_name(int index);
+
+/* Launch shape of the stencil pass (no reference counterpart: the
+ * reference's OpenMP loops have no tiling).  Each step the stencil kernel
+ * picks the tallest strip of 4 * 2^m rows, m >= 0, up to max_strip_rows,
+ * that still leaves about target_strips strips for the step's active cells.
+ * max_strip_rows: 4, 8, 16, 32 or 64; target_strips >= 1.  Results do not
+ * depend on these settings; defaults 32 and 2048. */
+int imexsw_set_tuning(imexsw_ctx *ctx, int max_strip_rows, int target_strips);
 
 /* Wait for all work queued on the context's stream. */
 int imexsw_sync(imexsw_ctx *ctx);
