@@ -342,3 +342,125 @@ def test_source_ring_matches_host_setup():
     assert np.array_equal(bits, bits2)
     assert np.array_equal(v, v2)
     assert (sc == 1).sum() > 0 and (sc == 2).sum() > 0
+
+
+# --------------------------------------------------------------------------
+# elementary functions with a fixed operation sequence (shared with the
+# device): accuracy against the platform's correctly rounded libm
+# --------------------------------------------------------------------------
+def test_deterministic_pow_cos_accuracy():
+    import ctypes
+    from oracle.pyoracle import load_oracle
+    lib = load_oracle()
+    lib.orc_det_pow.restype = ctypes.c_double
+    lib.orc_det_pow.argtypes = [ctypes.c_double, ctypes.c_double]
+    lib.orc_det_cos.restype = ctypes.c_double
+    lib.orc_det_cos.argtypes = [ctypes.c_double]
+    rng = np.random.default_rng(11)
+    # the exponents and argument ranges of the settling (Re^0.687,
+    # hindrance^4.65), entrainment (Ri^2.4) and source (t^1.5, t^0.5) laws
+    # exp(y log x) with a log good to about an ulp: relative error within
+    # 2 eps (1 + |y log x|), the conditioning of the exponential
+    eps = np.finfo(float).eps
+    xs = 10.0 ** rng.uniform(-8, 4, 20000)
+    for y in (0.687, 4.65, 2.4, 1.5, 0.5):
+        got = np.array([lib.orc_det_pow(x, y) for x in xs])
+        ref = xs ** y
+        bound = 2.0 * eps * (1.0 + np.abs(y * np.log(xs)))
+        assert np.all(np.abs(got - ref) / ref <= bound), y
+    assert lib.orc_det_pow(0.0, 2.4) == 0.0
+    assert lib.orc_det_pow(1.0, 4.65) == 1.0
+    assert lib.orc_det_pow(3.0, 0.0) == 1.0
+    th = rng.uniform(-4 * math.pi, 4 * math.pi, 20000)
+    got = np.array([lib.orc_det_cos(x) for x in th])
+    assert np.max(np.abs(got - np.cos(th))) < 2e-15
+    assert lib.orc_det_cos(0.0) == 1.0
+
+
+# --------------------------------------------------------------------------
+# topography_reconstruction: the 5-point stencils are exact on quadratics
+# --------------------------------------------------------------------------
+def test_topography_reconstruction_exact_on_quadratic():
+    n, ny, dx = 12, 10, 2.5
+    a, b, c2, d2, e2 = 0.3, -0.2, 0.01, -0.004, 0.007
+    p = make_params(n, ny, dx, 1, slope_correction=True, curvature=True)
+    xc, yc, _, _ = decks.cell_centres(p)
+    X, Y = np.meshgrid(xc, yc, indexing="ij")
+    B = 100.0 + a * X + b * Y + c2 * X ** 2 + d2 * Y ** 2 + e2 * X * Y
+    o = Oracle(p)
+    o.lib.orc_set_topography(o.h, np.asfortranarray(B).ctypes.data_as(
+        __import__("ctypes").POINTER(__import__("ctypes").c_double)), None)
+    o.lib.orc_topography_reconstruction(o.h)
+    Bc, Bx, By, Bxx, Bxy, Byy = o.geometry()
+    inner = (slice(2, n - 2), slice(2, ny - 2))
+    np.testing.assert_allclose(Bx[inner], (a + 2 * c2 * X + e2 * Y)[inner], rtol=0, atol=1e-12)
+    np.testing.assert_allclose(By[inner], (b + 2 * d2 * Y + e2 * X)[inner], rtol=0, atol=1e-12)
+    np.testing.assert_allclose(Bxx, 2 * c2, rtol=0, atol=1e-12)
+    np.testing.assert_allclose(Byy, 2 * d2, rtol=0, atol=1e-12)
+    np.testing.assert_allclose(Bxy, e2, rtol=0, atol=1e-12)
+    # border cells take the value of the nearest cell of the 5-point interior
+    np.testing.assert_array_equal(Bx[0, 3], Bx[2, 3])
+    np.testing.assert_array_equal(By[4, ny - 1], By[4, ny - 3])
+
+
+# --------------------------------------------------------------------------
+# check_solve: N_RK-fold 4-neighbour dilation == L1 distance <= N_RK
+# --------------------------------------------------------------------------
+def test_check_solve_matches_taxicab_distance():
+    from scipy.ndimage import distance_transform_cdt
+    d = decks.small_2d(n=30, ny=26, seed=12)
+    o = Oracle(d.params).load_deck(d)
+    rng = np.random.default_rng(12)
+    h = np.where(rng.random((30, 26)) < 0.04, 1.0, 0.0)
+    q = np.zeros_like(d.q)
+    q[0] = h * 10.0
+    o.set_q(q)
+    n = o.check_solve(False)
+    m = o.mask().astype(bool)
+    seed = h > 0
+    seed[0, :] = seed[-1, :] = seed[:, 0] = seed[:, -1] = True
+    dist = distance_transform_cdt(~seed, metric="taxicab")
+    expect = dist <= d.params.n_rk
+    assert np.array_equal(m, expect)
+    assert n == int(expect.sum())
+
+
+# --------------------------------------------------------------------------
+# erosion / deposition: the published rate law on one cell, and the solid
+# budget between flow and bed closing to round-off
+# --------------------------------------------------------------------------
+def test_erosion_rate_law_and_solid_budget():
+    d = decks.small_2d(n=16, ny=14, seed=13, erosion=True, settling=True, n_solid=2,
+                       topo_change=True)
+    p = d.params
+    o = Oracle(p).load_deck(d)
+    er0, dep0, ers0, B0 = o.erosion_state()
+    q0, qp = o.q(), o.qp()
+    dt = 0.05
+    o.erosion_deposition(dt)
+    er1, dep1, ers1, B1 = o.erosion_state()
+    q1 = o.q()
+    # cells the update empties (all solids deposited, g' <= 0: the cell is
+    # zeroed, src/solver_2d.f90:2583-2587) leave the budget by design
+    wet = (q0[0] > 1e-3) & (q1[0] > 0.0)
+    assert wet.sum() > 20
+    rho_s = np.array([p.rho_s[i] for i in range(2)])
+    for i in range(2):
+        dq = (q1[4 + i] - q0[4 + i])[wet]
+        bed = rho_s[i] * ((ers1[..., i] - ers0[..., i]) - (dep1[..., i] - dep0[..., i]))[wet]
+        np.testing.assert_allclose(dq, bed, rtol=1e-9, atol=1e-12 * np.abs(q0[4 + i]).max())
+        # the erodible layer loses what was eroded (no deposit feedback)
+        np.testing.assert_allclose((er1[i] - er0[i])[wet], -(ers1[..., i] - ers0[..., i])[wet],
+                                   rtol=0, atol=1e-15)
+    # bed elevation change = (deposit - erosion) / (1 - porosity), summed
+    ddep = (dep1 - dep0).sum(axis=2)
+    ders = (ers1 - ers0).sum(axis=2)
+    np.testing.assert_allclose((B1 - B0)[wet], ((ddep - ders) / (1.0 - p.erodible_porosity))[wet],
+                               rtol=1e-12, atol=1e-15)
+    # erosion rate: coeff * |u| * h * (1 - alpha_tot) * (1 - porosity) * fract
+    j, k = np.argwhere(wet)[0]
+    h, u, v = qp[0, j, k], qp[p.n_solid + 4, j, k], qp[p.n_solid + 5, j, k]
+    al = qp[4:4 + p.n_solid, j, k] / h
+    rate = p.erosion_coeff * math.hypot(u, v) * h * (1.0 - al.sum()) * (1.0 - p.erodible_porosity)
+    expect = [min(rate * p.erodible_fract[i], er0[i, j, k] / dt) for i in range(2)]
+    np.testing.assert_allclose((ers1 - ers0)[j, k] / dt, expect, rtol=1e-12)
