@@ -77,22 +77,18 @@ def step_bytes(p):
 
 def parse_traffic(kernel, workload):
     """HBM bytes per dispatch of `kernel` from the committed rocprofv3
-    counter summary (FETCH_SIZE and WRITE_SIZE in KiB per dispatch, scaled by
-    the copy-kernel calibration factor recorded in the same file)."""
+    counter summary: FETCH_SIZE + WRITE_SIZE per dispatch, each scaled by
+    what the same counter reads for a device copy of known size (the file
+    records the raw values, the factors and the product)."""
     path = os.path.join(HERE, "profiles", "%s_%s_traffic.txt" % (kernel, workload))
     if not os.path.exists(path):
         return None
-    vals = {}
     with open(path) as f:
         for line in f:
-            m = re.match(r"\s*(fetch_kib_per_dispatch|write_kib_per_dispatch|"
-                         r"calibration_factor)\s*[:=]\s*([0-9.eE+-]+)", line)
+            m = re.match(r"\s*hbm_bytes_per_dispatch\s*=\s*([0-9.eE+-]+)", line)
             if m:
-                vals[m.group(1)] = float(m.group(2))
-    if "fetch_kib_per_dispatch" not in vals or "write_kib_per_dispatch" not in vals:
-        return None
-    k = vals.get("calibration_factor", 1.0)
-    return (vals["fetch_kib_per_dispatch"] + vals["write_kib_per_dispatch"]) * 1024.0 * k
+                return float(m.group(1))
+    return None
 
 
 def make_deck(workload):
