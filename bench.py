@@ -124,12 +124,82 @@ def cpu_baseline(workload, steps, threads):
     }
 
 
+def bench_ensemble(args):
+    """Batched ensemble: `members` grids of log-uniform sizes 128^2..1024^2
+    with sampled parameters, `steps` (default 500) steps each, advanced by
+    batched launches in captured graphs after the strip-height autotuner has
+    picked the shape on a warm-up copy; compared with a sequential loop that
+    advances the members one after the other, each in its own context."""
+    from imex_sflow2d_v2_amd import decks
+    from imex_sflow2d_v2_amd.solver import Ensemble, Solver
+    steps = args.steps if args.steps != 1000 else 500
+    ds = decks.ensemble_members(count=args.members)
+    params = [d.params for d in ds]
+    cells = sum(p.nx * p.ny for p in params)
+    dt0 = ds[0].dt0
+    warm = Ensemble(params).load(ds)
+    best = warm.autotune(max(1, args.warmup // 3 or 1), dt0)
+    warm.close()
+    ens = Ensemble(params).load(ds)
+    ens.set_tuning(best, 25)
+    for m in ens.members:
+        m.sync()
+    t0 = time.perf_counter()
+    n = ens.advance(steps, dt0)
+    el = time.perf_counter() - t0
+    dev_ms = ens.last_ms()
+    iters = sum(m.stats()["newton_iters"] for m in ens.members)
+    ncell = sum(m.stats()["newton_cells"] for m in ens.members)
+    ens.close()
+    # sequential reference loop: one context per member, one after the other
+    seq = 0.0
+    for d in ds:
+        g = Solver(d.params).load_deck(d)
+        g.sync()
+        a = time.perf_counter()
+        g.advance(steps, dt0=d.dt0)
+        g.sync()
+        seq += time.perf_counter() - a
+        g.close()
+    return {
+        "metric": "ensemble members/s (%d steps each)" % steps,
+        "value": len(ds) / el,
+        "unit": "members/s",
+        "n_gpus": 1,
+        "steps": n,
+        "warmup": args.warmup,
+        "ms_per_step": el / max(n, 1) * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": "ensemble_%d" % len(ds),
+            "members": len(ds),
+            "total_cells": cells,
+            "sizes_min_max": [min(p.nx for p in params), max(p.nx for p in params)],
+            "strip_rows_autotuned": best,
+            "graph_steps": 25,
+            "grid_cell_updates_per_s": cells * n / el,
+            "device_ms": dev_ms,
+            "sequential_members_per_s": len(ds) / seq,
+            "speedup_vs_sequential": seq / el,
+            "newton_iters_per_cell": iters / max(1, ncell),
+        },
+        "roofline": None,
+        "cpu_baseline": None,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--workload", default="volcano", choices=["volcano", "wet"])
+    ap.add_argument("--workload", default="volcano", choices=["volcano", "wet", "ensemble"])
+    ap.add_argument("--members", type=int, default=256,
+                    help="ensemble workload: number of members")
     ap.add_argument("--cpu-steps", type=int, default=30)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
@@ -159,6 +229,10 @@ def main():
         b.copy_(a)
         torch.cuda.synchronize()
         print(json.dumps({"copy_probe_bytes_read": 8 * n, "copy_probe_bytes_written": 8 * n}))
+        return 0
+
+    if args.workload == "ensemble":
+        print(json.dumps(bench_ensemble(args)), flush=True)
         return 0
 
     d = make_deck(args.workload)
@@ -217,6 +291,9 @@ def main():
             "active_cell_updates_per_s": active / el,
             "active_fraction": active / float(cells * steps),
             "newton_iters_per_cell": st["newton_iters"] / max(1, st["newton_cells"]),
+            "newton_iter_hist_1_2_3_4_8_16_32_100": st["newton_iter_hist"],
+            "newton_iter_max": st["newton_iter_max"],
+            "line_search_evals_per_cell": st["line_search_evals"] / max(1, st["newton_cells"]),
             "simulated_time_s": g.t,
             "kernel_ms_per_step": {k: v[0] / steps for k, v in kt.items() if v[1]},
             "step_bytes_per_active_cell": step_bytes(p),

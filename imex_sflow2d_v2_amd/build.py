@@ -12,7 +12,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libimexsw.so")
 SOURCES = ["imexsw.hip"]
-HEADERS = ["imexsw_math.h", "imexsw_physics.h", "imexsw_newton.h", "imexsw_kernels.h"]
+HEADERS = ["imexsw_math.h", "imexsw_physics.h", "imexsw_newton.h", "imexsw_kernels.h",
+           "imexsw_batch.h"]
 ARCH = os.environ.get("IMEXSW_ARCH", "gfx950")
 
 HIPCC_FLAGS = [

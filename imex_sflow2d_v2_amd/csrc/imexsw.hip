@@ -14,14 +14,15 @@
 #include <string>
 #include <vector>
 
+#include "imexsw_batch.h"
 #include "imexsw_kernels.h"
 
 using namespace imexsw;
 
 namespace {
 
-enum KernelId { K_MASK = 0, K_STENCIL, K_LOCAL, K_TOPO, K_DTPOLICY, K_POST, K_NK };
-const char* kKernelNames[K_NK] = {"mask", "stencil", "local", "topo", "dtpolicy", "post"};
+enum KernelId { K_MASK = 0, K_STENCIL, K_LOCAL, K_TOPO, K_DTPOLICY, K_POST, K_EROSION, K_NK };
+const char* kKernelNames[K_NK] = {"mask", "stencil", "local", "topo", "dtpolicy", "post", "erosion"};
 
 thread_local std::string g_create_error;
 
@@ -37,9 +38,11 @@ struct imexsw_ctx {
   size_t plane = 0;
   double* stage = nullptr;  // layout-conversion staging
   size_t stage_bytes = 0;
-  dim3 grid_tiles, grid_mask, grid_strips, grid_stag;
-  int strip_h = 32;         // tallest stencil strip (rows)
-  int row_unit = 4;         // stencil grid granularity (rows)
+  dim3 grid_tiles, grid_mask, grid_stag;
+  int n_tiles = 0;  // work tiles (kStripW x kTileH)
+  int n_cu = 256;   // compute units of the device
+  int strip_h = 16;         // tallest stencil strip (rows)
+  int row_unit = 1;         // shortest stencil strip (rows)
   int target_strips = 2048; // strips wanted in flight
   bool d_valid = false;   // D holds the residual of the current q
   bool dt_hist = false;   // dt_old / dt_old_old initialised
@@ -49,7 +52,15 @@ struct imexsw_ctx {
   std::vector<hipEvent_t> ev_pool;
   std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> pending;
   std::vector<void*> allocs;
+  // captured kGraphSteps-step graph of imexsw_advance and what it captured
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  DevParams graph_P{};
+  DevPtrs graph_D{};
+  long graph_shape = 0;
 };
+
+constexpr int kGraphSteps = 20;
 
 namespace {
 
@@ -131,18 +142,52 @@ int check_launch(imexsw_ctx* c, const char* what) {
 }
 
 // ---- NS dispatch -----------------------------------------------------------
+// The stencil and local kernels walk device-side work lists (listed tiles
+// and tile rows) with a grid-stride loop, so the grid is sized for the
+// device, not for the work: as many workgroups as fit on the device at once
+// (one round, no queue of workgroups that find nothing to do), never more
+// than the largest possible list needs.
+template <typename K>
+int resident_grid(K kernel, int threads, int n_cu, int items) {
+  int per_cu = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, threads, 0) != hipSuccess ||
+      per_cu < 1)
+    per_cu = 1;
+  return std::max(1, std::min(items, per_cu * n_cu));
+}
+
 template <int NS>
 void launch_stencil(imexsw_ctx* c, int compute_dt) {
-  if (c->P.ny > 1)
-    hipLaunchKernelGGL((k_stencil<NS, true>), c->grid_strips, dim3(64), 0, c->stream, c->P,
-                       c->D, c->row_unit, c->strip_h, c->target_strips, compute_dt);
-  else
-    hipLaunchKernelGGL((k_stencil<NS, false>), c->grid_strips, dim3(64), 0, c->stream, c->P,
-                       c->D, c->row_unit, c->strip_h, c->target_strips, compute_dt);
+  // work items: (listed tile, strip) pairs, at most tiles * kTileH / row_unit
+  const int items = c->n_tiles * (kTileH / c->row_unit);
+  if (c->P.ny > 1) {
+    static int res = 0;
+    if (!res) res = resident_grid(k_stencil<NS, true>, 64, c->n_cu, 1 << 30);
+    hipLaunchKernelGGL((k_stencil<NS, true>), dim3(std::min(items, res)), dim3(64), 0,
+                       c->stream, c->P, c->D, c->row_unit, c->strip_h, c->target_strips,
+                       compute_dt);
+  } else {
+    static int res = 0;
+    if (!res) res = resident_grid(k_stencil<NS, false>, 64, c->n_cu, 1 << 30);
+    hipLaunchKernelGGL((k_stencil<NS, false>), dim3(std::min(items, res)), dim3(64), 0,
+                       c->stream, c->P, c->D, c->row_unit, c->strip_h, c->target_strips,
+                       compute_dt);
+  }
 }
 template <int NS>
 void launch_local(imexsw_ctx* c, int ops) {
-  hipLaunchKernelGGL((k_local<NS>), c->grid_tiles, dim3(256), 0, c->stream, c->P, c->D, ops);
+  // work items: listed tile rows, four per workgroup; the stage update and
+  // the erosion/diagnostics part are two kernels
+  const int items = (c->n_tiles * kTileH + 3) / 4;
+  static int res0 = 0, res1 = 0;
+  if (!res0) res0 = resident_grid(k_local<NS, 0>, 256, c->n_cu, 1 << 30);
+  if (!res1) res1 = resident_grid(k_local<NS, 1>, 256, c->n_cu, 1 << 30);
+  if (ops & OP_RK)
+    hipLaunchKernelGGL((k_local<NS, 0>), dim3(std::min(items, res0)), dim3(256), 0, c->stream,
+                       c->P, c->D, (int)OP_RK);
+  if (ops & (OP_EROSION | OP_POST | OP_SUMS))
+    hipLaunchKernelGGL((k_local<NS, 1>), dim3(std::min(items, res1)), dim3(256), 0, c->stream,
+                       c->P, c->D, ops & ~(int)OP_RK);
 }
 template <int NS>
 void launch_qp(imexsw_ctx* c, double* dst) {
@@ -184,7 +229,7 @@ int run_local(imexsw_ctx* c, int ops, int kid) {
 int run_mask(imexsw_ctx* c, int solve_all) {
   {
     Timed tm(c, K_MASK);
-    hipLaunchKernelGGL(k_mask, c->grid_mask, dim3(256), 0, c->stream, c->P, c->D, solve_all);
+    hipLaunchKernelGGL(k_mask, c->grid_mask, dim3(64), 0, c->stream, c->P, c->D, solve_all);
   }
   return check_launch(c, "k_mask");
 }
@@ -192,11 +237,34 @@ int run_mask(imexsw_ctx* c, int solve_all) {
 int run_topo(imexsw_ctx* c, int sparse) {
   {
     Timed tm(c, K_TOPO);
-    hipLaunchKernelGGL(k_topo, c->grid_tiles, dim3(256), 0, c->stream, c->P, c->D, sparse);
+    hipLaunchKernelGGL(k_topo, c->grid_mask, dim3(256), 0, c->stream, c->P, c->D, sparse);
     if (c->D.gcoef)
       hipLaunchKernelGGL(k_gstag, c->grid_stag, dim3(256), 0, c->stream, c->P, c->D);
   }
   return check_launch(c, "k_topo");
+}
+
+// one time step of the main loop (src/IMEX_SfloW2D.f90:337-531), all on the
+// context's stream
+int enqueue_step(imexsw_ctx* c) {
+  hipLaunchKernelGGL(k_reset_step, dim3(1), dim3(1), 0, c->stream, c->P, c->D);
+  if (run_mask(c, 0)) return -1;
+  if (run_stencil(c, c->P.cfl != -1.0)) return -1;
+  {
+    Timed tm(c, K_DTPOLICY);
+    hipLaunchKernelGGL(k_dtpolicy, dim3(1), dim3(1), 0, c->stream, c->P, c->D);
+  }
+  if (run_local(c, OP_RK | OP_EROSION | OP_POST, K_LOCAL)) return -1;
+  if (c->P.topo && run_topo(c, 1)) return -1;
+  hipLaunchKernelGGL(k_advance_time, dim3(1), dim3(1), 0, c->stream, c->D);
+  return check_launch(c, "k_advance_time");
+}
+
+void drop_ctx_graph(imexsw_ctx* c) {
+  if (c->exec) (void)hipGraphExecDestroy(c->exec);
+  if (c->graph) (void)hipGraphDestroy(c->graph);
+  c->exec = nullptr;
+  c->graph = nullptr;
 }
 
 int set_scalar(imexsw_ctx* c, int slot, double v) {
@@ -414,12 +482,22 @@ int imexsw_create(const imexsw_params* params, imexsw_ctx** out) {
   rc |= dalloc(c, &D.scal, S_NSLOTS);
   rc |= dalloc(c, &D.counters, C_NCOUNTERS * kSlots);
   c->grid_tiles = dim3((P.nx + kTileX - 1) / kTileX, (P.ny + kTileY - 1) / kTileY);
-  c->grid_mask = dim3((P.nx + kMaskTX - 1) / kMaskTX, (P.ny + kMaskTY - 1) / kMaskTY);
+  c->grid_mask = dim3((P.nx + kStripW - 1) / kStripW, (P.ny + kTileH - 1) / kTileH);
   c->grid_stag = dim3((P.nx + 1 + kTileX - 1) / kTileX, (P.ny + 1 + kTileY - 1) / kTileY);
-  rc |= dalloc(c, &D.tile_sums, 4 * (size_t)c->grid_tiles.x * c->grid_tiles.y);
-  c->strip_h = (P.ny > 1) ? 32 : 1;
-  c->row_unit = (P.ny > 1) ? 4 : 1;
-  c->grid_strips = dim3((P.nx + kStripW - 1) / kStripW, (P.ny + c->row_unit - 1) / c->row_unit);
+  c->n_tiles = (int)(c->grid_mask.x * c->grid_mask.y);
+  rc |= dalloc(c, &D.tile_sums, 4 * (size_t)c->n_tiles * kTileH);
+  rc |= dalloc(c, &D.tile_flag, (size_t)c->n_tiles);
+  rc |= dalloc(c, &D.tile_list, (size_t)c->n_tiles);
+  rc |= dalloc(c, &D.row_list, (size_t)c->n_tiles * kTileH);
+  c->strip_h = (P.ny > 1) ? kTileH : 1;
+  c->row_unit = 1;
+  {
+    int dev = 0, ncu = 0;
+    if (hipGetDevice(&dev) == hipSuccess &&
+        hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
+        ncu > 0)
+      c->n_cu = ncu;
+  }
   if (rc) {
     g_create_error = c->err;
     imexsw_destroy(c);
@@ -438,6 +516,7 @@ void imexsw_destroy(imexsw_ctx* c) {
   if (!c) return;
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   collect_timing(c);
+  drop_ctx_graph(c);
   for (void* p : c->allocs) (void)hipFree(p);
   if (c->stage) (void)hipFree(c->stage);
   for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
@@ -549,6 +628,10 @@ int imexsw_check_solve(imexsw_ctx* c, int solve_all, double t, int32_t* solve_ce
   if (set_scalar(c, S_DONE, 0.0)) return -1;
   HIPCHK(c, hipMemsetAsync(c->D.counters + C_SOLVE_CELLS * kSlots, 0,
                            kSlots * sizeof(unsigned long long), c->stream));
+  HIPCHK(c, hipMemsetAsync(c->D.counters + C_COMPUTE_CELLS * kSlots, 0,
+                           kSlots * sizeof(unsigned long long), c->stream));
+  HIPCHK(c, hipMemsetAsync(c->D.counters + C_TILES * kSlots, 0,
+                           2 * kSlots * sizeof(unsigned long long), c->stream));  // tiles, rows
   if (run_mask(c, solve_all)) return -1;
   c->d_valid = false;
   if (solve_cells) {
@@ -590,7 +673,7 @@ int imexsw_rk_step(imexsw_ctx* c, double t, double dt) {
 int imexsw_erosion_deposition(imexsw_ctx* c, double dt) {
   if (!c) return fail(c, "null argument");
   if (set_scalar(c, S_DT, dt)) return -1;
-  if (run_local(c, OP_EROSION, K_LOCAL)) return -1;
+  if (run_local(c, OP_EROSION, K_EROSION)) return -1;
   c->d_valid = false;
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return 0;
@@ -606,9 +689,12 @@ int imexsw_topography_reconstruction(imexsw_ctx* c) {
 
 int imexsw_post_step(imexsw_ctx* c, double sums[4]) {
   if (!c) return fail(c, "null argument");
+  if (sums)
+    HIPCHK(c, hipMemsetAsync(c->D.tile_sums, 0, 4 * (size_t)c->n_tiles * kTileH * sizeof(double),
+                             c->stream));
   if (run_local(c, OP_POST | (sums ? OP_SUMS : 0), K_POST)) return -1;
   if (sums) {
-    const size_t nt = (size_t)c->grid_tiles.x * c->grid_tiles.y;
+    const size_t nt = (size_t)c->n_tiles * kTileH;
     std::vector<double> ts(4 * nt);
     HIPCHK(c, hipMemcpyAsync(ts.data(), c->D.tile_sums, ts.size() * sizeof(double),
                              hipMemcpyDeviceToHost, c->stream));
@@ -641,20 +727,37 @@ int imexsw_advance(imexsw_ctx* c, int32_t nsteps, double dt0, double* t_io, doub
   unsigned long long steps0 = 0;
   HIPCHK(c, hipMemcpyAsync(&steps0, c->D.counters + C_STEPS * kSlots, sizeof steps0, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
-  const int ops = OP_RK | OP_EROSION | OP_POST;
-  for (int n = 0; n < nsteps; n++) {
-    hipLaunchKernelGGL(k_reset_step, dim3(1), dim3(1), 0, c->stream, c->P, c->D);
-    if (run_mask(c, 0)) return -1;
-    if (run_stencil(c, c->P.cfl != -1.0)) return -1;
-    {
-      Timed tm(c, K_DTPOLICY);
-      hipLaunchKernelGGL(k_dtpolicy, dim3(1), dim3(1), 0, c->stream, c->P, c->D);
+  // Without per-kernel timing the steps run as replays of a captured graph
+  // of kGraphSteps steps (one host call per replay instead of eight kernel
+  // launches per step); the rest, and every step when timing, is launched
+  // directly.  The graph is recaptured when the context's constants or
+  // pointers changed since the capture.
+  int n = 0;
+  if (!c->timing && nsteps >= kGraphSteps) {
+    if (!c->exec || std::memcmp(&c->graph_P, &c->P, sizeof(DevParams)) ||
+        std::memcmp(&c->graph_D, &c->D, sizeof(DevPtrs)) || c->graph_shape != c->strip_h * 1000003 + c->target_strips) {
+      drop_ctx_graph(c);
+      HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+      int rc = 0;
+      for (int i = 0; i < kGraphSteps && !rc; i++) rc = enqueue_step(c);
+      hipGraph_t g = nullptr;
+      hipError_t err = hipStreamEndCapture(c->stream, &g);
+      if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return -1;
+      }
+      if (err != hipSuccess) return fail(c, "hipStreamEndCapture: %s", hipGetErrorString(err));
+      c->graph = g;
+      HIPCHK(c, hipGraphInstantiate(&c->exec, g, nullptr, nullptr, 0));
+      c->graph_P = c->P;
+      c->graph_D = c->D;
+      c->graph_shape = c->strip_h * 1000003 + c->target_strips;
     }
-    if (run_local(c, ops, K_LOCAL)) return -1;
-    if (c->P.topo && run_topo(c, 1)) return -1;
-    hipLaunchKernelGGL(k_advance_time, dim3(1), dim3(1), 0, c->stream, c->D);
-    if (check_launch(c, "k_advance_time")) return -1;
+    for (; n + kGraphSteps <= nsteps; n += kGraphSteps)
+      HIPCHK(c, hipGraphLaunch(c->exec, c->stream));
   }
+  for (; n < nsteps; n++)
+    if (enqueue_step(c)) return -1;
   c->d_valid = false;
   if (get_scalars(c, s)) return -1;
   *t_io = s[S_T];
@@ -734,7 +837,10 @@ int imexsw_stats_get(imexsw_ctx* c, imexsw_stats* st, int reset) {
   unsigned long long v[C_NCOUNTERS];
   for (int m = 0; m < C_NCOUNTERS; m++) {
     v[m] = 0;
-    for (int i = 0; i < kSlots; i++) v[m] += raw[(size_t)m * kSlots + i];
+    for (int i = 0; i < kSlots; i++) {
+      const unsigned long long x = raw[(size_t)m * kSlots + i];
+      v[m] = (m == C_ITMAX) ? std::max(v[m], x) : v[m] + x;
+    }
   }
   std::memset(st, 0, sizeof *st);
   st->steps = (int64_t)v[C_STEPS];
@@ -748,12 +854,15 @@ int imexsw_stats_get(imexsw_ctx* c, imexsw_stats* st, int reset) {
   st->neg_solid_events = (int64_t)v[C_NEGS];
   st->sum_solid_events = (int64_t)v[C_SUMS];
   st->low_T_events = (int64_t)v[C_LOWT];
+  st->line_search_evals = (int64_t)v[C_LS_EVALS];
+  for (int b = 0; b < 8; b++) st->newton_iter_hist[b] = (int64_t)v[C_HIST0 + b];
+  st->newton_iter_max = (int64_t)v[C_ITMAX];
   st->solve_cells = (int32_t)v[C_SOLVE_CELLS];
-  st->active_tiles = 0;
+  st->compute_cells = (int32_t)v[C_COMPUTE_CELLS];
   if (reset) {
-    // keep the current solve_cells slots, clear everything else
+    // keep the current solve_cells / compute_cells slots, clear the rest
     for (size_t i = 0; i < raw.size(); i++)
-      if (i / kSlots != C_SOLVE_CELLS) raw[i] = 0;
+      if (i / kSlots != C_SOLVE_CELLS && i / kSlots != C_COMPUTE_CELLS) raw[i] = 0;
     HIPCHK(c, hipMemcpyAsync(c->D.counters, raw.data(), raw.size() * sizeof(unsigned long long),
                              hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -792,8 +901,8 @@ const char* imexsw_kernel_name(int index) {
 int imexsw_set_tuning(imexsw_ctx* c, int max_strip_rows, int target_strips) {
   if (!c) return fail(c, "null argument");
   const int m = max_strip_rows;
-  if (!(m == 4 || m == 8 || m == 16 || m == 32 || m == 64))
-    return fail(c, "max_strip_rows must be 4, 8, 16, 32 or 64 (got %d)", m);
+  if (!(m == 1 || m == 2 || m == 4 || m == 8 || m == 16))
+    return fail(c, "max_strip_rows must be 1, 2, 4, 8 or 16 (got %d)", m);
   if (target_strips < 1) return fail(c, "target_strips must be >= 1 (got %d)", target_strips);
   if (c->P.ny > 1) c->strip_h = m;
   c->target_strips = target_strips;
@@ -805,5 +914,321 @@ int imexsw_sync(imexsw_ctx* c) {
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return 0;
 }
+
+}  // extern "C"
+
+// ===========================================================================
+// ensembles: many member grids advanced by one set of launches per step,
+// the steps captured in a HIP graph
+// ===========================================================================
+struct imexsw_ensemble {
+  std::vector<imexsw_ctx*> members;
+  int ns = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  DevParams* d_params = nullptr;
+  DevPtrs* d_ptrs = nullptr;
+  int* d_tile_off = nullptr;
+  int* d_tiles = nullptr;
+  int* d_rows = nullptr;
+  unsigned long long* d_counts = nullptr;
+  int total_tiles = 0;
+  int n_cu = 256;
+  int strip_h = 8;
+  int graph_steps = 25;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  int graph_strip_h = 0;
+  double last_ms = 0.0;
+};
+
+namespace {
+
+int efail(imexsw_ensemble* e, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  if (e) e->err = buf;
+  else g_create_error = buf;
+  return -1;
+}
+
+#define EHIPCHK(e, call)                                                            \
+  do {                                                                              \
+    hipError_t e_ = (call);                                                         \
+    if (e_ != hipSuccess) return efail((e), "%s: %s", #call, hipGetErrorString(e_)); \
+  } while (0)
+
+BatchLists batch_lists(imexsw_ensemble* e) {
+  return BatchLists{e->d_tiles, e->d_rows, e->d_counts, e->d_tile_off, (int)e->members.size()};
+}
+
+template <int NS>
+void launch_batch_step(imexsw_ensemble* e) {
+  const BatchLists L = batch_lists(e);
+  const int M = L.members;
+  const int per = kTileH / e->strip_h;
+  const dim3 mgrid((M + 255) / 256);
+  hipLaunchKernelGGL(kb_reset, mgrid, dim3(256), 0, e->stream, e->d_params, e->d_ptrs, L);
+  hipLaunchKernelGGL(kb_mask, dim3(e->total_tiles), dim3(64), 0, e->stream, e->d_params,
+                     e->d_ptrs, L);
+  static int rs = 0, rl0 = 0, rl1 = 0;
+  if (!rs) rs = resident_grid(kb_stencil<NS>, 64, e->n_cu, 1 << 30);
+  if (!rl0) rl0 = resident_grid(kb_local<NS, 0>, 256, e->n_cu, 1 << 30);
+  if (!rl1) rl1 = resident_grid(kb_local<NS, 1>, 256, e->n_cu, 1 << 30);
+  hipLaunchKernelGGL((kb_stencil<NS>), dim3(std::min(e->total_tiles * per, rs)), dim3(64), 0,
+                     e->stream, e->d_params, e->d_ptrs, L, e->strip_h);
+  hipLaunchKernelGGL(kb_dtpolicy, mgrid, dim3(256), 0, e->stream, e->d_params, e->d_ptrs, M);
+  const int litems = (e->total_tiles * kTileH + 3) / 4;
+  hipLaunchKernelGGL((kb_local<NS, 0>), dim3(std::min(litems, rl0)), dim3(256), 0, e->stream,
+                     e->d_params, e->d_ptrs, L, (int)OP_RK);
+  hipLaunchKernelGGL((kb_local<NS, 1>), dim3(std::min(litems, rl1)), dim3(256), 0, e->stream,
+                     e->d_params, e->d_ptrs, L, (int)(OP_EROSION | OP_POST));
+  hipLaunchKernelGGL(kb_advance, mgrid, dim3(256), 0, e->stream, e->d_ptrs, M);
+}
+
+int batch_step(imexsw_ensemble* e) {
+  switch (e->ns) {
+    case 1: launch_batch_step<1>(e); break;
+    case 2: launch_batch_step<2>(e); break;
+    case 3: launch_batch_step<3>(e); break;
+    default: return efail(e, "ensembles support 1 to 3 solid classes");
+  }
+  hipError_t err = hipGetLastError();
+  if (err != hipSuccess) return efail(e, "batch launch: %s", hipGetErrorString(err));
+  return 0;
+}
+
+void drop_graph(imexsw_ensemble* e) {
+  if (e->exec) (void)hipGraphExecDestroy(e->exec);
+  if (e->graph) (void)hipGraphDestroy(e->graph);
+  e->exec = nullptr;
+  e->graph = nullptr;
+  e->graph_strip_h = 0;
+}
+
+// graph of graph_steps batched steps (kernels only: no memset nodes)
+int build_graph(imexsw_ensemble* e) {
+  drop_graph(e);
+  EHIPCHK(e, hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+  int rc = 0;
+  for (int s = 0; s < e->graph_steps && !rc; s++) rc = batch_step(e);
+  hipGraph_t g = nullptr;
+  hipError_t err = hipStreamEndCapture(e->stream, &g);
+  if (rc) {
+    if (g) (void)hipGraphDestroy(g);
+    return rc;
+  }
+  if (err != hipSuccess) return efail(e, "hipStreamEndCapture: %s", hipGetErrorString(err));
+  e->graph = g;
+  EHIPCHK(e, hipGraphInstantiate(&e->exec, g, nullptr, nullptr, 0));
+  e->graph_strip_h = e->strip_h;
+  return 0;
+}
+
+// member scalars for a run: t, t_end, t_output = t_end, not done, dt history
+int seed_members(imexsw_ensemble* e, double dt0, const double* t, double t_end) {
+  for (size_t m = 0; m < e->members.size(); m++) {
+    imexsw_ctx* c = e->members[m];
+    double s[S_NSLOTS];
+    if (get_scalars(c, s)) return efail(e, "member %zu: %s", m, c->err.c_str());
+    s[S_T] = t ? t[m] : s[S_T];
+    s[S_T_END] = t_end;
+    s[S_T_OUT] = t_end;
+    s[S_DONE] = 0.0;
+    if (!c->dt_hist) {
+      s[S_DT_OLD] = dt0;
+      s[S_DT_OLD_OLD] = dt0;
+      c->dt_hist = true;
+    }
+    EHIPCHK(e, hipMemcpy(c->D.scal, s, sizeof s, hipMemcpyHostToDevice));
+    c->d_valid = false;
+  }
+  return 0;
+}
+
+int run_steps(imexsw_ensemble* e, int nsteps, bool use_graph) {
+  if (use_graph && nsteps >= e->graph_steps &&
+      (!e->exec || e->graph_strip_h != e->strip_h) && build_graph(e))
+    return -1;
+  int done = 0;
+  while (use_graph && nsteps - done >= e->graph_steps) {
+    EHIPCHK(e, hipGraphLaunch(e->exec, e->stream));
+    done += e->graph_steps;
+  }
+  for (; done < nsteps; done++)
+    if (batch_step(e)) return -1;
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int imexsw_ensemble_create(int32_t n_members, const imexsw_params* params,
+                           imexsw_ensemble** out) {
+  if (!params || !out || n_members < 1) return efail(nullptr, "null argument or no members");
+  if (n_members > (1 << (31 - kMemberShift)))
+    return efail(nullptr, "at most %d members", 1 << (31 - kMemberShift));
+  *out = nullptr;
+  const int ns = params[0].n_solid;
+  if (ns < 1 || ns > 3) return efail(nullptr, "ensembles support 1 to 3 solid classes");
+  for (int m = 0; m < n_members; m++) {
+    const imexsw_params& p = params[m];
+    if (p.n_solid != ns) return efail(nullptr, "member %d: all members need the same n_solid", m);
+    if (p.ny < 2) return efail(nullptr, "member %d: ensembles need 2-D grids", m);
+    if (p.topo_change_flag || p.radial_source_flag || p.slope_correction_flag || p.curvature_term_flag)
+      return efail(nullptr, "member %d: ensembles run the flat-source-free model (no topography "
+                            "change, radial source, slope correction or curvature)", m);
+    const long tiles = (long)((p.nx + kStripW - 1) / kStripW) * ((p.ny + kTileH - 1) / kTileH);
+    if (tiles * kTileH > kIdMask) return efail(nullptr, "member %d: grid too large for an ensemble", m);
+  }
+  imexsw_ensemble* e = new imexsw_ensemble();
+  e->ns = ns;
+  std::vector<DevParams> hp(n_members);
+  std::vector<DevPtrs> hd(n_members);
+  std::vector<int> off(n_members + 1, 0);
+  for (int m = 0; m < n_members; m++) {
+    imexsw_ctx* c = nullptr;
+    if (imexsw_create(&params[m], &c)) {
+      std::string msg = g_create_error;
+      imexsw_ensemble_destroy(e);
+      return efail(nullptr, "member %d: %s", m, msg.c_str());
+    }
+    e->members.push_back(c);
+    hp[m] = c->P;
+    hd[m] = c->D;
+    off[m + 1] = off[m] + c->n_tiles;
+    e->n_cu = c->n_cu;
+  }
+  e->total_tiles = off[n_members];
+  int rc = 0;
+  auto alloc = [&](void** p, size_t bytes) {
+    if (!rc && hipMalloc(p, std::max<size_t>(bytes, 1)) != hipSuccess) rc = 1;
+  };
+  alloc((void**)&e->d_params, sizeof(DevParams) * n_members);
+  alloc((void**)&e->d_ptrs, sizeof(DevPtrs) * n_members);
+  alloc((void**)&e->d_tile_off, sizeof(int) * (n_members + 1));
+  alloc((void**)&e->d_tiles, sizeof(int) * e->total_tiles);
+  alloc((void**)&e->d_rows, sizeof(int) * (size_t)e->total_tiles * kTileH);
+  alloc((void**)&e->d_counts, sizeof(unsigned long long) * 2);
+  if (!rc && hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess) rc = 1;
+  if (!rc && (hipMemcpy(e->d_params, hp.data(), sizeof(DevParams) * n_members, hipMemcpyHostToDevice) != hipSuccess ||
+              hipMemcpy(e->d_ptrs, hd.data(), sizeof(DevPtrs) * n_members, hipMemcpyHostToDevice) != hipSuccess ||
+              hipMemcpy(e->d_tile_off, off.data(), sizeof(int) * (n_members + 1), hipMemcpyHostToDevice) != hipSuccess))
+    rc = 1;
+  if (rc) {
+    imexsw_ensemble_destroy(e);
+    return efail(nullptr, "ensemble allocation failed");
+  }
+  *out = e;
+  return 0;
+}
+
+void imexsw_ensemble_destroy(imexsw_ensemble* e) {
+  if (!e) return;
+  if (e->stream) (void)hipStreamSynchronize(e->stream);
+  drop_graph(e);
+  for (void* p : {(void*)e->d_params, (void*)e->d_ptrs, (void*)e->d_tile_off, (void*)e->d_tiles,
+                  (void*)e->d_rows, (void*)e->d_counts})
+    if (p) (void)hipFree(p);
+  if (e->stream) (void)hipStreamDestroy(e->stream);
+  for (imexsw_ctx* c : e->members) imexsw_destroy(c);
+  delete e;
+}
+
+const char* imexsw_ensemble_last_error(const imexsw_ensemble* e) {
+  return e ? e->err.c_str() : g_create_error.c_str();
+}
+
+imexsw_ctx* imexsw_ensemble_member(imexsw_ensemble* e, int32_t i) {
+  if (!e || i < 0 || i >= (int32_t)e->members.size()) return nullptr;
+  return e->members[i];
+}
+
+int imexsw_ensemble_set_tuning(imexsw_ensemble* e, int strip_rows, int graph_steps) {
+  if (!e) return efail(e, "null argument");
+  if (!(strip_rows == 4 || strip_rows == 8 || strip_rows == 16))
+    return efail(e, "strip_rows must be 4, 8 or 16 (got %d)", strip_rows);
+  if (graph_steps < 1 || graph_steps > 1000) return efail(e, "graph_steps must be 1..1000");
+  e->strip_h = strip_rows;
+  if (graph_steps != e->graph_steps) drop_graph(e);
+  e->graph_steps = graph_steps;
+  return 0;
+}
+
+int imexsw_ensemble_advance(imexsw_ensemble* e, int32_t nsteps, double dt0, double* t,
+                            double t_end, int32_t* steps_done) {
+  if (!e || nsteps < 0) return efail(e, "bad argument");
+  if (seed_members(e, dt0, t, t_end)) return -1;
+  hipEvent_t a, b;
+  EHIPCHK(e, hipEventCreate(&a));
+  EHIPCHK(e, hipEventCreate(&b));
+  (void)hipEventRecord(a, e->stream);
+  if (run_steps(e, nsteps, true)) return -1;
+  (void)hipEventRecord(b, e->stream);
+  EHIPCHK(e, hipStreamSynchronize(e->stream));
+  float ms = 0.f;
+  (void)hipEventElapsedTime(&ms, a, b);
+  e->last_ms = ms;
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  int64_t steps0 = -1;
+  for (size_t m = 0; m < e->members.size(); m++) {
+    imexsw_ctx* c = e->members[m];
+    double s[S_NSLOTS];
+    if (get_scalars(c, s)) return efail(e, "member %zu: %s", m, c->err.c_str());
+    if (t) t[m] = s[S_T];
+    if (m == 0) {
+      unsigned long long v = 0;
+      EHIPCHK(e, hipMemcpy(&v, c->D.counters + C_STEPS * kSlots, sizeof v, hipMemcpyDeviceToHost));
+      steps0 = (int64_t)v;
+    }
+  }
+  if (steps_done) *steps_done = (int32_t)steps0;
+  return 0;
+}
+
+// Try each strip height for `steps_per_trial` real steps (results do not
+// depend on the shape, so the trials advance the run) and keep the fastest.
+int imexsw_ensemble_autotune(imexsw_ensemble* e, int32_t steps_per_trial, double dt0,
+                             double* t, double t_end, int* best_strip_rows) {
+  if (!e || steps_per_trial < 1) return efail(e, "bad argument");
+  if (seed_members(e, dt0, t, t_end)) return -1;
+  const int shapes[3] = {4, 8, 16};
+  float best = 1e30f;
+  int best_h = e->strip_h;
+  hipEvent_t a, b;
+  EHIPCHK(e, hipEventCreate(&a));
+  EHIPCHK(e, hipEventCreate(&b));
+  for (int h : shapes) {
+    e->strip_h = h;
+    (void)hipEventRecord(a, e->stream);
+    if (run_steps(e, steps_per_trial, false)) return -1;
+    (void)hipEventRecord(b, e->stream);
+    EHIPCHK(e, hipStreamSynchronize(e->stream));
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, a, b);
+    if (ms < best) {
+      best = ms;
+      best_h = h;
+    }
+  }
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  e->strip_h = best_h;
+  if (best_strip_rows) *best_strip_rows = best_h;
+  if (t)
+    for (size_t m = 0; m < e->members.size(); m++) {
+      double s[S_NSLOTS];
+      if (get_scalars(e->members[m], s)) return efail(e, "member %zu", m);
+      t[m] = s[S_T];
+    }
+  return 0;
+}
+
+double imexsw_ensemble_last_ms(const imexsw_ensemble* e) { return e ? e->last_ms : 0.0; }
 
 }  // extern "C"

@@ -23,7 +23,12 @@ namespace imexsw {
 
 constexpr int kWave = 64;
 constexpr int kStripW = kWave - 4;  // interior columns per stencil wavefront
-constexpr int kTileX = 64, kTileY = 4;
+constexpr int kTileX = 64, kTileY = 4;  // layout-conversion kernels
+// Work tiles of the time step: kStripW x kTileH cells.  Lane l of a
+// wavefront owns column 60 * tile_x - 2 + l (lanes 2..61 are the tile's
+// cells, lanes 0, 1, 62, 63 its two-cell halo), so k_mask, k_stencil,
+// k_local and k_topo all see a tile the same way.
+constexpr int kTileH = 16;
 
 // ---------------------------------------------------------------------------
 // pointers of one context (device addresses)
@@ -56,14 +61,25 @@ struct DevPtrs {
   unsigned long long* dt_bits;  // CFL min (ordered bits of a positive double)
   double* scal;      // device scalars, see ScalarSlot
   unsigned long long* counters;  // see CounterSlot
-  double* tile_sums;  // 4 per local tile
+  double* tile_sums;  // 4 per work tile
+  uint8_t* tile_flag; // per work tile: any cell to compute
+  int* tile_list;     // work tiles with a cell to compute (compacted)
+  int* row_list;      // tile rows (tile * kTileH + row) with a cell to compute
 };
 
 enum ScalarSlot { S_T = 0, S_DT = 1, S_DT_OLD = 2, S_DT_OLD_OLD = 3, S_T_END = 4,
                   S_T_OUT = 5, S_DONE = 6, S_DT_CFL = 7, S_NSLOTS = 8 };
 enum CounterSlot { C_SOLVE_CELLS = 0, C_NEWTON_CELLS, C_NEWTON_ITERS, C_NONCONV,
-                   C_NAN, C_NEGH, C_NEGS, C_SUMS, C_LOWT, C_STEPS, C_ACTIVE_UPD,
-                   C_NCOUNTERS };
+                   C_NAN, C_NEGH, C_NEGS, C_SUMS, C_LOWT, C_LS_EVALS,
+                   C_HIST0, C_HIST1, C_HIST2, C_HIST3, C_HIST4, C_HIST5, C_HIST6,
+                   C_HIST7, C_ITMAX, C_STEPS, C_ACTIVE_UPD, C_COMPUTE_CELLS,
+                   C_TILES, C_ROWS, C_NCOUNTERS };
+// k_local's per-cell event counters, in CounterSlot order from C_NEWTON_CELLS
+constexpr int kLocalEvents = C_ITMAX - C_NEWTON_CELLS;  // summed ones
+// Newton iteration histogram buckets: 1, 2, 3, 4, 5-8, 9-16, 17-32, 33-100
+__device__ __forceinline__ int iter_bucket(int it) {
+  return it <= 4 ? it - 1 : it <= 8 ? 4 : it <= 16 ? 5 : it <= 32 ? 6 : 7;
+}
 // Every counter has kSlots copies, counters[m * kSlots + slot]; workgroups
 // add into slot (workgroup id mod kSlots) so that thousands of workgroups do
 // not serialise on one L2 line.  Totals are the sum over the slots.
@@ -110,83 +126,135 @@ __device__ __forceinline__ double shfl_d(double v, int src) {
 }
 
 // ---------------------------------------------------------------------------
-// k_mask: check_solve (src/solver_2d.f90:666-857).  One workgroup per
-// 64 x 16 tile; the wet/base flags of the tile and a halo of two cells are
-// staged in LDS, then the N_RK-fold 4-neighbour dilation (an L1 diamond of
-// radius N_RK) and the quiet test read LDS only.
+// k_mask: check_solve (src/solver_2d.f90:666-857), one workgroup per work
+// tile.  The seed and wet flags of the tile plus a two-cell halo are staged
+// in LDS (lane l loads column 60*tx-2+l of 20 rows), then the N_RK-fold
+// 4-neighbour dilation (an L1 diamond of radius N_RK) and the quiet test
+// read LDS only.  Tiles with a cell to compute are appended to tile_list.
 // ---------------------------------------------------------------------------
-constexpr int kMaskTX = 64, kMaskTY = 16, kMaskH = 2;
+// Where k_mask appends the tiles and tile rows that have a cell to compute:
+// entries are `tag | tile` and `tag | (tile * kTileH + row)`.
+struct ListSink {
+  int* tiles;
+  int* rows;
+  unsigned long long* n_tiles;
+  unsigned long long* n_rows;
+  int tag;
+};
 
-__global__ void __launch_bounds__(256) k_mask(DevParams P, DevPtrs D, int solve_all) {
-  constexpr int LW = kMaskTX + 2 * kMaskH, LH = kMaskTY + 2 * kMaskH;
-  __shared__ uint8_t base[LH][LW];  // seeds of the dilation
-  __shared__ uint8_t wet[LH][LW];   // h != 0 or a source cell
-  const int j0 = blockIdx.x * kMaskTX, k0 = blockIdx.y * kMaskTY;
-  for (int e = threadIdx.x; e < LW * LH; e += 256) {
-    const int r = e / LW, cc = e - r * LW;
-    const int jj = j0 - kMaskH + cc, kk = k0 - kMaskH + r;
-    uint8_t b = 0, w = 0;
-    if (jj >= 0 && jj < P.nx && kk >= 0 && kk < P.ny) {
-      const size_t c = cidx(P, jj, kk);
-      const bool bnd = (jj == 0 || jj == P.nx - 1 || kk == 0 || kk == P.ny - 1);
-      const double h = D.q[c];
-      const int sc = (P.radial && D.scell) ? D.scell[c] : 0;
-      b = (bnd || solve_all || h > 0.0 || sc == 2) ? 1 : 0;
-      w = (!(h == 0.0) || sc != 0) ? 1 : 0;
-    }
-    base[r][cc] = b;
-    wet[r][cc] = w;
+__device__ __forceinline__ ListSink own_sink(const DevPtrs& D) {
+  return ListSink{D.tile_list, D.row_list, ctr(D, C_TILES, 0), ctr(D, C_ROWS, 0), 0};
+}
+
+// Bit-parallel check_solve for one tile (tx, ty), one 64-lane wavefront:
+// lane l holds column 60*tx-2+l, so a 64-bit ballot is one tile row with its
+// two-cell halo on either side.  Per row the seed mask (h > 0, boundary,
+// source ring) dilated to the L1 diamond of radius N_RK = 2 -- the set the
+// reference's two 4-neighbour dilation passes produce -- is
+//   S(r) | S(r)<<>>1 | S(r)<<>>2 | S(r+-1) | S(r+-1)<<>>1 | S(r+-2)
+// and the quiet test ORs the wet masks over the cross of radius 2.
+__device__ __forceinline__ unsigned long long dil(unsigned long long m, int d) {
+  unsigned long long r = m;
+  if (d >= 1) r |= (m << 1) | (m >> 1);
+  if (d >= 2) r |= (m << 2) | (m >> 2);
+  return r;
+}
+
+__device__ __forceinline__ void mask_tile(const DevParams& P, const DevPtrs& D, int tx,
+                                          int tyy, int tiles_x, int solve_all,
+                                          const ListSink& L) {
+  constexpr int LH = kTileH + 4;
+  const int lane = threadIdx.x & 63;
+  const int j = tx * kStripW - 2 + lane;
+  const int k0 = tyy * kTileH;
+  const bool injx = (j >= 0 && j < P.nx);
+  const bool own = injx && lane >= 2 && lane < 2 + kStripW;
+  const bool radial = P.radial && D.scell;
+  // all staging loads first (rows k0-2 .. k0+17 of this column)
+  double hv[LH];
+  int scv[LH];
+#pragma unroll
+  for (int r = 0; r < LH; r++) {
+    const int kk = k0 - 2 + r;
+    const bool in = injx && kk >= 0 && kk < P.ny;
+    hv[r] = in ? D.q[cidx(P, j, kk)] : 0.0;
+    scv[r] = (in && radial) ? D.scell[cidx(P, j, kk)] : 0;
   }
-  __syncthreads();
-  const int tx = threadIdx.x & 63;
-  const int R = P.nrk;
+  const unsigned long long own_m = __ballot(own);
+  const unsigned long long edge_m = __ballot(injx && (j == 0 || j == P.nx - 1));
+  const unsigned long long west_m = __ballot(injx && j <= 1);
+  const unsigned long long east_m = __ballot(injx && j >= P.nx - 2);
+  // rolling window of seed / wet / source-interior masks, rows r-2 .. r+2
+  unsigned long long Sm[5], Wm[5], Im[5];
+  auto row_masks = [&](int r, unsigned long long& s, unsigned long long& w,
+                       unsigned long long& in1) {
+    const int kk = k0 - 2 + r;
+    const bool in = injx && kk >= 0 && kk < P.ny;
+    const bool bnd = in && (j == 0 || j == P.nx - 1 || kk == 0 || kk == P.ny - 1);
+    s = __ballot(in && (bnd || solve_all || hv[r] > 0.0 || scv[r] == 2));
+    w = __ballot(in && (!(hv[r] == 0.0) || scv[r] != 0));
+    in1 = __ballot(in && scv[r] == 1);
+  };
+#pragma unroll
+  for (int r = 0; r < 4; r++) row_masks(r, Sm[r + 1], Wm[r + 1], Im[r + 1]);
   const size_t pl = plane(P);
-  int count = 0;
-  for (int ty = threadIdx.x >> 6; ty < kMaskTY; ty += 4) {
-    const int j = j0 + tx, k = k0 + ty;
-    if (j >= P.nx || k >= P.ny) continue;
-    const size_t c = cidx(P, j, k);
-    const int lr = ty + kMaskH, lc = tx + kMaskH;
-    const bool bnd = (j == 0 || j == P.nx - 1 || k == 0 || k == P.ny - 1);
-    int active = bnd ? 1 : 0;
-    if (!active) {
-      for (int dk = -R; dk <= R && !active; dk++) {
-        const int rem = R - (dk < 0 ? -dk : dk);
-        for (int dj = -rem; dj <= rem; dj++)
-          if (base[lr + dk][lc + dj]) {
-            active = 1;
-            break;
-          }
-      }
-    }
-    if (P.radial && D.scell && D.scell[c] == 1) active = 0;
-    int compute = 0;
-    if (active) {
-      bool quiet = true;
+  int count = 0, ccount = 0;
+  unsigned rows_m = 0;
 #pragma unroll
-      for (int d = -2; d <= 2; d++) quiet = quiet && !wet[lr][lc + d];
-      if (P.ny > 1) {
+  for (int rr = 0; rr < kTileH; rr++) {
 #pragma unroll
-        for (int d = -2; d <= 2; d++) quiet = quiet && !wet[lr + d][lc];
-      }
-      if (j <= 1) quiet = quiet && P.bc_quiet[IMEXSW_W];
-      if (j >= P.nx - 2) quiet = quiet && P.bc_quiet[IMEXSW_E];
-      if (P.ny > 1) {
-        if (k <= 1) quiet = quiet && P.bc_quiet[IMEXSW_S];
-        if (k >= P.ny - 2) quiet = quiet && P.bc_quiet[IMEXSW_N];
-      }
-      if (quiet)
-        for (int i = 1; i < P.nv; i++) quiet = quiet && (D.q[i * pl + c] == 0.0);
-      compute = quiet ? 0 : 1;
+    for (int i = 0; i < 4; i++) {
+      Sm[i] = Sm[i + 1];
+      Wm[i] = Wm[i + 1];
+      Im[i] = Im[i + 1];
     }
-    D.mask[c] = (uint8_t)((active ? M_ACTIVE : 0) | (compute ? M_COMPUTE : 0));
-    count += active;
+    row_masks(rr + 4, Sm[4], Wm[4], Im[4]);
+    const int k = k0 + rr;
+    if (k >= P.ny) break;
+    // Sm[2] is row k
+    unsigned long long act = dil(Sm[2], 2) | dil(Sm[1], 1) | dil(Sm[3], 1) | Sm[0] | Sm[4];
+    act |= edge_m;
+    if (k == 0 || k == P.ny - 1) act = ~0ull;
+    act &= own_m & ~Im[2];
+    unsigned long long busy = dil(Wm[2], 2);
+    if (P.ny > 1) busy |= Wm[0] | Wm[1] | Wm[3] | Wm[4];
+    if (!P.bc_quiet[IMEXSW_W]) busy |= west_m;
+    if (!P.bc_quiet[IMEXSW_E]) busy |= east_m;
+    if (P.ny > 1 && ((k <= 1 && !P.bc_quiet[IMEXSW_S]) || (k >= P.ny - 2 && !P.bc_quiet[IMEXSW_N])))
+      busy = ~0ull;
+    // quiet candidates: active, dry cross; their own state must be all zero
+    const bool cand = (((act & ~busy) >> lane) & 1ull) != 0;
+    bool nz = false;
+    if (cand) {
+      const size_t c = cidx(P, j, k);
+      for (int i = 1; i < P.nv; i++) nz = nz || !(D.q[i * pl + c] == 0.0);
+    }
+    const unsigned long long comp = act & (busy | __ballot(nz));
+    if (own) {
+      const int a = (int)((act >> lane) & 1ull), cm = (int)((comp >> lane) & 1ull);
+      D.mask[cidx(P, j, k)] = (uint8_t)((a ? M_ACTIVE : 0) | (cm ? M_COMPUTE : 0));
+    }
+    count += __popcll(act);
+    ccount += __popcll(comp);
+    if (comp) rows_m |= 1u << rr;
   }
-  // count of active cells (solve_cells): wave sum, one atomic per wave
-  for (int off = 32; off > 0; off >>= 1) count += __shfl_xor(count, off, 64);
-  if ((threadIdx.x & 63) == 0 && count)
-    atomicAdd(ctr(D, C_SOLVE_CELLS, blockIdx.y * gridDim.x + blockIdx.x),
-              (unsigned long long)count);
+  const int tile = tyy * tiles_x + tx;
+  if (lane == 0) {
+    if (count) atomicAdd(ctr(D, C_SOLVE_CELLS, tile), (unsigned long long)count);
+    if (ccount) atomicAdd(ctr(D, C_COMPUTE_CELLS, tile), (unsigned long long)ccount);
+    const int nrows = __popc(rows_m);
+    D.tile_flag[tile] = nrows ? 1 : 0;
+    if (nrows) {
+      L.tiles[atomicAdd(L.n_tiles, 1ull)] = L.tag | tile;
+      int at = (int)atomicAdd(L.n_rows, (unsigned long long)nrows);
+      for (int r = 0; r < kTileH; r++)
+        if (rows_m & (1u << r)) L.rows[at++] = L.tag | (tile * kTileH + r);
+    }
+  }
+}
+
+__global__ void __launch_bounds__(64) k_mask(DevParams P, DevPtrs D, int solve_all) {
+  mask_tile(P, D, blockIdx.x, blockIdx.y, gridDim.x, solve_all, own_sink(D));
 }
 
 // ---------------------------------------------------------------------------
@@ -418,7 +486,7 @@ __device__ __forceinline__ void ring_state(const DevParams& P, const DevPtrs& D,
 // k_stencil: the stage-1 stencil pass (timestep + eval_hyperbolic_terms +
 // eval_expl_terms at q^n).  One wavefront marches up a strip of rows keeping
 // a three-row window of physical states in registers; lane l owns column
-// j = 60*blockIdx.x - 2 + l.  Lanes 2..61 are the strip's cells, lanes 1 and
+// j = 60*tile_x - 2 + l.  Lanes 2..61 are the strip's cells, lanes 1 and
 // 62 reconstruct the halo sides of the strip's outer faces, lanes 0 and 63
 // only supply states.  x neighbours are exchanged with wave shuffles; rows
 // k0-2 .. k1+1 are read once each.
@@ -462,32 +530,51 @@ __device__ __forceinline__ void yface_sides(const DevParams& P, const double* hi
 }
 
 template <int NS, bool HASY>
+__device__ __forceinline__ void stencil_strip(const DevParams& P, const DevPtrs& D, int j,
+                                              int k0, int k1, int compute_dt);
+
+// The stencil pass over the listed work tiles.  Each tile is cut into
+// strips of strip_h rows, strip_h = unit * 2^m <= max_strip (a divisor of
+// the tile height): the tallest that still leaves about `target_strips`
+// strips for this step's computed cells (short strips when few cells are
+// computed, so the sequential march stays short; tall strips on a wet grid,
+// so the two halo rows of each strip are amortised).  Workgroups take
+// (tile, strip) items in turn until the list is exhausted.
+template <int NS, bool HASY>
 __global__ void __launch_bounds__(64) k_stencil(DevParams P, DevPtrs D, int unit,
                                                 int max_strip, int target_strips,
                                                 int compute_dt) {
-  constexpr int NV = 4 + NS;
   const int lane = threadIdx.x;
-  const int j = blockIdx.x * kStripW - 2 + lane;
-  // Strip height: the grid has one workgroup per `unit` rows; the height
-  // actually used is the largest unit * 2^m <= max_strip that still leaves
-  // about `target_strips` strips for this step's active cells (short strips
-  // when few cells are active, so the sequential march stays short; tall
-  // strips on a wet grid, so the halo rows of each strip are amortised).
-  // Workgroups whose first row is not a strip start exit.
+  if (D.scal[S_DONE] != 0.0) return;
   int strip_h = unit;
-  if (unit > 1) {
-    unsigned long long w = *ctr(D, C_SOLVE_CELLS, lane);
+  if (P.ny > 1) {
+    unsigned long long w = *ctr(D, C_COMPUTE_CELLS, lane);
     for (int off = 32; off > 0; off >>= 1) w += __shfl_xor(w, off, 64);
     while (2 * strip_h <= max_strip &&
            w / (unsigned long long)(kStripW * 2 * strip_h) >= (unsigned long long)target_strips)
       strip_h *= 2;
   }
-  const int k0 = blockIdx.y * unit;
-  if (k0 % strip_h) return;
-  const int k1 = min(k0 + strip_h, P.ny);
+  const int per = kTileH / strip_h;
+  const int ntiles = (int)*ctr(D, C_TILES, 0);
+  const int tiles_x = (P.nx + kStripW - 1) / kStripW;
+  for (int item = blockIdx.x; item < ntiles * per; item += gridDim.x) {
+    const int tile = D.tile_list[item / per];
+    const int tx = tile % tiles_x, tyy = tile / tiles_x;
+    const int k0 = tyy * kTileH + (item % per) * strip_h;
+    if (k0 >= P.ny) continue;
+    const int k1 = min(k0 + strip_h, P.ny);
+    stencil_strip<NS, HASY>(P, D, tx * kStripW - 2 + lane, k0, k1, compute_dt);
+  }
+}
+
+// One strip: lane l owns column j = 60 * tile_x - 2 + l, rows k0 .. k1-1.
+template <int NS, bool HASY>
+__device__ __forceinline__ void stencil_strip(const DevParams& P, const DevPtrs& D, int j,
+                                              int k0, int k1, int compute_dt) {
+  constexpr int NV = 4 + NS;
+  const int lane = threadIdx.x;
   const bool injx = (j >= 0 && j < P.nx);
   const bool own = (lane >= 2 && lane < 2 + kStripW && injx);
-  if (D.scal[S_DONE] != 0.0) return;
   const double t = D.scal[S_T];
 
   // skip strips without a cell to compute
@@ -707,7 +794,7 @@ __global__ void __launch_bounds__(64) k_stencil(DevParams P, DevPtrs D, int unit
 // ---------------------------------------------------------------------------
 // k_dtpolicy: src/IMEX_SfloW2D.f90:357-381 on the device (one thread)
 // ---------------------------------------------------------------------------
-__global__ void k_dtpolicy(DevParams P, DevPtrs D) {
+__device__ __forceinline__ void dtpolicy_member(const DevParams& P, const DevPtrs& D) {
   double* s = D.scal;
   if (s[S_DONE] != 0.0) return;
   double t = s[S_T], t_end = s[S_T_END], t_out = s[S_T_OUT];
@@ -726,9 +813,10 @@ __global__ void k_dtpolicy(DevParams P, DevPtrs D) {
   s[S_DT] = dt;
   s[S_T_OUT] = t_out;
 }
+__global__ void k_dtpolicy(DevParams P, DevPtrs D) { dtpolicy_member(P, D); }
 
-// t = t + dt, reset of the CFL accumulator and the solve-cells counter
-__global__ void k_advance_time(DevPtrs D) {
+// t = t + dt, step count and active cell-updates
+__device__ __forceinline__ void advance_member(const DevPtrs& D) {
   double* s = D.scal;
   if (s[S_DONE] != 0.0) return;
   s[S_T] = s[S_T] + s[S_DT];
@@ -737,11 +825,19 @@ __global__ void k_advance_time(DevPtrs D) {
   *ctr(D, C_STEPS, 0) += 1;
   *ctr(D, C_ACTIVE_UPD, 0) += n;
 }
+__global__ void k_advance_time(DevPtrs D) { advance_member(D); }
 
-__global__ void k_reset_step(DevParams P, DevPtrs D) {
+// CFL accumulator and per-step counters / work lists
+__device__ __forceinline__ void reset_member(const DevParams& P, const DevPtrs& D) {
   *D.dt_bits = (unsigned long long)__double_as_longlong(P.max_dt);
-  for (int i = 0; i < kSlots; i++) *ctr(D, C_SOLVE_CELLS, i) = 0;
+  for (int i = 0; i < kSlots; i++) {
+    *ctr(D, C_SOLVE_CELLS, i) = 0;
+    *ctr(D, C_COMPUTE_CELLS, i) = 0;
+  }
+  *ctr(D, C_TILES, 0) = 0;
+  *ctr(D, C_ROWS, 0) = 0;
 }
+__global__ void k_reset_step(DevParams P, DevPtrs D) { reset_member(P, D); }
 
 // ---------------------------------------------------------------------------
 // k_local: stage 2 + assembly (imex_RK_solver :1105-1626), erosion /
@@ -750,197 +846,224 @@ __global__ void k_reset_step(DevParams P, DevPtrs D) {
 // ---------------------------------------------------------------------------
 enum LocalOps { OP_RK = 1, OP_EROSION = 2, OP_POST = 4, OP_SUMS = 8 };
 
-template <int NS>
-__global__ void __launch_bounds__(256) k_local(DevParams P, DevPtrs D, int ops) {
+// one cell of k_local; e (the wavefront's counters in LDS) accumulates
+// counter C_NEWTON_CELLS + m in e[m]
+// PH selects the part compiled in: 0 = stage update and assembly checks
+// (OP_RK), 1 = erosion/deposition and diagnostics (OP_EROSION, OP_POST,
+// OP_SUMS).  Split in two kernels, each part keeps a register footprint
+// that lets two wavefronts share a SIMD.
+template <int NS, int PH>
+__device__ __forceinline__ void local_cell(const DevParams& P, const DevPtrs& D, size_t c,
+                                           int ops, unsigned long long* e, int& imax,
+                                           double& s_mass, double& s_vol, double& s_area,
+                                           double& s_sol) {
   constexpr int NV = 4 + NS;
-  const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
-  const int j = blockIdx.x * kTileX + tx;
-  const int k = blockIdx.y * kTileY + ty;
-  const bool inb = (j < P.nx && k < P.ny);
-  const size_t c = inb ? cidx(P, j, k) : 0;
-  const bool act = inb && (D.mask[c] & M_COMPUTE) && (D.scal[S_DONE] == 0.0);
-  __shared__ double red[4][4];
-  __shared__ unsigned long long cnt[4][8];
-  double s_mass = 0.0, s_vol = 0.0, s_area = 0.0, s_sol = 0.0;
-  unsigned long long e[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-  if (act) {
-    const size_t pl = plane(P);
-    const double dt = D.scal[S_DT];
-    double q[NV], qp[NV + 2];
+  const size_t pl = plane(P);
+  const double dt = D.scal[S_DT];
+  double q[NV], qp[NV + 2];
 #pragma unroll
-    for (int i = 0; i < NV; i++) q[i] = D.q[i * pl + c];
-    if (ops & OP_RK) {
-      double q0[NV], Dr[NV];
+  for (int i = 0; i < NV; i++) q[i] = D.q[i * pl + c];
+  if (PH == 0 && (ops & OP_RK)) {
+    double q0[NV], Dr[NV];
 #pragma unroll
-      for (int i = 0; i < NV; i++) {
-        q0[i] = q[i];
-        Dr[i] = D.D[i * pl + c];
-      }
-      const double bx = D.Bx[c], by = D.By[c];
-      const double bxx = D.Bxx ? D.Bxx[c] : 0.0, bxy = D.Bxy ? D.Bxy[c] : 0.0,
-                   byy = D.Byy ? D.Byy[c] : 0.0;
-      const double gc = D.gcoef ? D.gcoef[c] : 1.0;
-      int nonconv = 0;
-      int it = stage2<NS>(P, q0, Dr, dt, bx, by, bxx, bxy, byy, gc, q, nonconv);
-      if (it > 0) {
-        e[0] += 1;
-        e[1] += it;
-        e[2] += nonconv;
-      }
-      // assembly checks (:1379-1624)
-      bool isn = false;
+    for (int i = 0; i < NV; i++) {
+      q0[i] = q[i];
+      Dr[i] = D.D[i * pl + c];
+    }
+    const double bx = D.Bx[c], by = D.By[c];
+    const double bxx = D.Bxx ? D.Bxx[c] : 0.0, bxy = D.Bxy ? D.Bxy[c] : 0.0,
+                 byy = D.Byy ? D.Byy[c] : 0.0;
+    const double gc = D.gcoef ? D.gcoef[c] : 1.0;
+    int nonconv = 0, ls = 0;
+    int it = stage2<NS>(P, q0, Dr, dt, bx, by, bxx, bxy, byy, gc, q, nonconv, ls);
+    if (it > 0) {
+      atomicAdd(&e[C_NEWTON_CELLS - C_NEWTON_CELLS], 1ull);
+      atomicAdd(&e[C_NEWTON_ITERS - C_NEWTON_CELLS], (unsigned long long)it);
+      if (nonconv) atomicAdd(&e[C_NONCONV - C_NEWTON_CELLS], 1ull);
+      if (ls) atomicAdd(&e[C_LS_EVALS - C_NEWTON_CELLS], (unsigned long long)ls);
+      const int b = iter_bucket(it);
+      atomicAdd(&e[C_HIST0 - C_NEWTON_CELLS + b], 1ull);
+      imax = max(imax, it);
+    }
+    // assembly checks (:1379-1624)
+    bool isn = false;
 #pragma unroll
-      for (int i = 0; i < NV; i++) isn |= isnan(q[i]);
-      e[3] += isn;
-      if (q[0] < kEps) {
-        if (q[0] > -1.0e-7) {
-#pragma unroll
-          for (int i = 0; i < NV; i++) q[i] = 0.0;
-        } else {
-          e[4] += 1;
-        }
-      }
-      {
-        bool anyneg = false, anybig = false;
-#pragma unroll
-        for (int i = 0; i < NS; i++) {
-          anyneg |= q[4 + i] < 0.0;
-          anybig |= q[4 + i] <= -1.0e-7;
-        }
-        if (anyneg) {
-          if (anybig) e[5] += 1;
-          else {
-#pragma unroll
-            for (int i = 0; i < NS; i++)
-              if (q[4 + i] > -1.0e-7) q[4 + i] = 0.0;
-          }
-        }
-      }
-      phys<NS>(P, q, qp);
-      if (qp[3] < 273.0) e[7] += 1;
-      double ssol = 0.0;
-#pragma unroll
-      for (int i = 0; i < NS; i++) ssol += q[4 + i];
-      if (ssol > q[0]) {
-        if (((ssol - q[0]) / q[0] < 1.0e-10) || (q[0] < kEps)) {
-#pragma unroll
-          for (int i = 0; i < NS; i++) q[4 + i] = q[4 + i] / ssol * q[0];
-        } else {
-          e[6] += 1;
-        }
-      }
-      if (D.nodata && D.nodata[c]) {
+    for (int i = 0; i < NV; i++) isn |= isnan(q[i]);
+    if (isn) atomicAdd(&e[C_NAN - C_NEWTON_CELLS], 1ull);
+    if (q[0] < kEps) {
+      if (q[0] > -1.0e-7) {
 #pragma unroll
         for (int i = 0; i < NV; i++) q[i] = 0.0;
+      } else {
+        atomicAdd(&e[C_NEGH - C_NEWTON_CELLS], 1ull);
       }
     }
-    if ((ops & OP_EROSION) &&
-        !((P.erosion_coeff == 0.0) && !P.settling && !P.entrain)) {
-      phys_or_dry<NS>(P, q, qp);
-      double er[NS > 0 ? NS : 1], dep[NS > 0 ? NS : 1], eq[NV], topo;
-      double erod[NS > 0 ? NS : 1];
-#pragma unroll
-      for (int i = 0; i < NS; i++) erod[i] = D.erodible[i * pl + c];
-      mass_exchange<NS>(P, qp, D.Bx[c], D.By[c], erod, dt, er, dep, eq, topo);
-#pragma unroll
-      for (int i = 0; i < NV; i++) q[i] = q[i] + dt * eq[i];
-#pragma unroll
-      for (int i = 0; i < NS; i++) q[4 + i] = fmax(0.0, q[4 + i]);
+    {
+      bool anyneg = false, anybig = false;
 #pragma unroll
       for (int i = 0; i < NS; i++) {
-        D.deposit[i * pl + c] = D.deposit[i * pl + c] + dt * dep[i];
-        D.erosion[i * pl + c] = D.erosion[i * pl + c] + dt * er[i];
-        double ev = erod[i] - dt * er[i];
-        if (P.erod_dep) ev = ev + dt * dep[i];
-        D.erodible[i * pl + c] = ev;
+        anyneg |= q[4 + i] < 0.0;
+        anybig |= q[4 + i] <= -1.0e-7;
       }
-      if (P.topo) D.B[c] = D.B[c] + dt * topo;
-      if (q[0] <= 0.0 && q[0] > -1.0e-10) {
+      if (anyneg) {
+        if (anybig) atomicAdd(&e[C_NEGS - C_NEWTON_CELLS], 1ull);
+        else {
 #pragma unroll
-        for (int i = 0; i < NV; i++) q[i] = 0.0;
+          for (int i = 0; i < NS; i++)
+            if (q[4 + i] > -1.0e-7) q[4 + i] = 0.0;
+        }
       }
-      double ssol = 0.0;
+    }
+    phys<NS>(P, q, qp);
+    if (qp[3] < 273.0) atomicAdd(&e[C_LOWT - C_NEWTON_CELLS], 1ull);
+    double ssol = 0.0;
 #pragma unroll
-      for (int i = 0; i < NS; i++) ssol += q[4 + i];
-      if (ssol > q[0] && q[0] < 1.0e-10) {
+    for (int i = 0; i < NS; i++) ssol += q[4 + i];
+    if (ssol > q[0]) {
+      if (((ssol - q[0]) / q[0] < 1.0e-10) || (q[0] < kEps)) {
 #pragma unroll
         for (int i = 0; i < NS; i++) q[4 + i] = q[4 + i] / ssol * q[0];
-      }
-      double rg = 0.0;
-      if (q[0] > 0.0) {
-        phys<NS>(P, q, qp);
-        rg = mixt<NS, false>(P, qp).rg;
-      }
-      if (rg <= 0.0) {
-#pragma unroll
-        for (int i = 0; i < NV; i++) q[i] = 0.0;
+      } else {
+        atomicAdd(&e[C_SUMS - C_NEWTON_CELLS], 1ull);
       }
     }
-    if (ops & (OP_RK | OP_EROSION)) {
+    if (D.nodata && D.nodata[c]) {
 #pragma unroll
-      for (int i = 0; i < NV; i++) D.q[i * pl + c] = q[i];
+      for (int i = 0; i < NV; i++) q[i] = 0.0;
     }
-    if (ops & OP_POST) {
-      double pd = phys_or_dry<NS>(P, q, qp);
-      if (q[0] > 0.0) {
-        D.hmax[c] = fmax(D.hmax[c], qp[0]);
-        const double mv = sqrt(qp[NV] * qp[NV] + qp[NV + 1] * qp[NV + 1]);
-        if (qp[0] > 0.001) {
-          D.pdynmax[c] = fmax(D.pdynmax[c], pd);
-          D.modvel[c] = fmax(D.modvel[c], mv);
-        }
-        if (D.vuln) {
-          uint32_t bits = D.vuln[c];
-          int it = 0;
-          for (int a = 0; a < P.n_thk; a++) {
-            const bool th = qp[0] >= P.thickness_levels[a];
-            for (int b = 0; b < P.n_pdyn; b++) {
-              const bool pdb = pd >= P.dyn_pres_levels[b];
-              if (th && pdb) bits |= (1u << it);
-              it++;
-            }
+  }
+  if (PH == 1 && (ops & OP_EROSION) &&
+      !((P.erosion_coeff == 0.0) && !P.settling && !P.entrain)) {
+    phys_or_dry<NS>(P, q, qp);
+    double er[NS > 0 ? NS : 1], dep[NS > 0 ? NS : 1], eq[NV], topo;
+    double erod[NS > 0 ? NS : 1];
+#pragma unroll
+    for (int i = 0; i < NS; i++) erod[i] = D.erodible[i * pl + c];
+    mass_exchange<NS>(P, qp, D.Bx[c], D.By[c], erod, dt, er, dep, eq, topo);
+#pragma unroll
+    for (int i = 0; i < NV; i++) q[i] = q[i] + dt * eq[i];
+#pragma unroll
+    for (int i = 0; i < NS; i++) q[4 + i] = fmax(0.0, q[4 + i]);
+#pragma unroll
+    for (int i = 0; i < NS; i++) {
+      D.deposit[i * pl + c] = D.deposit[i * pl + c] + dt * dep[i];
+      D.erosion[i * pl + c] = D.erosion[i * pl + c] + dt * er[i];
+      double ev = erod[i] - dt * er[i];
+      if (P.erod_dep) ev = ev + dt * dep[i];
+      D.erodible[i * pl + c] = ev;
+    }
+    if (P.topo) D.B[c] = D.B[c] + dt * topo;
+    if (q[0] <= 0.0 && q[0] > -1.0e-10) {
+#pragma unroll
+      for (int i = 0; i < NV; i++) q[i] = 0.0;
+    }
+    double ssol = 0.0;
+#pragma unroll
+    for (int i = 0; i < NS; i++) ssol += q[4 + i];
+    if (ssol > q[0] && q[0] < 1.0e-10) {
+#pragma unroll
+      for (int i = 0; i < NS; i++) q[4 + i] = q[4 + i] / ssol * q[0];
+    }
+    double rg = 0.0;
+    if (q[0] > 0.0) {
+      phys<NS>(P, q, qp);
+      rg = mixt<NS, false>(P, qp).rg;
+    }
+    if (rg <= 0.0) {
+#pragma unroll
+      for (int i = 0; i < NV; i++) q[i] = 0.0;
+    }
+  }
+  if (ops & (OP_RK | OP_EROSION)) {
+#pragma unroll
+    for (int i = 0; i < NV; i++) D.q[i * pl + c] = q[i];
+  }
+  if (PH == 1 && (ops & OP_POST)) {
+    double pd = phys_or_dry<NS>(P, q, qp);
+    if (q[0] > 0.0) {
+      D.hmax[c] = fmax(D.hmax[c], qp[0]);
+      const double mv = sqrt(qp[NV] * qp[NV] + qp[NV + 1] * qp[NV + 1]);
+      if (qp[0] > 0.001) {
+        D.pdynmax[c] = fmax(D.pdynmax[c], pd);
+        D.modvel[c] = fmax(D.modvel[c], mv);
+      }
+      if (D.vuln) {
+        uint32_t bits = D.vuln[c];
+        int it = 0;
+        for (int a = 0; a < P.n_thk; a++) {
+          const bool th = qp[0] >= P.thickness_levels[a];
+          for (int b = 0; b < P.n_pdyn; b++) {
+            const bool pdb = pd >= P.dyn_pres_levels[b];
+            if (th && pdb) bits |= (1u << it);
+            it++;
           }
-          D.vuln[c] = bits;
         }
-      }
-      if (ops & OP_SUMS) {
-        s_mass = q[0];
-        s_vol = qp[0];
-        s_area = (q[0] > 1.0e-7) ? 1.0 : 0.0;
-#pragma unroll
-        for (int i = 0; i < NS; i++) s_sol += q[4 + i];
+        D.vuln[c] = bits;
       }
     }
-  }
-  // counters: wave reduction, one atomic per wave and counter
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (ops & OP_SUMS) {
+      s_mass += q[0];
+      s_vol += qp[0];
+      s_area += (q[0] > 1.0e-7) ? 1.0 : 0.0;
 #pragma unroll
-  for (int m = 0; m < 8; m++) {
-    unsigned long long v = e[m];
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-    if (lane == 0) cnt[w][m] = v;
+      for (int i = 0; i < NS; i++) s_sol += q[4 + i];
+    }
   }
-  if (ops & OP_SUMS) {
+}
+
+// One tile row (60 cells, one per lane 2..61) of k_local; per-row partial
+// sums go to tile_sums[4 * row ..] when OP_SUMS.
+template <int NS, int PH>
+__device__ __forceinline__ void local_row(const DevParams& P, const DevPtrs& D, int row,
+                                          int ops, unsigned long long* e, int& imax) {
+  const int lane = threadIdx.x & 63;
+  const int tiles_x = (P.nx + kStripW - 1) / kStripW;
+  const int tile = row / kTileH;
+  const int j = (tile % tiles_x) * kStripW - 2 + lane;
+  const int k = (tile / tiles_x) * kTileH + row % kTileH;
+  const bool own = lane >= 2 && lane < 2 + kStripW && j < P.nx && k < P.ny;
+  double s_mass = 0.0, s_vol = 0.0, s_area = 0.0, s_sol = 0.0;
+  if (own) {
+    const size_t c = cidx(P, j, k);
+    if (D.mask[c] & M_COMPUTE)
+      local_cell<NS, PH>(P, D, c, ops, e, imax, s_mass, s_vol, s_area, s_sol);
+  }
+  if (PH == 1 && (ops & OP_SUMS)) {
     double v4[4] = {s_mass, s_vol, s_area, s_sol};
 #pragma unroll
     for (int m = 0; m < 4; m++) {
       double v = v4[m];
       for (int off = 32; off > 0; off >>= 1) v += shfl_d(v, lane ^ off);
-      if (lane == 0) red[w][m] = v;
+      if (lane == 0) D.tile_sums[4 * (size_t)row + m] = v;
     }
   }
+}
+
+// One wavefront per listed tile row; the four wavefronts of a workgroup
+// take rows in turn until the list is exhausted.
+template <int NS, int PH>
+__global__ void __launch_bounds__(256) k_local(DevParams P, DevPtrs D, int ops) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  __shared__ unsigned long long cnt[4][kLocalEvents];
+  __shared__ int itmax[4];
+  for (int m = lane; m < kLocalEvents; m += 64) cnt[w][m] = 0;
+  __syncthreads();
+  int imax = 0;
+  const bool live = (D.scal[S_DONE] == 0.0);
+  const int nrows = (int)*ctr(D, C_ROWS, 0);
+  for (int it = blockIdx.x * 4 + w; live && it < nrows; it += gridDim.x * 4)
+    local_row<NS, PH>(P, D, D.row_list[it], ops, cnt[w], imax);
+  for (int off = 32; off > 0; off >>= 1) imax = max(imax, __shfl_xor(imax, off, 64));
+  if (lane == 0) itmax[w] = imax;
   __syncthreads();
   if (threadIdx.x == 0) {
-    const int slot = blockIdx.y * gridDim.x + blockIdx.x;
-    for (int m = 0; m < 8; m++) {
+    const int slot = blockIdx.x;
+    for (int m = 0; m < kLocalEvents; m++) {
       unsigned long long v = cnt[0][m] + cnt[1][m] + cnt[2][m] + cnt[3][m];
       if (v) atomicAdd(ctr(D, C_NEWTON_CELLS + m, slot), v);
     }
-    if (ops & OP_SUMS) {
-      const size_t tile = (size_t)blockIdx.y * gridDim.x + blockIdx.x;
-      for (int m = 0; m < 4; m++)
-        D.tile_sums[4 * tile + m] = ((red[0][m] + red[1][m]) + red[2][m]) + red[3][m];
-    }
+    const int im = max(max(itmax[0], itmax[1]), max(itmax[2], itmax[3]));
+    if (im) atomicMax(ctr(D, C_ITMAX, slot), (unsigned long long)im);
   }
 }
 
@@ -951,26 +1074,37 @@ __global__ void __launch_bounds__(256) k_local(DevParams P, DevPtrs D, int ops) 
 // four cells of a computed cell are recomputed: B_cent changes only where
 // the stage update runs.
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(256) k_topo(DevParams P, DevPtrs D, int sparse) {
-  const int j = blockIdx.x * kTileX + (threadIdx.x & 63);
-  const int k = blockIdx.y * kTileY + (threadIdx.x >> 6);
+__device__ __forceinline__ void topo_cell(const DevParams& P, const DevPtrs& D, int j, int k);
+
+// one workgroup per work tile (lane l: column 60*tx-2+l, four rows per
+// wavefront); when `sparse`, only tiles with a computed cell in the tile or
+// a neighbour tile (a computed cell within four cells)
+__device__ __forceinline__ void topo_tile(const DevParams& P, const DevPtrs& D, int tx,
+                                          int ty, int tiles_x, int tiles_y, int sparse) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   if (sparse) {
-    // any active cell within 4 cells (5-point stencil around a clamped
-    // centre)
-    int any = 0;
-    if (j < P.nx && k < P.ny) {
-      for (int dk = -4; dk <= 4 && !any; dk += 4)
-        for (int dj = -4; dj <= 4; dj += 4) {
-          int jj = min(max(j + dj, 0), P.nx - 1), kk = min(max(k + dk, 0), P.ny - 1);
-          if (D.mask[cidx(P, jj, kk)] & M_COMPUTE) {
-            any = 1;
-            break;
-          }
-        }
-    }
-    if (!__syncthreads_or(any)) return;
+    bool any = false;
+    for (int dy = -1; dy <= 1; dy++)
+      for (int dx = -1; dx <= 1; dx++) {
+        const int x = tx + dx, y = ty + dy;
+        if (x >= 0 && x < tiles_x && y >= 0 && y < tiles_y)
+          any = any || D.tile_flag[y * tiles_x + x];
+      }
+    if (!any) return;
   }
-  if (j >= P.nx || k >= P.ny) return;
+  const int j = tx * kStripW - 2 + lane;
+  if (lane < 2 || lane >= 2 + kStripW || j >= P.nx) return;
+  for (int rr = w; rr < kTileH; rr += 4) {
+    const int k = ty * kTileH + rr;
+    if (k < P.ny) topo_cell(P, D, j, k);
+  }
+}
+
+__global__ void __launch_bounds__(256) k_topo(DevParams P, DevPtrs D, int sparse) {
+  topo_tile(P, D, blockIdx.x, blockIdx.y, gridDim.x, gridDim.y, sparse);
+}
+
+__device__ __forceinline__ void topo_cell(const DevParams& P, const DevPtrs& D, int j, int k) {
   const size_t c = cidx(P, j, k);
   double bx = 0.0, by = 0.0, bxx = 0.0, byy = 0.0, bxy = 0.0;
   if (P.nx >= 5 && P.ny >= 5) {

@@ -56,6 +56,7 @@ __device__ __forceinline__ double norm2_(const double* x) {
 struct NewtonOut {
   int iters;
   int conv;
+  int ls_evals;  // residual evaluations inside the line search
 };
 
 // solve_rk_step with eval_jacobian and lnsrch (src/solver_2d.f90:1660-2365).
@@ -77,10 +78,16 @@ __device__ __forceinline__ NewtonOut solve_rk_step(const DevParams& P, double* q
   for (int i = 0; i < NV; i++) org[i] = fmax(fabs(qj[i]), 1.0e-3);
 #pragma unroll
   for (int i = 0; i < NV; i++) rel[i] = qj[i] / org[i];
-  NewtonOut out{0, 0};
+  NewtonOut out{0, 0, 0};
   int it;
+  // rt / scal already hold the residual at qj when the previous iteration
+  // ended on an evaluated point (an accepted line-search point or the full
+  // first step); the reference evaluates it again at the top of the
+  // iteration, with the same operands, so that evaluation is skipped
+  bool fresh = false;
   for (it = 1; it <= 100; it++) {
-    eval_f<NS>(P, qj, qold, Rj, a_diag, dt, Bx, By, rt, scal);
+    if (!fresh) eval_f<NS>(P, qj, qold, Rj, a_diag, dt, Bx, By, rt, scal);
+    bool restored = false;
     double mf = 0.0;
 #pragma unroll
     for (int i = 0; i < NV; i++) mf = fmax(mf, fabs(rt[i]));
@@ -197,11 +204,13 @@ __device__ __forceinline__ NewtonOut solve_rk_step(const DevParams& P, double* q
 #pragma unroll
           for (int i = 0; i < NV; i++) q[i] = rel[i] * org[i];
           eval_f<NS>(P, q, qold, Rj, a_diag, dt, Bx, By, rt, scal);
+          out.ls_evals++;
           if (scal <= 0.9 * scal_old) break;
           if (alam < alamin) {
 #pragma unroll
             for (int i = 0; i < NV; i++) rel[i] = old[i];
             scal = scal_old;
+            restored = true;  // rt belongs to the rejected trial point
             break;
           }
           if (alam == 1.0) {
@@ -236,6 +245,7 @@ __device__ __forceinline__ NewtonOut solve_rk_step(const DevParams& P, double* q
     }
 #pragma unroll
     for (int i = 0; i < NV; i++) qj[i] = rel[i] * org[i];
+    fresh = !restored;
     mf = 0.0;
 #pragma unroll
     for (int i = 0; i < NV; i++) mf = fmax(mf, fabs(rt[i]));
@@ -265,13 +275,14 @@ __device__ __forceinline__ int stage2(const DevParams& P, const double* q0,
                                       const double* D, double dt, double Bx,
                                       double By, double Bxx, double Bxy,
                                       double Byy, double gc, double* q,
-                                      int& nonconv) {
+                                      int& nonconv, int& ls_evals) {
   constexpr int NV = 4 + NS;
   double q_fv[NV], qp[NV + 2], q_si[NV];
 #pragma unroll
   for (int i = 0; i < NV; i++) q_fv[i] = q0[i] - dt * (D[i] - 0.0);
   phys<NS>(P, q_fv, qp);
   nonconv = 0;
+  ls_evals = 0;
   if (!(q_fv[0] > 0.0)) {
 #pragma unroll
     for (int i = 0; i < NV; i++) q[i] = q_fv[i];
@@ -306,6 +317,7 @@ __device__ __forceinline__ int stage2(const DevParams& P, const double* q0,
   }
   NewtonOut no = solve_rk_step<NS>(P, q, q0, Rj, 1.0, dt, Bx, By);
   nonconv = !no.conv;
+  ls_evals = no.ls_evals;
   if (P.ny == 1) q[2] = 0.0;
   if (P.nx == 1) q[1] = 0.0;
   if (q_si[1] * q_si[1] + q_si[2] * q_si[2] == 0.0) {

@@ -333,6 +333,54 @@ def wet_2048(n=2048, dx=5.0, steps=1000, energy=True):
                 description="fully wet %dx%d dx=%g incline" % (n, n, dx))
 
 
+def ensemble_members(count=256, seed=4, n_min=128, n_max=1024, dx=10.0,
+                     steps=500, energy=True, sizes=None):
+    """Batched ensemble: `count` (256) members, square grids with
+    n = round(exp(U[ln 128, ln 1024])) (seed 4), dx = 10 m.  Each member has
+    its own seeded DEM (a 5 degree plane plus 12 Gaussian bumps of amplitude
+    U[5, 40] m and width U[3, 12] % of L) and a Gaussian column
+    h = 15 exp(-r^2 / (2 (0.08 L)^2)) m at rest at (0.3 L, 0.5 L), dry
+    (h < 1e-3 m cut to 0) elsewhere.  Sampled per member: Voellmy mu ~
+    U[0.05, 0.3] (xi = 1000 m/s^2), carrier temperature ~ U[300, 900] K, total
+    solid volume fraction ~ U[0.005, 0.05] split 2:1 over two classes (0.5 mm,
+    2 mm); settling on.  500 steps each, CFL 0.24.  `sizes` overrides the
+    sampled grid sizes (tests)."""
+    rng = np.random.default_rng(seed)
+    if sizes is None:
+        sizes = np.rint(np.exp(rng.uniform(math.log(n_min), math.log(n_max), count))).astype(int)
+    out = []
+    for m, n in enumerate(sizes):
+        n = int(n)
+        mu = float(rng.uniform(0.05, 0.3))
+        T = float(rng.uniform(300.0, 900.0))
+        at = float(rng.uniform(0.005, 0.05))
+        p = make_params(n, n, dx, 2, energy_flag=energy, cfl=0.24, max_dt=1.0,
+                        mu=mu, xi=1000.0, rho_s=(2500.0, 2500.0),
+                        diam_s=(0.5e-3, 2.0e-3), sp_heat_s=(1617.0, 1617.0),
+                        settling=True, T_ambient=300.0)
+        xc, yc, _, _ = cell_centres(p)
+        L = n * dx
+        X, Y = np.meshgrid(xc, yc, indexing="ij")
+        B = (L - X) * math.tan(math.radians(5.0))
+        mrng = np.random.default_rng([seed, m])
+        for _ in range(12):
+            a = mrng.uniform(5.0, 40.0)
+            sg = mrng.uniform(0.03, 0.12) * L
+            x0, y0 = mrng.uniform(0.0, L, 2)
+            # separable Gaussian: outer product of the x and y factors
+            B += a * np.outer(np.exp(-(xc - x0) ** 2 / (2.0 * sg * sg)),
+                              np.exp(-(yc - y0) ** 2 / (2.0 * sg * sg)))
+        r2 = (X - 0.3 * L) ** 2 + (Y - 0.5 * L) ** 2
+        h = 15.0 * np.exp(-r2 / (2.0 * (0.08 * L) ** 2))
+        wet = h > 1e-3
+        q = np.zeros((p.n_vars, n, n))
+        q[:, wet] = _state_from_h(p, h[wet], T, [at * 2.0 / 3.0, at / 3.0])
+        out.append(Deck("member_%d" % m, p, B, q, steps=steps, dt0=1e-3,
+                        description="ensemble member %d: %dx%d mu=%.3f T=%.0f alpha=%.4f"
+                        % (m, n, n, mu, T, at)))
+    return out
+
+
 def small_2d(n=48, ny=None, dx=10.0, seed=0, energy=True, alpha=False,
              settling=True, erosion=False, source=False, topo_change=False,
              slope_correction=False, curvature=False, n_solid=2, rheology=True,

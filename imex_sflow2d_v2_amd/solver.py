@@ -28,8 +28,10 @@ class ImexswStats(C.Structure):
     _fields_ = [(n, C.c_int64) for n in (
         "steps", "cell_updates_active", "cell_updates_grid", "newton_cells",
         "newton_iters", "newton_nonconverged", "nan_events", "neg_h_events",
-        "neg_solid_events", "sum_solid_events", "low_T_events")] + [
-        ("solve_cells", C.c_int32), ("active_tiles", C.c_int32)]
+        "neg_solid_events", "sum_solid_events", "low_T_events",
+        "line_search_evals")] + [
+        ("newton_iter_hist", C.c_int64 * 8), ("newton_iter_max", C.c_int64),
+        ("solve_cells", C.c_int32), ("compute_cells", C.c_int32)]
 
 
 # name -> (restype, argtypes); the full set of symbols declared in
@@ -61,6 +63,17 @@ ABI = {
     "imexsw_kernel_times": (C.c_int, [C.c_void_p, _D, _I64, C.c_int]),
     "imexsw_kernel_name": (C.c_char_p, [C.c_int]),
     "imexsw_set_tuning": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
+    "imexsw_ensemble_create": (C.c_int, [C.c_int32, C.POINTER(ImexswParams),
+                                         C.POINTER(C.c_void_p)]),
+    "imexsw_ensemble_destroy": (None, [C.c_void_p]),
+    "imexsw_ensemble_last_error": (C.c_char_p, [C.c_void_p]),
+    "imexsw_ensemble_member": (C.c_void_p, [C.c_void_p, C.c_int32]),
+    "imexsw_ensemble_set_tuning": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
+    "imexsw_ensemble_advance": (C.c_int, [C.c_void_p, C.c_int32, C.c_double, _D,
+                                          C.c_double, C.POINTER(C.c_int32)]),
+    "imexsw_ensemble_autotune": (C.c_int, [C.c_void_p, C.c_int32, C.c_double, _D,
+                                           C.c_double, C.POINTER(C.c_int)]),
+    "imexsw_ensemble_last_ms": (C.c_double, [C.c_void_p]),
     "imexsw_sync": (C.c_int, [C.c_void_p]),
 }
 
@@ -118,7 +131,24 @@ class Solver:
             raise ImexswError(self.lib.imexsw_last_error(self.h).decode())
         return rc
 
+    @classmethod
+    def _member(cls, lib, handle, params, owner):
+        """A non-owning view of an ensemble member's context."""
+        self = cls.__new__(cls)
+        self.lib = lib
+        self.params = params.copy()
+        self.h = C.c_void_p(handle)
+        self.nv = 4 + self.params.n_solid
+        self.ns = self.params.n_solid
+        self.nx, self.ny = self.params.nx, self.params.ny
+        self.t = 0.0
+        self._owner = owner
+        return self
+
     def close(self):
+        if getattr(self, "_owner", None) is not None:
+            self.h = None
+            return
         if getattr(self, "h", None):
             self.lib.imexsw_destroy(self.h)
             self.h = None
@@ -211,7 +241,7 @@ class Solver:
         self.t = float(t[0])
         return n.value
 
-    def set_tuning(self, max_strip_rows=32, target_strips=2048):
+    def set_tuning(self, max_strip_rows=16, target_strips=2048):
         self._chk(self.lib.imexsw_set_tuning(self.h, int(max_strip_rows), int(target_strips)))
 
     def sync(self):
@@ -252,7 +282,9 @@ class Solver:
     def stats(self, reset=False):
         s = ImexswStats()
         self._chk(self.lib.imexsw_stats_get(self.h, C.byref(s), int(reset)))
-        return {n: getattr(s, n) for n, _ in ImexswStats._fields_}
+        out = {n: getattr(s, n) for n, _ in ImexswStats._fields_}
+        out["newton_iter_hist"] = list(s.newton_iter_hist)
+        return out
 
     def set_timing(self, on=True):
         self._chk(self.lib.imexsw_set_timing(self.h, int(on)))
@@ -263,3 +295,67 @@ class Solver:
         n = self._chk(self.lib.imexsw_kernel_times(self.h, _dp(ms), calls.ctypes.data_as(_I64), 16))
         return {self.lib.imexsw_kernel_name(i).decode(): (float(ms[i]), int(calls[i]))
                 for i in range(n)}
+
+
+class Ensemble:
+    """Many member grids advanced by batched launches (imexsw_ensemble_*).
+    ``members[i]`` is a Solver view of member i for set-up and output."""
+
+    def __init__(self, params_list):
+        self.lib = load_library()
+        n = len(params_list)
+        self._params = (ImexswParams * n)(*[p.copy() for p in params_list])
+        h = C.c_void_p()
+        rc = self.lib.imexsw_ensemble_create(n, self._params, C.byref(h))
+        if rc != 0:
+            raise ImexswError(self.lib.imexsw_ensemble_last_error(None).decode())
+        self.h = h
+        self.members = [Solver._member(self.lib, self.lib.imexsw_ensemble_member(h, i),
+                                       params_list[i], self) for i in range(n)]
+        self.t = np.zeros(n)
+
+    def _chk(self, rc):
+        if rc < 0:
+            raise ImexswError(self.lib.imexsw_ensemble_last_error(self.h).decode())
+        return rc
+
+    def load(self, decks):
+        for m, d in zip(self.members, decks):
+            m.load_deck(d)
+        self.t[:] = 0.0
+        return self
+
+    def set_tuning(self, strip_rows=8, graph_steps=25):
+        self._chk(self.lib.imexsw_ensemble_set_tuning(self.h, int(strip_rows), int(graph_steps)))
+
+    def autotune(self, steps_per_trial, dt0, t_end=1e30):
+        best = C.c_int(0)
+        self._chk(self.lib.imexsw_ensemble_autotune(self.h, int(steps_per_trial), dt0,
+                                                    _dp(self.t), t_end, C.byref(best)))
+        for m, t in zip(self.members, self.t):
+            m.t = float(t)
+        return best.value
+
+    def advance(self, nsteps, dt0, t_end=1e30):
+        n = C.c_int32(0)
+        self._chk(self.lib.imexsw_ensemble_advance(self.h, int(nsteps), dt0, _dp(self.t),
+                                                   t_end, C.byref(n)))
+        for m, t in zip(self.members, self.t):
+            m.t = float(t)
+        return n.value
+
+    def last_ms(self):
+        return self.lib.imexsw_ensemble_last_ms(self.h)
+
+    def close(self):
+        if getattr(self, "h", None):
+            for m in self.members:
+                m.h = None
+            self.lib.imexsw_ensemble_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

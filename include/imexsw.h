@@ -138,8 +138,13 @@ typedef struct imexsw_stats {
   int64_t neg_solid_events;     /* solids <= -1e-7, src/solver_2d.f90:1439 */
   int64_t sum_solid_events;     /* sum(solids) > h, src/solver_2d.f90:1569 */
   int64_t low_T_events;         /* T < 273 K, src/solver_2d.f90:1538 */
+  int64_t line_search_evals;    /* residual evaluations inside lnsrch */
+  int64_t newton_iter_hist[8];  /* cells by Newton iterations: 1, 2, 3, 4,
+                                   5-8, 9-16, 17-32, 33-100 */
+  int64_t newton_iter_max;      /* most iterations any cell took */
   int32_t solve_cells;          /* active cells of the last check_solve */
-  int32_t active_tiles;         /* active device tiles of the last check_solve */
+  int32_t compute_cells;        /* of those, cells whose update is computed
+                                   (the rest provably keep their state) */
 } imexsw_stats;
 
 typedef struct imexsw_ctx imexsw_ctx;
@@ -257,7 +262,7 @@ int imexsw_stats_get(imexsw_ctx *ctx, imexsw_stats *stats, int reset);
 
 /* Per-kernel device time, measured with HIP events recorded on the
  * context's stream around every launch.  names: mask, stencil, local,
- * topo, dtpolicy, post.  ms[i] accumulates milliseconds, calls[i] launches.
+ * topo, dtpolicy, post, erosion.  ms[i] accumulates milliseconds, calls[i] launches.
  * Returns the number of kernels (<= cap).  Timing is enabled by
  * imexsw_set_timing(ctx, 1). */
 int imexsw_set_timing(imexsw_ctx *ctx, int enable);
@@ -266,14 +271,59 @@ const char *imexsw_kernel_name(int index);
 
 /* Launch shape of the stencil pass (no reference counterpart: the
  * reference's OpenMP loops have no tiling).  Each step the stencil kernel
- * picks the tallest strip of 4 * 2^m rows, m >= 0, up to max_strip_rows,
- * that still leaves about target_strips strips for the step's active cells.
- * max_strip_rows: 4, 8, 16, 32 or 64; target_strips >= 1.  Results do not
- * depend on these settings; defaults 32 and 2048. */
+ * picks the tallest strip of 2^m rows, m >= 0, up to max_strip_rows, that
+ * still leaves about target_strips strips for the step's computed cells.
+ * max_strip_rows: 1, 2, 4, 8 or 16; target_strips >= 1.  Results do not
+ * depend on these settings; defaults 16 and 2048. */
 int imexsw_set_tuning(imexsw_ctx *ctx, int max_strip_rows, int target_strips);
 
 /* Wait for all work queued on the context's stream. */
 int imexsw_sync(imexsw_ctx *ctx);
+
+/* ---------------------------------------------------------------------------
+ * Ensembles: many independent grids (members) of possibly different sizes
+ * and parameters advanced together, one set of batched launches per time
+ * step for all members, the steps captured in a HIP graph.  The reference
+ * runs an ensemble as a farm of separate processes, one per member
+ * (EXAMPLES/EXAMPLE_ENSEMBLE_ETNA/launch_jobs.py); there is no solver_2d
+ * counterpart.  Every member advances exactly as it would alone in its own
+ * context (same kernels per cell, same bits).
+ * Supported: 1..3 solid classes (the same for all members), 2-D grids, no
+ * topography change, radial source, slope correction or curvature.
+ * ------------------------------------------------------------------------- */
+typedef struct imexsw_ensemble imexsw_ensemble;
+
+/* params[n_members]: one imexsw_params per member. */
+int imexsw_ensemble_create(int32_t n_members, const imexsw_params *params,
+                           imexsw_ensemble **ens);
+void imexsw_ensemble_destroy(imexsw_ensemble *ens);
+const char *imexsw_ensemble_last_error(const imexsw_ensemble *ens);
+
+/* Member i as an ordinary context for set-up and output (topography, q,
+ * check_solve, downloads, stats).  Owned by the ensemble. */
+imexsw_ctx *imexsw_ensemble_member(imexsw_ensemble *ens, int32_t i);
+
+/* Stencil strip height for all members (4, 8 or 16 rows) and the number of
+ * time steps per captured graph (1..1000).  Results do not depend on these. */
+int imexsw_ensemble_set_tuning(imexsw_ensemble *ens, int strip_rows, int graph_steps);
+
+/* Run nsteps batched steps of the main loop for every member (the dt policy
+ * of src/IMEX_SfloW2D.f90:357-381 per member, dt history initialised from
+ * dt0 on a member's first run).  t[n_members] holds each member's time in
+ * and out (NULL: keep the members' device times); members stop at t_end.
+ * steps_done = steps taken by member 0. */
+int imexsw_ensemble_advance(imexsw_ensemble *ens, int32_t nsteps, double dt0, double *t,
+                            double t_end, int32_t *steps_done);
+
+/* Autotuner: runs steps_per_trial real steps with each strip height, keeps
+ * the fastest (best_strip_rows may be NULL).  The trials advance the
+ * members (t as in imexsw_ensemble_advance). */
+int imexsw_ensemble_autotune(imexsw_ensemble *ens, int32_t steps_per_trial, double dt0,
+                             double *t, double t_end, int *best_strip_rows);
+
+/* Device time of the last imexsw_ensemble_advance, milliseconds (HIP
+ * events on the ensemble's stream). */
+double imexsw_ensemble_last_ms(const imexsw_ensemble *ens);
 
 #ifdef __cplusplus
 }
