@@ -28,6 +28,8 @@ import re
 import sys
 import time
 
+import numpy as np
+
 HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, HERE)
 
@@ -192,12 +194,103 @@ def bench_ensemble(args):
     }
 
 
+def bench_warmstart(args):
+    """Learned Newton warm start: harvest (state entering Newton, converged
+    solution) pairs from the solver on the fully wet 2048^2 grid, train the
+    MLP on the device, then run the same steps plain and warm-started on the
+    fully wet grid and on the volcano from t = 0 and compare Newton
+    iterations per cell, fallback rate, step time and the difference of the
+    states."""
+    from imex_sflow2d_v2_amd import learned
+    from imex_sflow2d_v2_amd.solver import Solver
+    harvest_steps = 12
+    dw = make_deck("wet")
+    g = Solver(dw.params).load_deck(dw)
+    learned.start_harvest(g, 60_000_000)
+    t0 = time.perf_counter()
+    g.advance(harvest_steps, dt0=dw.dt0)
+    g.sync()
+    pairs, produced = learned.harvested(g)
+    learned.stop_harvest(g)
+    g.close()
+    t_h = time.perf_counter() - t0
+    m = learned.Mlp(seed=1)
+    t0 = time.perf_counter()
+    hist = m.train(pairs, epochs=3, batch=8192, lr=1e-3, seed=2)
+    t_train = time.perf_counter() - t0
+    del pairs
+    res = {}
+    steps = {"wet": 40, "volcano": 300}
+    for wl in ("wet", "volcano"):
+        d = make_deck(wl)
+        out = {}
+        qs = {}
+        for mode in ("plain", "warm"):
+            s_ = Solver(d.params).load_deck(d)
+            if mode == "warm":
+                learned.set_warm_start(s_, m)
+            s_.advance(2, dt0=d.dt0)   # warm-up (graph capture etc.)
+            s_.sync()
+            s_.stats(reset=True)
+            a = time.perf_counter()
+            s_.advance(steps[wl], dt0=d.dt0)
+            s_.sync()
+            el = time.perf_counter() - a
+            st = s_.stats()
+            out[mode] = {"ms_per_step": el / steps[wl] * 1e3,
+                         "newton_iters_per_cell": st["newton_iters"] / max(1, st["newton_cells"]),
+                         "line_search_evals_per_cell": st["line_search_evals"] / max(1, st["newton_cells"]),
+                         "warm_cells": st["warm_cells"], "fallbacks": st["warm_fallbacks"],
+                         "newton_cells": st["newton_cells"],
+                         "nonconverged": st["newton_nonconverged"]}
+            qs[mode] = s_.q()
+            s_.close()
+        rel = 0.0
+        for v in range(qs["plain"].shape[0]):
+            sc = max(float(np.abs(qs["plain"][v]).max()), 1e-300)
+            rel = max(rel, float(np.abs(qs["warm"][v] - qs["plain"][v]).max()) / sc)
+        w, p_ = out["warm"], out["plain"]
+        res[wl] = {
+            "steps": steps[wl],
+            "plain": p_, "warm": w,
+            "iteration_reduction": p_["newton_iters_per_cell"] / max(w["newton_iters_per_cell"], 1e-30),
+            "end_to_end_speedup": p_["ms_per_step"] / w["ms_per_step"],
+            "fallback_rate": w["fallbacks"] / max(1, w["newton_cells"]),
+            "max_rel_diff_warm_vs_plain": rel,
+        }
+    return {
+        "metric": "Newton iterations per cell, plain / learned warm start (volcano 2048^2)",
+        "value": res["volcano"]["iteration_reduction"],
+        "unit": "x",
+        "n_gpus": 1,
+        "steps": steps["volcano"],
+        "warmup": 2,
+        "ms_per_step": res["volcano"]["warm"]["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64 (MLP bf16/fp32)",
+        "data": "synthetic",
+        "config": {
+            "workload": "warmstart",
+            "harvest": {"deck": "wet_2048x2048", "steps": harvest_steps, "pairs": int(produced),
+                        "seconds": t_h},
+            "training": {"epochs": 3, "batch": 8192, "lr": 1e-3, "loss_history": [float(x) for x in hist],
+                         "seconds": t_train},
+            "results": res,
+        },
+        "roofline": None,
+        "cpu_baseline": None,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--workload", default="volcano", choices=["volcano", "wet", "ensemble"])
+    ap.add_argument("--workload", default="volcano",
+                    choices=["volcano", "wet", "ensemble", "warmstart"])
     ap.add_argument("--members", type=int, default=256,
                     help="ensemble workload: number of members")
     ap.add_argument("--cpu-steps", type=int, default=30)
@@ -233,6 +326,9 @@ def main():
 
     if args.workload == "ensemble":
         print(json.dumps(bench_ensemble(args)), flush=True)
+        return 0
+    if args.workload == "warmstart":
+        print(json.dumps(bench_warmstart(args)), flush=True)
         return 0
 
     d = make_deck(args.workload)

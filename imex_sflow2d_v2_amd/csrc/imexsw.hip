@@ -52,12 +52,17 @@ struct imexsw_ctx {
   std::vector<hipEvent_t> ev_pool;
   std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> pending;
   std::vector<void*> allocs;
+  // learned warm start (guess planes in D.guess) and harvest buffer
+  bool warm = false;
+  MlpDev mlp{};
   // captured kGraphSteps-step graph of imexsw_advance and what it captured
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   DevParams graph_P{};
   DevPtrs graph_D{};
   long graph_shape = 0;
+  MlpDev graph_mlp{};
+  bool graph_warm = false;
 };
 
 constexpr int kGraphSteps = 20;
@@ -182,9 +187,19 @@ void launch_local(imexsw_ctx* c, int ops) {
   static int res0 = 0, res1 = 0;
   if (!res0) res0 = resident_grid(k_local<NS, 0>, 256, c->n_cu, 1 << 30);
   if (!res1) res1 = resident_grid(k_local<NS, 1>, 256, c->n_cu, 1 << 30);
-  if (ops & OP_RK)
+  if (ops & OP_RK) {
+    int rk = OP_RK;
+    if (c->warm) {
+      static int resg = 0;
+      if (!resg) resg = resident_grid(k_mlp_guess<NS>, 64, c->n_cu, 1 << 30);
+      hipLaunchKernelGGL((k_mlp_guess<NS>), dim3(std::min(c->n_tiles * kTileH, resg)), dim3(64),
+                         0, c->stream, c->P, c->D, c->mlp);
+      rk |= OP_WARM;
+    }
+    if (c->D.harvest) rk |= OP_HARVEST;
     hipLaunchKernelGGL((k_local<NS, 0>), dim3(std::min(items, res0)), dim3(256), 0, c->stream,
-                       c->P, c->D, (int)OP_RK);
+                       c->P, c->D, rk);
+  }
   if (ops & (OP_EROSION | OP_POST | OP_SUMS))
     hipLaunchKernelGGL((k_local<NS, 1>), dim3(std::min(items, res1)), dim3(256), 0, c->stream,
                        c->P, c->D, ops & ~(int)OP_RK);
@@ -517,6 +532,7 @@ void imexsw_destroy(imexsw_ctx* c) {
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   collect_timing(c);
   drop_ctx_graph(c);
+  if (c->D.harvest) (void)hipFree(c->D.harvest);
   for (void* p : c->allocs) (void)hipFree(p);
   if (c->stage) (void)hipFree(c->stage);
   for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
@@ -735,7 +751,9 @@ int imexsw_advance(imexsw_ctx* c, int32_t nsteps, double dt0, double* t_io, doub
   int n = 0;
   if (!c->timing && nsteps >= kGraphSteps) {
     if (!c->exec || std::memcmp(&c->graph_P, &c->P, sizeof(DevParams)) ||
-        std::memcmp(&c->graph_D, &c->D, sizeof(DevPtrs)) || c->graph_shape != c->strip_h * 1000003 + c->target_strips) {
+        std::memcmp(&c->graph_D, &c->D, sizeof(DevPtrs)) ||
+        c->graph_shape != c->strip_h * 1000003 + c->target_strips || c->graph_warm != c->warm ||
+        std::memcmp(&c->graph_mlp, &c->mlp, sizeof(MlpDev))) {
       drop_ctx_graph(c);
       HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
       int rc = 0;
@@ -752,6 +770,8 @@ int imexsw_advance(imexsw_ctx* c, int32_t nsteps, double dt0, double* t_io, doub
       c->graph_P = c->P;
       c->graph_D = c->D;
       c->graph_shape = c->strip_h * 1000003 + c->target_strips;
+      c->graph_warm = c->warm;
+      c->graph_mlp = c->mlp;
     }
     for (; n + kGraphSteps <= nsteps; n += kGraphSteps)
       HIPCHK(c, hipGraphLaunch(c->exec, c->stream));
@@ -857,6 +877,8 @@ int imexsw_stats_get(imexsw_ctx* c, imexsw_stats* st, int reset) {
   st->line_search_evals = (int64_t)v[C_LS_EVALS];
   for (int b = 0; b < 8; b++) st->newton_iter_hist[b] = (int64_t)v[C_HIST0 + b];
   st->newton_iter_max = (int64_t)v[C_ITMAX];
+  st->warm_cells = (int64_t)v[C_WARM_USED];
+  st->warm_fallbacks = (int64_t)v[C_WARM_FALLBACK];
   st->solve_cells = (int32_t)v[C_SOLVE_CELLS];
   st->compute_cells = (int32_t)v[C_COMPUTE_CELLS];
   if (reset) {
@@ -1230,5 +1252,385 @@ int imexsw_ensemble_autotune(imexsw_ensemble* e, int32_t steps_per_trial, double
 }
 
 double imexsw_ensemble_last_ms(const imexsw_ensemble* e) { return e ? e->last_ms : 0.0; }
+
+}  // extern "C"
+
+// ===========================================================================
+// learned Newton warm start: the MLP, its training on the device, and the
+// harvest of (state entering Newton, converged solution) pairs
+// ===========================================================================
+struct imexsw_mlp {
+  std::string err;
+  hipStream_t stream = nullptr;
+  float* w = nullptr;     // fp32 master parameters
+  short* wb = nullptr;    // bf16 copies
+  float* norm = nullptr;  // kMlpNorm
+  float* m1 = nullptr;
+  float* m2 = nullptr;
+  float* grad = nullptr;
+  float* loss = nullptr;
+  int cap = 0;            // training buffer capacity (samples, multiple of 64)
+  float* tz = nullptr;    // 3 * cap * kMlpH
+  short* ta = nullptr;    // cap * 16 + 3 * cap * kMlpH
+  short* tdz = nullptr;   // 3 * cap * kMlpH + cap * 16
+  float* x = nullptr;     // data buffers (standardised): n * 16, n * 2
+  float* t = nullptr;
+  size_t data_cap = 0;
+  long steps = 0;         // Adam steps taken
+  float host_norm[kMlpNorm];
+};
+
+namespace {
+
+int mfail(imexsw_mlp* m, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  if (m) m->err = buf;
+  else g_create_error = buf;
+  return -1;
+}
+
+#define MHIPCHK(m, call)                                                            \
+  do {                                                                              \
+    hipError_t e_ = (call);                                                         \
+    if (e_ != hipSuccess) return mfail((m), "%s: %s", #call, hipGetErrorString(e_)); \
+  } while (0)
+
+MlpDev mlp_dev(imexsw_mlp* m) { return MlpDev{m->w, m->wb, m->norm}; }
+
+int mlp_reserve(imexsw_mlp* m, int n) {
+  const int cap = (n + 63) / 64 * 64;
+  if (cap <= m->cap) return 0;
+  for (void* p : {(void*)m->tz, (void*)m->ta, (void*)m->tdz})
+    if (p) (void)hipFree(p);
+  m->tz = nullptr;
+  m->ta = nullptr;
+  m->tdz = nullptr;
+  MHIPCHK(m, hipMalloc(&m->tz, sizeof(float) * 3 * cap * kMlpH));
+  MHIPCHK(m, hipMalloc(&m->ta, sizeof(short) * ((size_t)cap * kMlpInPad + 3 * (size_t)cap * kMlpH)));
+  MHIPCHK(m, hipMalloc(&m->tdz, sizeof(short) * (3 * (size_t)cap * kMlpH + (size_t)cap * kMlpOutPad)));
+  m->cap = cap;
+  return 0;
+}
+
+MlpTrainBuf mlp_buf(imexsw_mlp* m) {
+  MlpTrainBuf T;
+  for (int k = 0; k < 3; k++) T.z[k] = m->tz + (size_t)k * m->cap * kMlpH;
+  T.a[0] = m->ta;
+  for (int k = 1; k < 4; k++) T.a[k] = m->ta + (size_t)m->cap * kMlpInPad + (size_t)(k - 1) * m->cap * kMlpH;
+  for (int k = 0; k < 3; k++) T.dz[k] = m->tdz + (size_t)k * m->cap * kMlpH;
+  T.dz[3] = m->tdz + 3 * (size_t)m->cap * kMlpH;
+  T.grad = m->grad;
+  T.loss = m->loss;
+  return T;
+}
+
+int mlp_data(imexsw_mlp* m, size_t n) {
+  if (n <= m->data_cap) return 0;
+  if (m->x) (void)hipFree(m->x);
+  if (m->t) (void)hipFree(m->t);
+  m->x = nullptr;
+  m->t = nullptr;
+  MHIPCHK(m, hipMalloc(&m->x, sizeof(float) * kMlpInPad * std::max<size_t>(n, 1)));
+  MHIPCHK(m, hipMalloc(&m->t, sizeof(float) * kMlpOut * std::max<size_t>(n, 1)));
+  m->data_cap = n;
+  return 0;
+}
+
+// raw features / targets -> standardised, inputs padded to 16
+void standardise(const float* norm, const float* xr, const float* tr, size_t n,
+                 std::vector<float>& xs, std::vector<float>& ts, size_t xstride) {
+  xs.assign(n * kMlpInPad, 0.f);
+  if (tr) ts.assign(n * kMlpOut, 0.f);
+  for (size_t s = 0; s < n; s++) {
+    for (int f = 0; f < kMlpIn; f++)
+      xs[s * kMlpInPad + f] = (xr[s * xstride + f] - norm[f]) / norm[kMlpIn + f];
+    if (tr)
+      for (int o = 0; o < kMlpOut; o++)
+        ts[s * kMlpOut + o] =
+            (tr[s * xstride + o] - norm[2 * kMlpIn + o]) / norm[2 * kMlpIn + kMlpOut + o];
+  }
+}
+
+uint64_t splitmix(uint64_t& s) {
+  uint64_t z = (s += 0x9E3779B97F4A7C15ull);
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+double urand(uint64_t& s) { return ((splitmix(s) >> 11) + 0.5) * (1.0 / 9007199254740992.0); }
+double nrand(uint64_t& s) {
+  const double u = urand(s), v = urand(s);
+  return std::sqrt(-2.0 * std::log(u)) * std::cos(6.283185307179586 * v);
+}
+
+}  // namespace
+
+extern "C" {
+
+int imexsw_mlp_num_params(void) { return kMlpParams; }
+
+int imexsw_mlp_create(imexsw_mlp** out, uint64_t seed) {
+  if (!out) return mfail(nullptr, "null argument");
+  *out = nullptr;
+  imexsw_mlp* m = new imexsw_mlp();
+  int rc = 0;
+  auto al = [&](void** p, size_t bytes) {
+    if (!rc && hipMalloc(p, bytes) != hipSuccess) rc = 1;
+  };
+  al((void**)&m->w, sizeof(float) * kMlpParams);
+  al((void**)&m->wb, sizeof(short) * kMlpParams);
+  al((void**)&m->norm, sizeof(float) * kMlpNorm);
+  al((void**)&m->m1, sizeof(float) * kMlpParams);
+  al((void**)&m->m2, sizeof(float) * kMlpParams);
+  al((void**)&m->grad, sizeof(float) * kMlpParams);
+  al((void**)&m->loss, sizeof(float));
+  if (!rc && hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking) != hipSuccess) rc = 1;
+  if (rc) {
+    imexsw_mlp_destroy(m);
+    return mfail(nullptr, "MLP allocation failed");
+  }
+  // He-normal initial weights (deterministic in seed), zero biases; padded
+  // input columns and output rows stay zero
+  std::vector<float> w(kMlpParams, 0.f);
+  uint64_t st = seed ^ 0x5DEECE66Dull;
+  auto fill = [&](int off, int rows, int cols, int use_rows, int use_cols, int fan_in) {
+    const double sd = std::sqrt(2.0 / fan_in);
+    for (int r = 0; r < use_rows; r++)
+      for (int cc = 0; cc < use_cols; cc++) w[off + r * cols + cc] = (float)(sd * nrand(st));
+  };
+  fill(kOffW1, kMlpH, kMlpInPad, kMlpH, kMlpIn, kMlpIn);
+  fill(kOffW2, kMlpH, kMlpH, kMlpH, kMlpH, kMlpH);
+  fill(kOffW3, kMlpH, kMlpH, kMlpH, kMlpH, kMlpH);
+  fill(kOffW4, kMlpOutPad, kMlpH, kMlpOut, kMlpH, 4 * kMlpH);
+  for (int f = 0; f < kMlpNorm; f++) m->host_norm[f] = (f >= kMlpIn && f < 2 * kMlpIn) ||
+                                                       f >= 2 * kMlpIn + kMlpOut ? 1.f : 0.f;
+  if (imexsw_mlp_set_params(m, w.data(), m->host_norm)) {
+    std::string e = m->err;
+    imexsw_mlp_destroy(m);
+    return mfail(nullptr, "%s", e.c_str());
+  }
+  *out = m;
+  return 0;
+}
+
+void imexsw_mlp_destroy(imexsw_mlp* m) {
+  if (!m) return;
+  if (m->stream) (void)hipStreamSynchronize(m->stream);
+  for (void* p : {(void*)m->w, (void*)m->wb, (void*)m->norm, (void*)m->m1, (void*)m->m2,
+                  (void*)m->grad, (void*)m->loss, (void*)m->tz, (void*)m->ta, (void*)m->tdz,
+                  (void*)m->x, (void*)m->t})
+    if (p) (void)hipFree(p);
+  if (m->stream) (void)hipStreamDestroy(m->stream);
+  delete m;
+}
+
+const char* imexsw_mlp_last_error(const imexsw_mlp* m) {
+  return m ? m->err.c_str() : g_create_error.c_str();
+}
+
+int imexsw_mlp_set_params(imexsw_mlp* m, const float* params, const float* norm) {
+  if (!m) return mfail(m, "null argument");
+  if (params) {
+    MHIPCHK(m, hipMemcpyAsync(m->w, params, sizeof(float) * kMlpParams, hipMemcpyHostToDevice, m->stream));
+    hipLaunchKernelGGL(k_mlp_cast, dim3((kMlpParams + 255) / 256), dim3(256), 0, m->stream, m->w, m->wb);
+    MHIPCHK(m, hipMemsetAsync(m->m1, 0, sizeof(float) * kMlpParams, m->stream));
+    MHIPCHK(m, hipMemsetAsync(m->m2, 0, sizeof(float) * kMlpParams, m->stream));
+    m->steps = 0;
+  }
+  if (norm) {
+    for (int f = 0; f < kMlpNorm; f++) m->host_norm[f] = norm[f];
+    MHIPCHK(m, hipMemcpyAsync(m->norm, m->host_norm, sizeof(float) * kMlpNorm, hipMemcpyHostToDevice, m->stream));
+  }
+  MHIPCHK(m, hipStreamSynchronize(m->stream));
+  return 0;
+}
+
+int imexsw_mlp_get_params(imexsw_mlp* m, float* params, float* norm) {
+  if (!m) return mfail(m, "null argument");
+  if (params) MHIPCHK(m, hipMemcpy(params, m->w, sizeof(float) * kMlpParams, hipMemcpyDeviceToHost));
+  if (norm)
+    for (int f = 0; f < kMlpNorm; f++) norm[f] = m->host_norm[f];
+  return 0;
+}
+
+// inference on raw features x[n][9] -> raw targets y[n][2]
+int imexsw_mlp_forward(imexsw_mlp* m, const float* x, int64_t n, float* y) {
+  if (!m || !x || !y || n < 0) return mfail(m, "bad argument");
+  if (n == 0) return 0;
+  std::vector<float> xs, ts;
+  standardise(m->host_norm, x, nullptr, (size_t)n, xs, ts, kMlpIn);
+  if (mlp_data(m, (size_t)n)) return -1;
+  MHIPCHK(m, hipMemcpyAsync(m->x, xs.data(), xs.size() * sizeof(float), hipMemcpyHostToDevice, m->stream));
+  hipLaunchKernelGGL(k_mlp_forward, dim3((unsigned)((n + 63) / 64)), dim3(64), 0, m->stream,
+                     mlp_dev(m), m->x, (int)n, m->t);
+  std::vector<float> ys((size_t)n * kMlpOut);
+  MHIPCHK(m, hipMemcpyAsync(ys.data(), m->t, ys.size() * sizeof(float), hipMemcpyDeviceToHost, m->stream));
+  MHIPCHK(m, hipStreamSynchronize(m->stream));
+  const float* on = m->host_norm + 2 * kMlpIn;
+  for (int64_t s = 0; s < n; s++)
+    for (int o = 0; o < kMlpOut; o++)
+      y[s * kMlpOut + o] = ys[s * kMlpOut + o] * on[kMlpOut + o] + on[o];
+  return 0;
+}
+
+// loss and parameter gradient of one minibatch, in the standardised space
+// (x[n][9], t[n][2] already standardised), for gradient checks
+int imexsw_mlp_grad(imexsw_mlp* m, const float* x, const float* t, int32_t n, float* grad,
+                    float* loss) {
+  if (!m || !x || !t || n < 1) return mfail(m, "bad argument");
+  if (mlp_reserve(m, n) || mlp_data(m, (size_t)n)) return -1;
+  std::vector<float> xs((size_t)n * kMlpInPad, 0.f);
+  for (int s = 0; s < n; s++)
+    for (int f = 0; f < kMlpIn; f++) xs[(size_t)s * kMlpInPad + f] = x[(size_t)s * kMlpIn + f];
+  MHIPCHK(m, hipMemcpyAsync(m->x, xs.data(), xs.size() * sizeof(float), hipMemcpyHostToDevice, m->stream));
+  MHIPCHK(m, hipMemcpyAsync(m->t, t, sizeof(float) * kMlpOut * n, hipMemcpyHostToDevice, m->stream));
+  MHIPCHK(m, hipMemsetAsync(m->loss, 0, sizeof(float), m->stream));
+  const MlpTrainBuf T = mlp_buf(m);
+  hipLaunchKernelGGL(k_mlp_fwd_bwd, dim3((n + 63) / 64), dim3(64), 0, m->stream, mlp_dev(m), T,
+                     m->x, m->t, n);
+  hipLaunchKernelGGL(k_mlp_wgrad, dim3(40), dim3(64), 0, m->stream, T, n);
+  if (grad) MHIPCHK(m, hipMemcpyAsync(grad, m->grad, sizeof(float) * kMlpParams, hipMemcpyDeviceToHost, m->stream));
+  float l = 0.f;
+  MHIPCHK(m, hipMemcpyAsync(&l, m->loss, sizeof(float), hipMemcpyDeviceToHost, m->stream));
+  MHIPCHK(m, hipStreamSynchronize(m->stream));
+  if (loss) *loss = l;
+  return 0;
+}
+
+// Train on raw pairs[n][11] (9 features, 2 targets): the standardisation is
+// taken from the data, then `epochs` passes of Adam over minibatches of
+// `batch` samples in a seeded shuffled order.  loss_hist[e] = mean minibatch
+// loss of epoch e (standardised units).
+int imexsw_mlp_train(imexsw_mlp* m, const float* pairs, int64_t n, int32_t epochs, int32_t batch,
+                     float lr, uint64_t seed, float* loss_hist) {
+  if (!m || !pairs || n < 1 || epochs < 1 || batch < 64) return mfail(m, "bad argument");
+  // standardisation from the data
+  double mean[kHarvestWidth] = {0}, var[kHarvestWidth] = {0};
+  for (int64_t s = 0; s < n; s++)
+    for (int f = 0; f < kHarvestWidth; f++) mean[f] += pairs[s * kHarvestWidth + f];
+  for (int f = 0; f < kHarvestWidth; f++) mean[f] /= (double)n;
+  for (int64_t s = 0; s < n; s++)
+    for (int f = 0; f < kHarvestWidth; f++) {
+      const double d = pairs[s * kHarvestWidth + f] - mean[f];
+      var[f] += d * d;
+    }
+  float norm[kMlpNorm];
+  for (int f = 0; f < kMlpIn; f++) {
+    norm[f] = (float)mean[f];
+    norm[kMlpIn + f] = (float)std::max(std::sqrt(var[f] / (double)n), 1e-6);
+  }
+  for (int o = 0; o < kMlpOut; o++) {
+    norm[2 * kMlpIn + o] = (float)mean[kMlpIn + o];
+    norm[2 * kMlpIn + kMlpOut + o] = (float)std::max(std::sqrt(var[kMlpIn + o] / (double)n), 1e-6);
+  }
+  if (imexsw_mlp_set_params(m, nullptr, norm)) return -1;
+  std::vector<float> xs, ts;
+  standardise(norm, pairs, pairs + kMlpIn, (size_t)n, xs, ts, kHarvestWidth);
+  if (mlp_reserve(m, batch) || mlp_data(m, (size_t)n)) return -1;
+  std::vector<int64_t> perm((size_t)n);
+  for (int64_t s = 0; s < n; s++) perm[s] = s;
+  uint64_t st = seed;
+  std::vector<float> xe((size_t)n * kMlpInPad), te((size_t)n * kMlpOut);
+  const MlpTrainBuf T = mlp_buf(m);
+  const int nb = (int)((n + batch - 1) / batch);
+  for (int ep = 0; ep < epochs; ep++) {
+    for (int64_t s = n - 1; s > 0; s--) std::swap(perm[s], perm[splitmix(st) % (uint64_t)(s + 1)]);
+    for (int64_t s = 0; s < n; s++) {
+      std::memcpy(&xe[s * kMlpInPad], &xs[perm[s] * kMlpInPad], sizeof(float) * kMlpInPad);
+      std::memcpy(&te[s * kMlpOut], &ts[perm[s] * kMlpOut], sizeof(float) * kMlpOut);
+    }
+    MHIPCHK(m, hipMemcpyAsync(m->x, xe.data(), xe.size() * sizeof(float), hipMemcpyHostToDevice, m->stream));
+    MHIPCHK(m, hipMemcpyAsync(m->t, te.data(), te.size() * sizeof(float), hipMemcpyHostToDevice, m->stream));
+    MHIPCHK(m, hipMemsetAsync(m->loss, 0, sizeof(float), m->stream));
+    for (int b = 0; b < nb; b++) {
+      const int64_t off = (int64_t)b * batch;
+      const int cnt = (int)std::min<int64_t>(batch, n - off);
+      hipLaunchKernelGGL(k_mlp_fwd_bwd, dim3((cnt + 63) / 64), dim3(64), 0, m->stream, mlp_dev(m),
+                         T, m->x + off * kMlpInPad, m->t + off * kMlpOut, cnt);
+      hipLaunchKernelGGL(k_mlp_wgrad, dim3(40), dim3(64), 0, m->stream, T, cnt);
+      m->steps++;
+      const float b1 = 0.9f, b2 = 0.999f;
+      const float c1 = 1.f - std::pow(b1, (float)m->steps), c2 = 1.f - std::pow(b2, (float)m->steps);
+      hipLaunchKernelGGL(k_mlp_adam, dim3((kMlpParams + 255) / 256), dim3(256), 0, m->stream, m->w,
+                         m->wb, m->grad, m->m1, m->m2, lr, b1, b2, 1e-8f, c1, c2);
+    }
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return mfail(m, "training launch: %s", hipGetErrorString(e));
+    float l = 0.f;
+    MHIPCHK(m, hipMemcpyAsync(&l, m->loss, sizeof(float), hipMemcpyDeviceToHost, m->stream));
+    MHIPCHK(m, hipStreamSynchronize(m->stream));
+    if (loss_hist) loss_hist[ep] = l / (float)nb;
+  }
+  return 0;
+}
+
+// layout probe of the matrix-core operands: D = A . B for 16x16 bf16 inputs
+// (row-major bit patterns), fp32 D
+int imexsw_mfma_probe(const uint16_t* A, const uint16_t* B, float* D) {
+  short *a = nullptr, *b = nullptr;
+  float* d = nullptr;
+  if (hipMalloc(&a, 512) != hipSuccess || hipMalloc(&b, 512) != hipSuccess ||
+      hipMalloc(&d, 1024) != hipSuccess)
+    return mfail(nullptr, "probe allocation failed");
+  (void)hipMemcpy(a, A, 512, hipMemcpyHostToDevice);
+  (void)hipMemcpy(b, B, 512, hipMemcpyHostToDevice);
+  hipLaunchKernelGGL(k_mfma_probe, dim3(1), dim3(64), 0, 0, a, b, d);
+  hipError_t e = hipMemcpy(D, d, 1024, hipMemcpyDeviceToHost);
+  (void)hipFree(a);
+  (void)hipFree(b);
+  (void)hipFree(d);
+  return e == hipSuccess ? 0 : mfail(nullptr, "probe: %s", hipGetErrorString(e));
+}
+
+// harvest: converged Newton pairs of every following stage-2 update are
+// appended to a device buffer of `capacity` pairs (0 stops and frees)
+int imexsw_harvest(imexsw_ctx* c, int64_t capacity) {
+  if (!c || capacity < 0) return fail(c, "bad argument");
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (c->D.harvest) {
+    (void)hipFree(c->D.harvest);
+    c->D.harvest = nullptr;
+    c->D.harvest_cap = 0;
+  }
+  if (capacity > 0) {
+    HIPCHK(c, hipMalloc(&c->D.harvest, sizeof(float) * kHarvestWidth * (size_t)capacity));
+    c->D.harvest_cap = capacity;
+  }
+  HIPCHK(c, hipMemsetAsync(c->D.counters + C_HARVEST * kSlots, 0, sizeof(unsigned long long), c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+// copy out the harvested pairs (at most cap); n = pairs produced so far
+// (may exceed the capacity: the rest were dropped)
+int imexsw_harvest_get(imexsw_ctx* c, float* pairs, int64_t cap, int64_t* n) {
+  if (!c) return fail(c, "null argument");
+  unsigned long long cnt = 0;
+  HIPCHK(c, hipMemcpyAsync(&cnt, c->D.counters + C_HARVEST * kSlots, sizeof cnt, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (n) *n = (int64_t)cnt;
+  const int64_t k = std::min<int64_t>(std::min<int64_t>((int64_t)cnt, c->D.harvest_cap), cap);
+  if (pairs && k > 0 && c->D.harvest)
+    HIPCHK(c, hipMemcpy(pairs, c->D.harvest, sizeof(float) * kHarvestWidth * (size_t)k, hipMemcpyDeviceToHost));
+  return 0;
+}
+
+// learned warm start for the following stage-2 Newton solves (NULL: off)
+int imexsw_set_warm_start(imexsw_ctx* c, imexsw_mlp* m) {
+  if (!c) return fail(c, "null argument");
+  if (!m) {
+    c->warm = false;
+    return 0;
+  }
+  if (!c->D.guess && dalloc(c, &c->D.guess, 2 * c->plane)) return -1;
+  c->mlp = mlp_dev(m);
+  c->warm = true;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return 0;
+}
 
 }  // extern "C"

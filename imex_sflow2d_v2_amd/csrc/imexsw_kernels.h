@@ -17,7 +17,7 @@
 //              topography changes)
 #pragma once
 
-#include "imexsw_newton.h"
+#include "imexsw_mlp.h"
 
 namespace imexsw {
 
@@ -65,6 +65,9 @@ struct DevPtrs {
   uint8_t* tile_flag; // per work tile: any cell to compute
   int* tile_list;     // work tiles with a cell to compute (compacted)
   int* row_list;      // tile rows (tile * kTileH + row) with a cell to compute
+  double* guess;      // 2 planes: learned Newton start for the momenta (NaN: none)
+  float* harvest;     // harvest_cap pairs of kHarvestWidth floats, or null
+  long long harvest_cap;
 };
 
 enum ScalarSlot { S_T = 0, S_DT = 1, S_DT_OLD = 2, S_DT_OLD_OLD = 3, S_T_END = 4,
@@ -72,8 +75,9 @@ enum ScalarSlot { S_T = 0, S_DT = 1, S_DT_OLD = 2, S_DT_OLD_OLD = 3, S_T_END = 4
 enum CounterSlot { C_SOLVE_CELLS = 0, C_NEWTON_CELLS, C_NEWTON_ITERS, C_NONCONV,
                    C_NAN, C_NEGH, C_NEGS, C_SUMS, C_LOWT, C_LS_EVALS,
                    C_HIST0, C_HIST1, C_HIST2, C_HIST3, C_HIST4, C_HIST5, C_HIST6,
-                   C_HIST7, C_ITMAX, C_STEPS, C_ACTIVE_UPD, C_COMPUTE_CELLS,
-                   C_TILES, C_ROWS, C_NCOUNTERS };
+                   C_HIST7, C_WARM_USED, C_WARM_FALLBACK, C_ITMAX, C_STEPS,
+                   C_ACTIVE_UPD, C_COMPUTE_CELLS, C_TILES, C_ROWS, C_HARVEST,
+                   C_NCOUNTERS };
 // k_local's per-cell event counters, in CounterSlot order from C_NEWTON_CELLS
 constexpr int kLocalEvents = C_ITMAX - C_NEWTON_CELLS;  // summed ones
 // Newton iteration histogram buckets: 1, 2, 3, 4, 5-8, 9-16, 17-32, 33-100
@@ -844,7 +848,8 @@ __global__ void k_reset_step(DevParams P, DevPtrs D) { reset_member(P, D); }
 // deposition (update_erosion_deposition_cell :2380-2595) and the post-step
 // loop (src/IMEX_SfloW2D.f90:391-447), fused per cell.
 // ---------------------------------------------------------------------------
-enum LocalOps { OP_RK = 1, OP_EROSION = 2, OP_POST = 4, OP_SUMS = 8 };
+enum LocalOps { OP_RK = 1, OP_EROSION = 2, OP_POST = 4, OP_SUMS = 8, OP_WARM = 16,
+                OP_HARVEST = 32 };
 
 // one cell of k_local; e (the wavefront's counters in LDS) accumulates
 // counter C_NEWTON_CELLS + m in e[m]
@@ -874,8 +879,33 @@ __device__ __forceinline__ void local_cell(const DevParams& P, const DevPtrs& D,
     const double bxx = D.Bxx ? D.Bxx[c] : 0.0, bxy = D.Bxy ? D.Bxy[c] : 0.0,
                  byy = D.Byy ? D.Byy[c] : 0.0;
     const double gc = D.gcoef ? D.gcoef[c] : 1.0;
-    int nonconv = 0, ls = 0;
-    int it = stage2<NS>(P, q0, Dr, dt, bx, by, bxx, bxy, byy, gc, q, nonconv, ls);
+    int nonconv = 0, ls = 0, wu = -1;
+    double wg[2];
+    if (ops & OP_WARM) {
+      wg[0] = D.guess[c];
+      wg[1] = D.guess[pl + c];
+    }
+    float hv[kHarvestWidth];
+    bool hok = false;
+    int it = stage2<NS>(P, q0, Dr, dt, bx, by, bxx, bxy, byy, gc, q, nonconv, ls,
+                        (ops & OP_WARM) ? wg : nullptr, &wu,
+                        (ops & OP_HARVEST) ? hv : nullptr, &hok);
+    if (wu == 1) atomicAdd(&e[C_WARM_USED - C_NEWTON_CELLS], 1ull);
+    if (wu == 0) atomicAdd(&e[C_WARM_FALLBACK - C_NEWTON_CELLS], 1ull);
+    if (ops & OP_HARVEST) {
+      // wave-aggregated append of the converged pairs
+      const unsigned long long b = __ballot(hok);
+      unsigned long long base = 0;
+      if ((threadIdx.x & 63) == __ffsll((long long)b) - 1)
+        base = atomicAdd(ctr(D, C_HARVEST, 0), (unsigned long long)__popcll(b));
+      base = __shfl(base, __ffsll((long long)b) - 1, 64);
+      if (hok) {
+        const unsigned long long at =
+            base + __popcll(b & ((1ull << (threadIdx.x & 63)) - 1ull));
+        if ((long long)at < D.harvest_cap)
+          for (int i = 0; i < kHarvestWidth; i++) D.harvest[at * kHarvestWidth + i] = hv[i];
+      }
+    }
     if (it > 0) {
       atomicAdd(&e[C_NEWTON_CELLS - C_NEWTON_CELLS], 1ull);
       atomicAdd(&e[C_NEWTON_ITERS - C_NEWTON_CELLS], (unsigned long long)it);
@@ -1036,6 +1066,84 @@ __device__ __forceinline__ void local_row(const DevParams& P, const DevPtrs& D, 
       for (int off = 32; off > 0; off >>= 1) v += shfl_d(v, lane ^ off);
       if (lane == 0) D.tile_sums[4 * (size_t)row + m] = v;
     }
+  }
+}
+
+// Learned Newton starting points for this step: for every computed cell of
+// the listed rows, the stage-2 predictor, its features, and the MLP on the
+// matrix cores (the 64 lanes of a wavefront are the 64 samples); the
+// starting momenta q_si * (1 + prediction) go to the guess planes (NaN where
+// the cell does not enter Newton).
+template <int NS>
+__global__ void __launch_bounds__(64) k_mlp_guess(DevParams P, DevPtrs D, MlpDev M) {
+  constexpr int NV = 4 + NS;
+  __shared__ short xs[64][kMlpInPad];
+  __shared__ float ys[64][kMlpOut];
+  const int lane = threadIdx.x, i = lane & 15, g = lane >> 4;
+  if (D.scal[S_DONE] != 0.0) return;
+  const int nrows = (int)*ctr(D, C_ROWS, 0);
+  const int tiles_x = (P.nx + kStripW - 1) / kStripW;
+  const size_t pl = plane(P);
+  const double dt = D.scal[S_DT];
+  for (int it = blockIdx.x; it < nrows; it += gridDim.x) {
+    const int row = D.row_list[it];
+    const int tile = row / kTileH;
+    const int j = (tile % tiles_x) * kStripW - 2 + lane;
+    const int k = (tile / tiles_x) * kTileH + row % kTileH;
+    const bool own = lane >= 2 && lane < 2 + kStripW && j < P.nx && k < P.ny;
+    const size_t c = own ? cidx(P, j, k) : 0;
+    bool valid = false;
+    double s1 = 0.0, s2 = 0.0;
+    float f[kMlpIn];
+#pragma unroll
+    for (int m = 0; m < kMlpIn; m++) f[m] = 0.f;
+    if (own && (D.mask[c] & M_COMPUTE)) {
+      double q0[NV], Dr[NV], q_fv[NV], qp[NV + 2], q_si[NV];
+#pragma unroll
+      for (int m = 0; m < NV; m++) {
+        q0[m] = D.q[m * pl + c];
+        Dr[m] = D.D[m * pl + c];
+      }
+      const double bx = D.Bx[c], by = D.By[c];
+      const double bxx = D.Bxx ? D.Bxx[c] : 0.0, bxy = D.Bxy ? D.Bxy[c] : 0.0,
+                   byy = D.Byy ? D.Byy[c] : 0.0;
+      const double gc = D.gcoef ? D.gcoef[c] : 1.0;
+      if (predictor<NS>(P, q0, Dr, dt, bx, by, bxx, bxy, byy, gc, q_fv, qp, q_si)) {
+        valid = true;
+        s1 = q_si[1];
+        s2 = q_si[2];
+        mlp_features_raw<NS>(P, q_fv, qp, q_si, dt, bx, by, f);
+      }
+    }
+#pragma unroll
+    for (int m = 0; m < kMlpInPad; m++)
+      xs[lane][m] = (valid && m < kMlpIn)
+                        ? f2bf((f[m] - M.norm[m]) / M.norm[kMlpIn + m])
+                        : (short)0;
+    __syncthreads();
+    bf16x4 xb[4];
+    input_operands(xs, xb);
+    f32x4 y[4];
+    mlp_forward_wave(M, xb, y, nullptr);
+    if (g == 0)
+#pragma unroll
+      for (int nb = 0; nb < 4; nb++)
+#pragma unroll
+        for (int o = 0; o < kMlpOut; o++) ys[nb * 16 + i][o] = y[nb][o];
+    __syncthreads();
+    if (own) {
+      double g1 = __builtin_nan(""), g2 = __builtin_nan("");
+      if (valid) {
+        const float* on = M.norm + 2 * kMlpIn;
+        const double a = (double)(ys[lane][0] * on[2] + on[0]);
+        const double b = (double)(ys[lane][1] * on[3] + on[1]);
+        g1 = s1 * (1.0 + a);
+        g2 = s2 * (1.0 + b);
+      }
+      D.guess[c] = g1;
+      D.guess[pl + c] = g2;
+    }
+    __syncthreads();
   }
 }
 

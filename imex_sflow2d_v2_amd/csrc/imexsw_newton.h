@@ -68,7 +68,9 @@ template <int NS>
 __device__ __forceinline__ NewtonOut solve_rk_step(const DevParams& P, double* qj,
                                                   const double* qold,
                                                   const double* Rj, double a_diag,
-                                                  double dt, double Bx, double By) {
+                                                  double dt, double Bx, double By,
+                                                  const double* rt0 = nullptr,
+                                                  double scal0 = 0.0) {
   constexpr int NV = 4 + NS;
   const double TOLF = 1.0e-10;
   const double TOLX = kEps;
@@ -85,6 +87,13 @@ __device__ __forceinline__ NewtonOut solve_rk_step(const DevParams& P, double* q
   // first step); the reference evaluates it again at the top of the
   // iteration, with the same operands, so that evaluation is skipped
   bool fresh = false;
+  if (rt0) {
+    // residual at the starting point already evaluated by the caller
+#pragma unroll
+    for (int i = 0; i < NV; i++) rt[i] = rt0[i];
+    scal = scal0;
+    fresh = true;
+  }
   for (it = 1; it <= 100; it++) {
     if (!fresh) eval_f<NS>(P, qj, qold, Rj, a_diag, dt, Bx, By, rt, scal);
     bool restored = false;
@@ -265,29 +274,39 @@ __device__ __forceinline__ NewtonOut solve_rk_step(const DevParams& P, double* q
   return out;
 }
 
-// Stage 2 of the N_RK = 2 IMEX scheme for one cell (src/solver_2d.f90:
-// 1105-1280), from q0 = q^n and D = divFlux - expl_terms of stage 1.  The
-// implicit term refresh of :1234-1236 is not evaluated: for the stiffly
-// accurate N_RK = 2 tableau it is overwritten at :1271-1277 and never enters
-// q.  Returns the Newton iteration count (0: no Newton).
+// Raw warm-start features of one cell entering Newton: log h, the
+// predictor's velocity (u, v), T, three solid volume fractions, log dt and
+// the bed slope magnitude.
 template <int NS>
-__device__ __forceinline__ int stage2(const DevParams& P, const double* q0,
-                                      const double* D, double dt, double Bx,
-                                      double By, double Bxx, double Bxy,
-                                      double Byy, double gc, double* q,
-                                      int& nonconv, int& ls_evals) {
+__device__ __forceinline__ void mlp_features_raw(const DevParams& P, const double* q_fv,
+                                                 const double* qp, const double* q_si,
+                                                 double dt, double Bx, double By, float* f) {
+  const double h = qp[0];
+  f[0] = (float)log(h);
+  f[1] = (float)(q_si[1] / q_fv[0]);
+  f[2] = (float)(q_si[2] / q_fv[0]);
+  f[3] = (float)qp[3];
+#pragma unroll
+  for (int s = 0; s < 3; s++)
+    f[4 + s] = (s < NS) ? (float)(P.alpha ? qp[4 + s] : qp[4 + s] / h) : 0.f;
+  f[7] = (float)log(dt);
+  f[8] = (float)sqrt(Bx * Bx + By * By);
+}
+
+// The stage-2 predictor (src/solver_2d.f90:1132-1196): q_fv = q0 - dt*D, its
+// physical variables, and the semi-implicit Coulomb step q_si realigned with
+// q_fv.  Returns false when q_fv has no mass (no Newton solve).
+template <int NS>
+__device__ __forceinline__ bool predictor(const DevParams& P, const double* q0,
+                                          const double* D, double dt, double Bx,
+                                          double By, double Bxx, double Bxy, double Byy,
+                                          double gc, double* q_fv, double* qp,
+                                          double* q_si) {
   constexpr int NV = 4 + NS;
-  double q_fv[NV], qp[NV + 2], q_si[NV];
 #pragma unroll
   for (int i = 0; i < NV; i++) q_fv[i] = q0[i] - dt * (D[i] - 0.0);
   phys<NS>(P, q_fv, qp);
-  nonconv = 0;
-  ls_evals = 0;
-  if (!(q_fv[0] > 0.0)) {
-#pragma unroll
-    for (int i = 0; i < NV; i++) q[i] = q_fv[i];
-    return 0;
-  }
+  if (!(q_fv[0] > 0.0)) return false;
   double s1, s2;
   semi_impl<NS>(P, Bx, By, Bxx, Bxy, Byy, gc, qp, s1, s2);
 #pragma unroll
@@ -308,6 +327,31 @@ __device__ __forceinline__ int stage2(const DevParams& P, const double* q0,
     q_si[1] = m * f2 / den;
     q_si[2] = m * f3 / den;
   }
+  return true;
+}
+
+// Stage 2 of the N_RK = 2 IMEX scheme for one cell (src/solver_2d.f90:
+// 1105-1280), from q0 = q^n and D = divFlux - expl_terms of stage 1.  The
+// implicit term refresh of :1234-1236 is not evaluated: for the stiffly
+// accurate N_RK = 2 tableau it is overwritten at :1271-1277 and never enters
+// q.  Returns the Newton iteration count (0: no Newton).
+template <int NS>
+__device__ __forceinline__ int stage2(const DevParams& P, const double* q0,
+                                      const double* D, double dt, double Bx,
+                                      double By, double Bxx, double Bxy,
+                                      double Byy, double gc, double* q,
+                                      int& nonconv, int& ls_evals,
+                                      const double* warm = nullptr, int* warm_used = nullptr,
+                                      float* harvest = nullptr, bool* harvested = nullptr) {
+  constexpr int NV = 4 + NS;
+  double q_fv[NV], qp[NV + 2], q_si[NV];
+  nonconv = 0;
+  ls_evals = 0;
+  if (!predictor<NS>(P, q0, D, dt, Bx, By, Bxx, Bxy, Byy, gc, q_fv, qp, q_si)) {
+#pragma unroll
+    for (int i = 0; i < NV; i++) q[i] = q_fv[i];
+    return 0;
+  }
   double Rj[NV];
 #pragma unroll
   for (int i = 0; i < NV; i++) {
@@ -315,9 +359,41 @@ __device__ __forceinline__ int stage2(const DevParams& P, const double* q0,
     Rj[i] = (D[i] - 0.0) - 1.0 * si;
     q[i] = q_si[i];
   }
-  NewtonOut no = solve_rk_step<NS>(P, q, q0, Rj, 1.0, dt, Bx, By);
+  NewtonOut no;
+  if (warm && warm[0] == warm[0] && warm[1] == warm[1]) {
+    // learned starting point for the momenta: Newton starts from whichever
+    // of it and the plain predictor has the smaller residual (the plain one
+    // stays the fallback), and the equation solved is the same
+    double qw[NV], rtw[NV], rtp[NV], sw, sp;
+#pragma unroll
+    for (int i = 0; i < NV; i++) qw[i] = q_si[i];
+    qw[1] = warm[0];
+    qw[2] = warm[1];
+    eval_f<NS>(P, qw, q0, Rj, 1.0, dt, Bx, By, rtw, sw);
+    eval_f<NS>(P, q, q0, Rj, 1.0, dt, Bx, By, rtp, sp);
+    const bool use = sw < sp;
+    *warm_used = use ? 1 : 0;
+    if (use) {
+#pragma unroll
+      for (int i = 0; i < NV; i++) q[i] = qw[i];
+    }
+    no = solve_rk_step<NS>(P, q, q0, Rj, 1.0, dt, Bx, By, use ? rtw : rtp, use ? sw : sp);
+  } else {
+    if (warm_used) *warm_used = -1;
+    no = solve_rk_step<NS>(P, q, q0, Rj, 1.0, dt, Bx, By);
+  }
   nonconv = !no.conv;
   ls_evals = no.ls_evals;
+  if (harvest && no.conv) {
+    // (state entering Newton, converged momenta) pair for the warm start
+    float f[9];
+    mlp_features_raw<NS>(P, q_fv, qp, q_si, dt, Bx, By, f);
+#pragma unroll
+    for (int i = 0; i < 9; i++) harvest[i] = f[i];
+    harvest[9] = (q_si[1] != 0.0) ? (float)(q[1] / q_si[1] - 1.0) : 0.f;
+    harvest[10] = (q_si[2] != 0.0) ? (float)(q[2] / q_si[2] - 1.0) : 0.f;
+    *harvested = true;
+  }
   if (P.ny == 1) q[2] = 0.0;
   if (P.nx == 1) q[1] = 0.0;
   if (q_si[1] * q_si[1] + q_si[2] * q_si[2] == 0.0) {

@@ -22,6 +22,7 @@ _D = C.POINTER(C.c_double)
 _U8 = C.POINTER(C.c_uint8)
 _I32 = C.POINTER(C.c_int32)
 _I64 = C.POINTER(C.c_int64)
+_F = C.POINTER(C.c_float)
 
 
 class ImexswStats(C.Structure):
@@ -31,6 +32,7 @@ class ImexswStats(C.Structure):
         "neg_solid_events", "sum_solid_events", "low_T_events",
         "line_search_evals")] + [
         ("newton_iter_hist", C.c_int64 * 8), ("newton_iter_max", C.c_int64),
+        ("warm_cells", C.c_int64), ("warm_fallbacks", C.c_int64),
         ("solve_cells", C.c_int32), ("compute_cells", C.c_int32)]
 
 
@@ -74,6 +76,20 @@ ABI = {
     "imexsw_ensemble_autotune": (C.c_int, [C.c_void_p, C.c_int32, C.c_double, _D,
                                            C.c_double, C.POINTER(C.c_int)]),
     "imexsw_ensemble_last_ms": (C.c_double, [C.c_void_p]),
+    "imexsw_mlp_num_params": (C.c_int, []),
+    "imexsw_mlp_create": (C.c_int, [C.POINTER(C.c_void_p), C.c_uint64]),
+    "imexsw_mlp_destroy": (None, [C.c_void_p]),
+    "imexsw_mlp_last_error": (C.c_char_p, [C.c_void_p]),
+    "imexsw_mlp_set_params": (C.c_int, [C.c_void_p, _F, _F]),
+    "imexsw_mlp_get_params": (C.c_int, [C.c_void_p, _F, _F]),
+    "imexsw_mlp_forward": (C.c_int, [C.c_void_p, _F, C.c_int64, _F]),
+    "imexsw_mlp_grad": (C.c_int, [C.c_void_p, _F, _F, C.c_int32, _F, _F]),
+    "imexsw_mlp_train": (C.c_int, [C.c_void_p, _F, C.c_int64, C.c_int32, C.c_int32,
+                                   C.c_float, C.c_uint64, _F]),
+    "imexsw_mfma_probe": (C.c_int, [C.POINTER(C.c_uint16), C.POINTER(C.c_uint16), _F]),
+    "imexsw_harvest": (C.c_int, [C.c_void_p, C.c_int64]),
+    "imexsw_harvest_get": (C.c_int, [C.c_void_p, _F, C.c_int64, C.POINTER(C.c_int64)]),
+    "imexsw_set_warm_start": (C.c_int, [C.c_void_p, C.c_void_p]),
     "imexsw_sync": (C.c_int, [C.c_void_p]),
 }
 

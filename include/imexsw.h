@@ -142,6 +142,8 @@ typedef struct imexsw_stats {
   int64_t newton_iter_hist[8];  /* cells by Newton iterations: 1, 2, 3, 4,
                                    5-8, 9-16, 17-32, 33-100 */
   int64_t newton_iter_max;      /* most iterations any cell took */
+  int64_t warm_cells;           /* Newton solves started from the learned guess */
+  int64_t warm_fallbacks;       /* ... that fell back to the plain predictor */
   int32_t solve_cells;          /* active cells of the last check_solve */
   int32_t compute_cells;        /* of those, cells whose update is computed
                                    (the rest provably keep their state) */
@@ -324,6 +326,55 @@ int imexsw_ensemble_autotune(imexsw_ensemble *ens, int32_t steps_per_trial, doub
 /* Device time of the last imexsw_ensemble_advance, milliseconds (HIP
  * events on the ensemble's stream). */
 double imexsw_ensemble_last_ms(const imexsw_ensemble *ens);
+
+/* ---------------------------------------------------------------------------
+ * Learned Newton warm start (no reference counterpart: an addition).  A
+ * per-cell MLP 9 -> 64 -> 64 -> 64 -> 2 (GELU, BF16 weights, FP32
+ * accumulation on the matrix cores) predicts the converged momenta of the
+ * stage-2 Newton solve (src/solver_2d.f90:1207-1212) from the state that
+ * enters it.  Newton then starts from whichever of the prediction and the
+ * plain predictor q_si (:1196) has the smaller residual and still iterates
+ * to the reference tolerance, so the plain method stays both fallback and
+ * judge.  Features: log h, predictor velocity u, v, T, three solid volume
+ * fractions, log dt, |grad B|; targets: q*_x / q_si_x - 1, q*_y / q_si_y - 1.
+ * ------------------------------------------------------------------------- */
+typedef struct imexsw_mlp imexsw_mlp;
+
+int imexsw_mlp_num_params(void);
+/* He-normal initial weights, deterministic in seed. */
+int imexsw_mlp_create(imexsw_mlp **mlp, uint64_t seed);
+void imexsw_mlp_destroy(imexsw_mlp *mlp);
+const char *imexsw_mlp_last_error(const imexsw_mlp *mlp);
+/* params: imexsw_mlp_num_params() floats (W1[64][16] b1[64] W2[64][64] b2[64]
+ * W3[64][64] b3[64] W4[16][64] b4[16]); norm: 22 floats (input mean[9],
+ * input std[9], target mean[2], target std[2]).  Either may be NULL. */
+int imexsw_mlp_set_params(imexsw_mlp *mlp, const float *params, const float *norm);
+int imexsw_mlp_get_params(imexsw_mlp *mlp, float *params, float *norm);
+/* Inference: raw features x[n][9] -> raw targets y[n][2]. */
+int imexsw_mlp_forward(imexsw_mlp *mlp, const float *x, int64_t n, float *y);
+/* Loss (mean over samples of the squared error summed over the outputs) and
+ * its parameter gradient for one minibatch given in the standardised space:
+ * x[n][9], t[n][2]; grad: imexsw_mlp_num_params() floats. */
+int imexsw_mlp_grad(imexsw_mlp *mlp, const float *x, const float *t, int32_t n, float *grad,
+                    float *loss);
+/* Train on raw pairs[n][11] (9 features then 2 targets): standardisation
+ * from the data, Adam (lr) over seeded shuffled minibatches of `batch`
+ * samples; loss_hist[epochs] (may be NULL). */
+int imexsw_mlp_train(imexsw_mlp *mlp, const float *pairs, int64_t n, int32_t epochs,
+                     int32_t batch, float lr, uint64_t seed, float *loss_hist);
+/* D = A . B on the matrix cores for 16 x 16 row-major bf16 bit patterns
+ * (operand-layout check). */
+int imexsw_mfma_probe(const uint16_t *A, const uint16_t *B, float *D);
+
+/* Collect (features, targets) pairs of every converged stage-2 Newton solve
+ * of the following steps into a device buffer of `capacity` pairs (0 stops
+ * and frees).  imexsw_harvest_get copies out at most cap pairs and reports
+ * how many were produced. */
+int imexsw_harvest(imexsw_ctx *ctx, int64_t capacity);
+int imexsw_harvest_get(imexsw_ctx *ctx, float *pairs, int64_t cap, int64_t *n);
+/* Use `mlp` for the warm start of the following steps (NULL: off).  The MLP
+ * must outlive its use by the context. */
+int imexsw_set_warm_start(imexsw_ctx *ctx, imexsw_mlp *mlp);
 
 #ifdef __cplusplus
 }
