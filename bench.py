@@ -203,17 +203,30 @@ def bench_warmstart(args):
     states."""
     from imex_sflow2d_v2_amd import learned
     from imex_sflow2d_v2_amd.solver import Solver
-    harvest_steps = 12
-    dw = make_deck("wet")
-    g = Solver(dw.params).load_deck(dw)
-    learned.start_harvest(g, 60_000_000)
-    t0 = time.perf_counter()
-    g.advance(harvest_steps, dt0=dw.dt0)
-    g.sync()
-    pairs, produced = learned.harvested(g)
-    learned.stop_harvest(g)
-    g.close()
-    t_h = time.perf_counter() - t0
+    from imex_sflow2d_v2_amd import decks
+    # harvest from the benchmark's configurations: the 1-D dam break, the
+    # incline collapse, a volcano on another seeded DEM (the evaluation DEM
+    # stays unseen) and one step of the fully wet grid
+    sources = [("dambreak_1d", decks.dambreak_1d(), 400),
+               ("incline_512", decks.incline_2d(), 100),
+               ("volcano_seed5", decks.volcano_2048(seed=5), 1000),
+               ("wet_2048", decks.wet_2048(), 1)]
+    chunks, produced, t_h, harvest_info = [], 0, 0.0, []
+    for name, dd, ns in sources:
+        g = Solver(dd.params).load_deck(dd)
+        learned.start_harvest(g, 30_000_000)
+        t0 = time.perf_counter()
+        g.advance(ns, dt0=dd.dt0)
+        g.sync()
+        pp, nprod = learned.harvested(g)
+        t_h += time.perf_counter() - t0
+        learned.stop_harvest(g)
+        g.close()
+        chunks.append(pp)
+        produced += nprod
+        harvest_info.append({"deck": name, "steps": ns, "pairs": int(pp.shape[0])})
+    pairs = np.concatenate(chunks, axis=0)
+    del chunks
     m = learned.Mlp(seed=1)
     t0 = time.perf_counter()
     hist = m.train(pairs, epochs=3, batch=8192, lr=1e-3, seed=2)
@@ -273,8 +286,7 @@ def bench_warmstart(args):
         "data": "synthetic",
         "config": {
             "workload": "warmstart",
-            "harvest": {"deck": "wet_2048x2048", "steps": harvest_steps, "pairs": int(produced),
-                        "seconds": t_h},
+            "harvest": {"sources": harvest_info, "pairs": int(produced), "seconds": t_h},
             "training": {"epochs": 3, "batch": 8192, "lr": 1e-3, "loss_history": [float(x) for x in hist],
                          "seconds": t_train},
             "results": res,
@@ -362,7 +374,8 @@ def main():
     dom = max(((k, v) for k, v in kt.items() if v[1] > 0), key=lambda kv: kv[1][0])
     dname, (dms, dcalls) = dom
     avg_s = dms / dcalls * 1e-3
-    units = active / steps if dname in ("stencil", "local") else cells
+    computed = st["cell_updates_computed"]
+    units = computed / steps if dname in ("stencil", "local") else cells
     alg_bytes = kernel_bytes(p, dname) * units
     achieved = alg_bytes / avg_s / 1e9
     traffic = parse_traffic(dname, args.workload)
@@ -386,6 +399,7 @@ def main():
             "dx": p.cell_size,
             "active_cell_updates_per_s": active / el,
             "active_fraction": active / float(cells * steps),
+            "computed_cell_updates_per_s": computed / el,
             "newton_iters_per_cell": st["newton_iters"] / max(1, st["newton_cells"]),
             "newton_iter_hist_1_2_3_4_8_16_32_100": st["newton_iter_hist"],
             "newton_iter_max": st["newton_iter_max"],

@@ -184,9 +184,10 @@ void launch_local(imexsw_ctx* c, int ops) {
   // work items: listed tile rows, four per workgroup; the stage update and
   // the erosion/diagnostics part are two kernels
   const int items = (c->n_tiles * kTileH + 3) / 4;
-  static int res0 = 0, res1 = 0;
+  static int res0 = 0, res1 = 0, res2 = 0;
   if (!res0) res0 = resident_grid(k_local<NS, 0>, 256, c->n_cu, 1 << 30);
   if (!res1) res1 = resident_grid(k_local<NS, 1>, 256, c->n_cu, 1 << 30);
+  if (!res2) res2 = resident_grid(k_local<NS, 2>, 256, c->n_cu, 1 << 30);
   if (ops & OP_RK) {
     int rk = OP_RK;
     if (c->warm) {
@@ -197,8 +198,12 @@ void launch_local(imexsw_ctx* c, int ops) {
       rk |= OP_WARM;
     }
     if (c->D.harvest) rk |= OP_HARVEST;
-    hipLaunchKernelGGL((k_local<NS, 0>), dim3(std::min(items, res0)), dim3(256), 0, c->stream,
-                       c->P, c->D, rk);
+    if (rk == OP_RK)
+      hipLaunchKernelGGL((k_local<NS, 0>), dim3(std::min(items, res0)), dim3(256), 0, c->stream,
+                         c->P, c->D, rk);
+    else
+      hipLaunchKernelGGL((k_local<NS, 2>), dim3(std::min(items, res2)), dim3(256), 0, c->stream,
+                         c->P, c->D, rk);
   }
   if (ops & (OP_EROSION | OP_POST | OP_SUMS))
     hipLaunchKernelGGL((k_local<NS, 1>), dim3(std::min(items, res1)), dim3(256), 0, c->stream,
@@ -865,6 +870,7 @@ int imexsw_stats_get(imexsw_ctx* c, imexsw_stats* st, int reset) {
   std::memset(st, 0, sizeof *st);
   st->steps = (int64_t)v[C_STEPS];
   st->cell_updates_active = (int64_t)v[C_ACTIVE_UPD];
+  st->cell_updates_computed = (int64_t)v[C_COMPUTE_UPD];
   st->cell_updates_grid = (int64_t)v[C_STEPS] * c->P.nx * c->P.ny;
   st->newton_cells = (int64_t)v[C_NEWTON_CELLS];
   st->newton_iters = (int64_t)v[C_NEWTON_ITERS];
@@ -1508,6 +1514,27 @@ int imexsw_mlp_grad(imexsw_mlp* m, const float* x, const float* t, int32_t n, fl
 int imexsw_mlp_train(imexsw_mlp* m, const float* pairs, int64_t n, int32_t epochs, int32_t batch,
                      float lr, uint64_t seed, float* loss_hist) {
   if (!m || !pairs || n < 1 || epochs < 1 || batch < 64) return mfail(m, "bad argument");
+  // rows with a non-finite value are left out
+  std::vector<float> kept;
+  {
+    int64_t good = 0;
+    for (int64_t s = 0; s < n; s++) {
+      bool ok = true;
+      for (int f = 0; f < kHarvestWidth; f++) ok = ok && std::isfinite(pairs[s * kHarvestWidth + f]);
+      good += ok;
+    }
+    if (good < n) {
+      kept.reserve((size_t)good * kHarvestWidth);
+      for (int64_t s = 0; s < n; s++) {
+        bool ok = true;
+        for (int f = 0; f < kHarvestWidth; f++) ok = ok && std::isfinite(pairs[s * kHarvestWidth + f]);
+        if (ok) kept.insert(kept.end(), pairs + s * kHarvestWidth, pairs + (s + 1) * kHarvestWidth);
+      }
+      pairs = kept.data();
+      n = good;
+      if (n < 1) return mfail(m, "no finite training pairs");
+    }
+  }
   // standardisation from the data
   double mean[kHarvestWidth] = {0}, var[kHarvestWidth] = {0};
   for (int64_t s = 0; s < n; s++)

@@ -77,7 +77,7 @@ enum CounterSlot { C_SOLVE_CELLS = 0, C_NEWTON_CELLS, C_NEWTON_ITERS, C_NONCONV,
                    C_HIST0, C_HIST1, C_HIST2, C_HIST3, C_HIST4, C_HIST5, C_HIST6,
                    C_HIST7, C_WARM_USED, C_WARM_FALLBACK, C_ITMAX, C_STEPS,
                    C_ACTIVE_UPD, C_COMPUTE_CELLS, C_TILES, C_ROWS, C_HARVEST,
-                   C_NCOUNTERS };
+                   C_COMPUTE_UPD, C_NCOUNTERS };
 // k_local's per-cell event counters, in CounterSlot order from C_NEWTON_CELLS
 constexpr int kLocalEvents = C_ITMAX - C_NEWTON_CELLS;  // summed ones
 // Newton iteration histogram buckets: 1, 2, 3, 4, 5-8, 9-16, 17-32, 33-100
@@ -824,10 +824,14 @@ __device__ __forceinline__ void advance_member(const DevPtrs& D) {
   double* s = D.scal;
   if (s[S_DONE] != 0.0) return;
   s[S_T] = s[S_T] + s[S_DT];
-  unsigned long long n = 0;
-  for (int i = 0; i < kSlots; i++) n += *ctr(D, C_SOLVE_CELLS, i);
+  unsigned long long n = 0, nc = 0;
+  for (int i = 0; i < kSlots; i++) {
+    n += *ctr(D, C_SOLVE_CELLS, i);
+    nc += *ctr(D, C_COMPUTE_CELLS, i);
+  }
   *ctr(D, C_STEPS, 0) += 1;
   *ctr(D, C_ACTIVE_UPD, 0) += n;
+  *ctr(D, C_COMPUTE_UPD, 0) += nc;
 }
 __global__ void k_advance_time(DevPtrs D) { advance_member(D); }
 
@@ -855,8 +859,9 @@ enum LocalOps { OP_RK = 1, OP_EROSION = 2, OP_POST = 4, OP_SUMS = 8, OP_WARM = 1
 // counter C_NEWTON_CELLS + m in e[m]
 // PH selects the part compiled in: 0 = stage update and assembly checks
 // (OP_RK), 1 = erosion/deposition and diagnostics (OP_EROSION, OP_POST,
-// OP_SUMS).  Split in two kernels, each part keeps a register footprint
-// that lets two wavefronts share a SIMD.
+// OP_SUMS), 2 = the stage update with the learned warm start and the pair
+// harvest (OP_WARM, OP_HARVEST).  Separate kernels keep the plain stage
+// update's register footprint free of the learned part.
 template <int NS, int PH>
 __device__ __forceinline__ void local_cell(const DevParams& P, const DevPtrs& D, size_t c,
                                            int ops, unsigned long long* e, int& imax,
@@ -868,7 +873,7 @@ __device__ __forceinline__ void local_cell(const DevParams& P, const DevPtrs& D,
   double q[NV], qp[NV + 2];
 #pragma unroll
   for (int i = 0; i < NV; i++) q[i] = D.q[i * pl + c];
-  if (PH == 0 && (ops & OP_RK)) {
+  if ((PH == 0 || PH == 2) && (ops & OP_RK)) {
     double q0[NV], Dr[NV];
 #pragma unroll
     for (int i = 0; i < NV; i++) {
@@ -881,18 +886,18 @@ __device__ __forceinline__ void local_cell(const DevParams& P, const DevPtrs& D,
     const double gc = D.gcoef ? D.gcoef[c] : 1.0;
     int nonconv = 0, ls = 0, wu = -1;
     double wg[2];
-    if (ops & OP_WARM) {
+    if (PH == 2 && (ops & OP_WARM)) {
       wg[0] = D.guess[c];
       wg[1] = D.guess[pl + c];
     }
     float hv[kHarvestWidth];
     bool hok = false;
     int it = stage2<NS>(P, q0, Dr, dt, bx, by, bxx, bxy, byy, gc, q, nonconv, ls,
-                        (ops & OP_WARM) ? wg : nullptr, &wu,
-                        (ops & OP_HARVEST) ? hv : nullptr, &hok);
+                        (PH == 2 && (ops & OP_WARM)) ? wg : nullptr, &wu,
+                        (PH == 2 && (ops & OP_HARVEST)) ? hv : nullptr, &hok);
     if (wu == 1) atomicAdd(&e[C_WARM_USED - C_NEWTON_CELLS], 1ull);
     if (wu == 0) atomicAdd(&e[C_WARM_FALLBACK - C_NEWTON_CELLS], 1ull);
-    if (ops & OP_HARVEST) {
+    if (PH == 2 && (ops & OP_HARVEST)) {
       // wave-aggregated append of the converged pairs
       const unsigned long long b = __ballot(hok);
       unsigned long long base = 0;

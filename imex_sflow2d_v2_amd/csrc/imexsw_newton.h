@@ -385,14 +385,22 @@ __device__ __forceinline__ int stage2(const DevParams& P, const double* q0,
   nonconv = !no.conv;
   ls_evals = no.ls_evals;
   if (harvest && no.conv) {
-    // (state entering Newton, converged momenta) pair for the warm start
+    // (state entering Newton, converged momenta) pair for the warm start;
+    // a momentum component below 1e-6 of the predictor's magnitude gets a
+    // zero target, and pairs with non-finite values are not recorded
     float f[9];
     mlp_features_raw<NS>(P, q_fv, qp, q_si, dt, Bx, By, f);
+    const double mag = sqrt(q_si[1] * q_si[1] + q_si[2] * q_si[2]);
+    bool ok = mag > 0.0;
 #pragma unroll
-    for (int i = 0; i < 9; i++) harvest[i] = f[i];
-    harvest[9] = (q_si[1] != 0.0) ? (float)(q[1] / q_si[1] - 1.0) : 0.f;
-    harvest[10] = (q_si[2] != 0.0) ? (float)(q[2] / q_si[2] - 1.0) : 0.f;
-    *harvested = true;
+    for (int i = 0; i < 9; i++) {
+      harvest[i] = f[i];
+      ok = ok && isfinite(f[i]);
+    }
+    harvest[9] = (fabs(q_si[1]) > 1e-6 * mag) ? (float)(q[1] / q_si[1] - 1.0) : 0.f;
+    harvest[10] = (fabs(q_si[2]) > 1e-6 * mag) ? (float)(q[2] / q_si[2] - 1.0) : 0.f;
+    ok = ok && isfinite(harvest[9]) && isfinite(harvest[10]);
+    *harvested = ok;
   }
   if (P.ny == 1) q[2] = 0.0;
   if (P.nx == 1) q[1] = 0.0;

@@ -27,7 +27,8 @@ _F = C.POINTER(C.c_float)
 
 class ImexswStats(C.Structure):
     _fields_ = [(n, C.c_int64) for n in (
-        "steps", "cell_updates_active", "cell_updates_grid", "newton_cells",
+        "steps", "cell_updates_active", "cell_updates_grid", "cell_updates_computed",
+        "newton_cells",
         "newton_iters", "newton_nonconverged", "nan_events", "neg_h_events",
         "neg_solid_events", "sum_solid_events", "low_T_events",
         "line_search_evals")] + [

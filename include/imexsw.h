@@ -130,6 +130,9 @@ typedef struct imexsw_stats {
   int64_t steps;
   int64_t cell_updates_active;  /* sum over steps of solve_cells */
   int64_t cell_updates_grid;    /* nx*ny per step */
+  int64_t cell_updates_computed;/* sum over steps of the computed cells (the
+                                   active cells minus those that provably
+                                   keep their state) */
   int64_t newton_cells;         /* cells that entered the Newton solve */
   int64_t newton_iters;         /* total Newton iterations */
   int64_t newton_nonconverged;  /* cells that hit max_nl_iter */
