@@ -166,11 +166,17 @@ void launch_stencil(imexsw_ctx* c, int compute_dt) {
   // work items: (listed tile, strip) pairs, at most tiles * kTileH / row_unit
   const int items = c->n_tiles * (kTileH / c->row_unit);
   if (c->P.ny > 1) {
-    static int res = 0;
+    // strips taller than one row, and one-row strips (the kernel that finds
+    // the step's strip height is not its own returns at once)
+    static int res = 0, resr = 0;
     if (!res) res = resident_grid(k_stencil<NS, true>, 64, c->n_cu, 1 << 30);
+    if (!resr) resr = resident_grid(k_stencil_rows<NS>, 192, c->n_cu, 1 << 30);
     hipLaunchKernelGGL((k_stencil<NS, true>), dim3(std::min(items, res)), dim3(64), 0,
                        c->stream, c->P, c->D, c->row_unit, c->strip_h, c->target_strips,
                        compute_dt);
+    hipLaunchKernelGGL((k_stencil_rows<NS>), dim3(std::min(c->n_tiles * kTileH, resr)),
+                       dim3(192), 0, c->stream, c->P, c->D, c->row_unit, c->strip_h,
+                       c->target_strips, compute_dt);
   } else {
     static int res = 0;
     if (!res) res = resident_grid(k_stencil<NS, false>, 64, c->n_cu, 1 << 30);
@@ -249,7 +255,9 @@ int run_local(imexsw_ctx* c, int ops, int kid) {
 int run_mask(imexsw_ctx* c, int solve_all) {
   {
     Timed tm(c, K_MASK);
-    hipLaunchKernelGGL(k_mask, c->grid_mask, dim3(64), 0, c->stream, c->P, c->D, solve_all);
+    const dim3 grid(c->grid_mask.x, (c->grid_mask.y + kMaskGroup - 1) / kMaskGroup);
+    hipLaunchKernelGGL(k_mask, grid, dim3(64 * kMaskGroup), 0, c->stream, c->P, c->D,
+                       solve_all);
   }
   return check_launch(c, "k_mask");
 }
@@ -475,6 +483,7 @@ int imexsw_create(const imexsw_params* params, imexsw_ctx** out) {
   rc |= dalloc(c, &D.q, c->nv * pl);
   rc |= dalloc(c, &D.D, c->nv * pl);
   rc |= dalloc(c, &D.mask, pl);
+  rc |= dalloc(c, &D.sflag, pl);
   rc |= dalloc(c, &D.B, pl);
   rc |= dalloc(c, &D.Bx, pl);
   rc |= dalloc(c, &D.By, pl);
@@ -601,23 +610,38 @@ int imexsw_set_source(imexsw_ctx* c, const int32_t* source_cell, const uint8_t* 
   if (!c || !source_cell || !dir_bits || !vect) return fail(c, "null argument");
   const DevParams& P = c->P;
   DevPtrs& D = c->D;
-  if (!D.scell) {
-    int8_t* sc = nullptr;
-    uint8_t* sb = nullptr;
-    double* sv = nullptr;
-    if (dalloc(c, &sc, c->plane) || dalloc(c, &sb, c->plane) || dalloc(c, &sv, 8 * c->plane)) return -1;
-    D.scell = sc;
-    D.sbits = sb;
-    D.svect = sv;
-  }
-  std::vector<int8_t> sc8((size_t)P.nx * P.ny);
-  for (size_t i = 0; i < sc8.size(); i++) {
+  // one source byte per cell (source_cell, and for ring cells the boundary
+  // directions) and the direction vector of the first direction a ring
+  // cell has (E, W, S, N order, the order the stencil tests them in)
+  const size_t n = (size_t)P.nx * P.ny;
+  std::vector<uint8_t> info(n);
+  std::vector<double> ring(2 * n, 0.0);
+  for (size_t i = 0; i < n; i++) {
     if (source_cell[i] < 0 || source_cell[i] > 2) return fail(c, "source_cell values must be 0, 1 or 2");
-    sc8[i] = (int8_t)source_cell[i];
+    uint8_t b = source_cell[i] == 1 ? SI_SRC : source_cell[i] == 2 ? SI_RING : 0;
+    if (source_cell[i] == 2) {
+      const uint8_t d = dir_bits[i];
+      const int o = (d & IMEXSW_SRC_E) ? 0 : (d & IMEXSW_SRC_W) ? 2 : (d & IMEXSW_SRC_S) ? 4
+                    : (d & IMEXSW_SRC_N) ? 6 : -1;
+      b |= ((d & IMEXSW_SRC_E) ? SI_E : 0) | ((d & IMEXSW_SRC_W) ? SI_W : 0) |
+           ((d & IMEXSW_SRC_S) ? SI_S : 0) | ((d & IMEXSW_SRC_N) ? SI_N : 0);
+      if (o >= 0) {
+        b |= SI_DIR;
+        ring[2 * i] = vect[8 * i + o];
+        ring[2 * i + 1] = vect[8 * i + o + 1];
+      }
+    }
+    info[i] = b;
   }
-  if (upload_plane(c, sc8.data(), const_cast<int8_t*>(D.scell), P.nx, P.ny, P.pitch)) return -1;
-  if (upload_plane(c, dir_bits, const_cast<uint8_t*>(D.sbits), P.nx, P.ny, P.pitch)) return -1;
-  if (upload_aos(c, vect, 8, const_cast<double*>(D.svect))) return -1;
+  if (!D.sinfo) {
+    uint8_t* si = nullptr;
+    double* sr = nullptr;
+    if (dalloc(c, &si, c->plane) || dalloc(c, &sr, 2 * c->plane)) return -1;
+    D.sinfo = si;
+    D.sring = sr;
+  }
+  if (upload_plane(c, info.data(), const_cast<uint8_t*>(D.sinfo), P.nx, P.ny, P.pitch)) return -1;
+  if (upload_aos(c, ring.data(), 2, const_cast<double*>(D.sring))) return -1;
   c->d_valid = false;
   return 0;
 }
@@ -625,6 +649,9 @@ int imexsw_set_source(imexsw_ctx* c, const int32_t* source_cell, const uint8_t* 
 int imexsw_upload_q(imexsw_ctx* c, const double* q) {
   if (!c || !q) return fail(c, "null argument");
   if (upload_aos(c, q, c->nv, c->D.q)) return -1;
+  hipLaunchKernelGGL(k_sflag, c->grid_tiles, dim3(256), 0, c->stream, c->P, c->D);
+  if (check_launch(c, "k_sflag")) return -1;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
   c->d_valid = false;
   return 0;
 }
@@ -1000,8 +1027,8 @@ void launch_batch_step(imexsw_ensemble* e) {
   const int per = kTileH / e->strip_h;
   const dim3 mgrid((M + 255) / 256);
   hipLaunchKernelGGL(kb_reset, mgrid, dim3(256), 0, e->stream, e->d_params, e->d_ptrs, L);
-  hipLaunchKernelGGL(kb_mask, dim3(e->total_tiles), dim3(64), 0, e->stream, e->d_params,
-                     e->d_ptrs, L);
+  hipLaunchKernelGGL(kb_mask, dim3((e->total_tiles + kMaskGroup - 1) / kMaskGroup),
+                     dim3(64 * kMaskGroup), 0, e->stream, e->d_params, e->d_ptrs, L);
   static int rs = 0, rl0 = 0, rl1 = 0;
   if (!rs) rs = resident_grid(kb_stencil<NS>, 64, e->n_cu, 1 << 30);
   if (!rl0) rl0 = resident_grid(kb_local<NS, 0>, 256, e->n_cu, 1 << 30);

@@ -35,10 +35,12 @@ __global__ void kb_reset(const DevParams* __restrict__ Ps, const DevPtrs* __rest
   if (m < L.members) reset_member(Ps[m], Ds[m]);
 }
 
-// one workgroup per tile of every member (global tile id = blockIdx.x)
-__global__ void __launch_bounds__(64) kb_mask(const DevParams* __restrict__ Ps,
-                                               const DevPtrs* __restrict__ Ds, BatchLists L) {
-  const int g = blockIdx.x;
+// one wavefront per tile of every member (global tile id g), kMaskGroup
+// consecutive tiles per workgroup
+__global__ void __launch_bounds__(64 * kMaskGroup) kb_mask(const DevParams* __restrict__ Ps,
+                                                           const DevPtrs* __restrict__ Ds,
+                                                           BatchLists L) {
+  const int g = blockIdx.x * kMaskGroup + (threadIdx.x >> 6);
   int lo = 0, hi = L.members - 1;  // member with tile_off[m] <= g < tile_off[m+1]
   while (lo < hi) {
     const int mid = (lo + hi + 1) >> 1;
@@ -48,11 +50,13 @@ __global__ void __launch_bounds__(64) kb_mask(const DevParams* __restrict__ Ps,
   const int m = lo;
   const DevParams& P = Ps[m];
   const DevPtrs& D = Ds[m];
-  if (D.scal[S_DONE] != 0.0) return;
   const int tiles_x = (P.nx + kStripW - 1) / kStripW;
   const int t = g - L.tile_off[m];
+  unsigned rows_m = 0;
+  if (g < L.tile_off[L.members] && D.scal[S_DONE] == 0.0)
+    rows_m = mask_tile(P, D, t % tiles_x, t / tiles_x, tiles_x, 0);
   const ListSink sink{L.tiles, L.rows, &L.counts[0], &L.counts[1], m << kMemberShift};
-  mask_tile(P, D, t % tiles_x, t / tiles_x, tiles_x, 0, sink);
+  append_group<kMaskGroup>(sink, t, rows_m);
 }
 
 // (tile, strip) items of all members, strip height fixed for the batch
@@ -70,8 +74,9 @@ __global__ void __launch_bounds__(64) kb_stencil(const DevParams* __restrict__ P
     const int tiles_x = (P.nx + kStripW - 1) / kStripW;
     const int k0 = (tile / tiles_x) * kTileH + (item % per) * strip_h;
     if (k0 >= P.ny) continue;
-    stencil_strip<NS, true>(P, D, (tile % tiles_x) * kStripW - 2 + (int)threadIdx.x, k0,
-                            min(k0 + strip_h, P.ny), P.cfl != -1.0);
+    stencil_strip<NS, true>(P, D, src_time(P, D.scal[S_T]),
+                            (tile % tiles_x) * kStripW - 2 + (int)threadIdx.x, k0,
+                            min(k0 + strip_h, P.ny), true, P.cfl != -1.0);
   }
 }
 

@@ -37,9 +37,9 @@ struct DevPtrs {
   double* q;         // NV planes
   double* D;         // NV planes
   uint8_t* mask;     // 1 plane (bytes)
-  const int8_t* scell;   // source_cell plane (0/1/2) or null
-  const uint8_t* sbits;  // ring direction bits or null
-  const double* svect;   // 8 planes or null
+  uint8_t* sflag;    // 1 plane (bytes): StateFlags of the current state
+  const uint8_t* sinfo;  // SourceBits plane (source_cell and ring direction) or null
+  const double* sring;   // 2 planes: the ring cell's direction vector, or null
   double* B;         // B_cent
   double* Bx;        // B_prime_x_geom
   double* By;
@@ -95,12 +95,24 @@ __device__ __forceinline__ unsigned long long* ctr(const DevPtrs& D, int m, int 
 // mask byte: bit 0 = check_solve's active flag (solve_mask), bit 1 = the
 // cell's update has to be computed.  An active cell whose update is known
 // to leave it unchanged is "quiet": it and every cell of its reconstruction
-// stencil (two cells along x and along y) hold h == 0, no source-ring cell
-// is in that stencil, its own state is all zeros and no boundary condition
-// within reach can feed mass in.  Its faces then carry dry states (zero
+// stencil (two cells along x and along y) hold an all-zero state, no source
+// cell is in that stencil and no boundary condition within reach can feed
+// mass in.  Its faces then carry dry states (zero
 // flux, zero wave speed) and the stage update, erosion/deposition and
 // diagnostics return the state and accumulators as they are.
 enum MaskBits { M_ACTIVE = 1, M_COMPUTE = 2 };
+// source byte of a cell: source_cell == 1 (interior), source_cell == 2
+// (ring), the ring's boundary directions and whether one of them is set
+// (the direction vector sring is then the one of the first of E, W, S, N)
+enum SourceBits { SI_SRC = 1, SI_RING = 2, SI_E = 4, SI_W = 8, SI_S = 16, SI_N = 32,
+                  SI_DIR = 64 };
+// state flag byte, kept next to the state by every kernel that writes it, so
+// that k_mask reads one byte per cell instead of the state planes
+enum StateFlags { SF_ZERO = 1,   // every conservative variable is 0.0
+                  SF_POS = 2 };  // h > 0
+__device__ __forceinline__ uint8_t state_flags(double h, bool all_zero) {
+  return (uint8_t)((all_zero ? SF_ZERO : 0) | (h > 0.0 ? SF_POS : 0));
+}
 
 __device__ __forceinline__ size_t cidx(const DevParams& P, int j, int k) {
   return (size_t)k * P.pitch + j;
@@ -119,6 +131,12 @@ __device__ __forceinline__ double ycomp(const DevParams& P, int k) {
   return 0.5 * (ystag(P, k) + ystag(P, k + 1));
 }
 
+// A value that has to be in registers at this point of the program: the
+// load that produces it is not moved past it (into a branch further down).
+__device__ __forceinline__ void pin(double& x) { asm volatile("" : "+v"(x)); }
+__device__ __forceinline__ void pin(int& x) { asm volatile("" : "+v"(x)); }
+__device__ __forceinline__ void pin(unsigned long long& x) { asm volatile("" : "+v"(x)); }
+
 // ---------------------------------------------------------------------------
 // wave shuffles of doubles (ds_bpermute on 32-bit halves)
 // ---------------------------------------------------------------------------
@@ -130,12 +148,15 @@ __device__ __forceinline__ double shfl_d(double v, int src) {
 }
 
 // ---------------------------------------------------------------------------
-// k_mask: check_solve (src/solver_2d.f90:666-857), one workgroup per work
-// tile.  The seed and wet flags of the tile plus a two-cell halo are staged
-// in LDS (lane l loads column 60*tx-2+l of 20 rows), then the N_RK-fold
-// 4-neighbour dilation (an L1 diamond of radius N_RK) and the quiet test
-// read LDS only.  Tiles with a cell to compute are appended to tile_list.
+// k_mask: check_solve (src/solver_2d.f90:666-857), one wavefront per work
+// tile and kMaskGroup tiles (stacked along y) per workgroup.  Lane l reads
+// the state flags of column 60*tx-2+l over the tile's 16 rows and a two-row
+// halo; the N_RK-fold 4-neighbour dilation (an L1 diamond of radius N_RK)
+// and the quiet test are bit operations on those 20-bit columns.  Tiles and
+// tile rows with a cell to compute are appended to the work lists, one list
+// reservation per workgroup.
 // ---------------------------------------------------------------------------
+constexpr int kMaskGroup = 4;
 // Where k_mask appends the tiles and tile rows that have a cell to compute:
 // entries are `tag | tile` and `tag | (tile * kTileH + row)`.
 struct ListSink {
@@ -150,115 +171,164 @@ __device__ __forceinline__ ListSink own_sink(const DevPtrs& D) {
   return ListSink{D.tile_list, D.row_list, ctr(D, C_TILES, 0), ctr(D, C_ROWS, 0), 0};
 }
 
-// Bit-parallel check_solve for one tile (tx, ty), one 64-lane wavefront:
-// lane l holds column 60*tx-2+l, so a 64-bit ballot is one tile row with its
-// two-cell halo on either side.  Per row the seed mask (h > 0, boundary,
-// source ring) dilated to the L1 diamond of radius N_RK = 2 -- the set the
-// reference's two 4-neighbour dilation passes produce -- is
-//   S(r) | S(r)<<>>1 | S(r)<<>>2 | S(r+-1) | S(r+-1)<<>>1 | S(r+-2)
-// and the quiet test ORs the wet masks over the cross of radius 2.
-__device__ __forceinline__ unsigned long long dil(unsigned long long m, int d) {
-  unsigned long long r = m;
+// Bit-parallel check_solve for one tile (tx, ty), one 64-lane wavefront.
+// Lane l holds column 60*tx-2+l and keeps its 20 rows (the tile and a
+// two-row halo, row r = bit r) as bit words: seeds S (h > 0, boundary,
+// source ring), wet W (a non-zero state or any source cell), source
+// interior I.  The
+// reference's two 4-neighbour dilation passes reach the L1 diamond of
+// radius N_RK = 2, which per lane is
+//   V2(S) | V1(S of lanes l+-1) | S of lanes l+-2,
+// with V1, V2 the vertical dilations by one and two bits; the quiet test
+// ORs W over the radius-2 cross.  Everything is 32-bit lane arithmetic plus
+// a few cross-lane reads.
+__device__ __forceinline__ unsigned vdil(unsigned m, int d) {
+  unsigned r = m;
   if (d >= 1) r |= (m << 1) | (m >> 1);
   if (d >= 2) r |= (m << 2) | (m >> 2);
   return r;
 }
+__device__ __forceinline__ unsigned lane_read(unsigned v, int src) {
+  return (unsigned)__shfl((int)v, src & 63, 64);
+}
 
-__device__ __forceinline__ void mask_tile(const DevParams& P, const DevPtrs& D, int tx,
-                                          int tyy, int tiles_x, int solve_all,
-                                          const ListSink& L) {
+// Writes the tile's mask bytes, counters and tile flag; returns the
+// (wavefront-uniform) bit word of the tile rows with a cell to compute.
+__device__ __forceinline__ unsigned mask_tile(const DevParams& P, const DevPtrs& D, int tx,
+                                              int tyy, int tiles_x, int solve_all) {
   constexpr int LH = kTileH + 4;
   const int lane = threadIdx.x & 63;
   const int j = tx * kStripW - 2 + lane;
   const int k0 = tyy * kTileH;
   const bool injx = (j >= 0 && j < P.nx);
   const bool own = injx && lane >= 2 && lane < 2 + kStripW;
-  const bool radial = P.radial && D.scell;
-  // all staging loads first (rows k0-2 .. k0+17 of this column)
-  double hv[LH];
-  int scv[LH];
+  const bool radial = P.radial && D.sinfo;
+  unsigned S = 0, W = 0, I = 0, rowin = 0;
+  // unconditional loads from clamped (always valid) addresses, selected
+  // afterwards: straight-line code lets all loads be in flight at once
+  const int jc = min(max(j, 0), P.nx - 1);
+  int fv[LH], sv[LH];
+#pragma unroll
+  for (int r = 0; r < LH; r++) {
+    const int kc = min(max(k0 - 2 + r, 0), P.ny - 1);
+    fv[r] = D.sflag[cidx(P, jc, kc)];
+    sv[r] = radial ? D.sinfo[cidx(P, jc, kc)] : 0;
+  }
 #pragma unroll
   for (int r = 0; r < LH; r++) {
     const int kk = k0 - 2 + r;
     const bool in = injx && kk >= 0 && kk < P.ny;
-    hv[r] = in ? D.q[cidx(P, j, kk)] : 0.0;
-    scv[r] = (in && radial) ? D.scell[cidx(P, j, kk)] : 0;
-  }
-  const unsigned long long own_m = __ballot(own);
-  const unsigned long long edge_m = __ballot(injx && (j == 0 || j == P.nx - 1));
-  const unsigned long long west_m = __ballot(injx && j <= 1);
-  const unsigned long long east_m = __ballot(injx && j >= P.nx - 2);
-  // rolling window of seed / wet / source-interior masks, rows r-2 .. r+2
-  unsigned long long Sm[5], Wm[5], Im[5];
-  auto row_masks = [&](int r, unsigned long long& s, unsigned long long& w,
-                       unsigned long long& in1) {
-    const int kk = k0 - 2 + r;
-    const bool in = injx && kk >= 0 && kk < P.ny;
+    const int f = in ? fv[r] : SF_ZERO;
+    const int sc = in ? sv[r] : 0;
     const bool bnd = in && (j == 0 || j == P.nx - 1 || kk == 0 || kk == P.ny - 1);
-    s = __ballot(in && (bnd || solve_all || hv[r] > 0.0 || scv[r] == 2));
-    w = __ballot(in && (!(hv[r] == 0.0) || scv[r] != 0));
-    in1 = __ballot(in && scv[r] == 1);
-  };
+    S |= (in && (bnd || solve_all || (f & SF_POS) || (sc & SI_RING))) ? (1u << r) : 0u;
+    // "wet" for the quiet test: any non-zero variable or a source cell
+    W |= (in && (!(f & SF_ZERO) || (sc & (SI_SRC | SI_RING)))) ? (1u << r) : 0u;
+    I |= (in && (sc & SI_SRC)) ? (1u << r) : 0u;
+    rowin |= (kk >= 0 && kk < P.ny) ? (1u << r) : 0u;
+  }
+  // active: diamond of radius 2 around the seeds, boundary cells, minus the
+  // source interior; bits 2..17 are the tile's rows
+  const unsigned v1 = vdil(S, 1);
+  unsigned act = vdil(S, 2) | lane_read(v1, lane - 1) | lane_read(v1, lane + 1) |
+                 lane_read(S, lane - 2) | lane_read(S, lane + 2);
+  // lane reads wrap at the wavefront's ends: lanes 0, 1, 62, 63 are halo
+  // columns whose results are not used
+  unsigned edge_rows = 0;  // rows k == 0 or k == ny-1
+  unsigned lo_rows = 0, hi_rows = 0;  // rows k <= 1, k >= ny-2
 #pragma unroll
-  for (int r = 0; r < 4; r++) row_masks(r, Sm[r + 1], Wm[r + 1], Im[r + 1]);
-  const size_t pl = plane(P);
-  int count = 0, ccount = 0;
-  unsigned rows_m = 0;
+  for (int r = 0; r < LH; r++) {
+    const int kk = k0 - 2 + r;
+    if (kk == 0 || kk == P.ny - 1) edge_rows |= 1u << r;
+    if (kk <= 1) lo_rows |= 1u << r;
+    if (kk >= P.ny - 2) hi_rows |= 1u << r;
+  }
+  if (j == 0 || j == P.nx - 1) act = ~0u;
+  act |= edge_rows;
+  act &= ~I & rowin;
+  if (!own) act = 0;
+  // busy: not quiet
+  unsigned busy = W | lane_read(W, lane - 1) | lane_read(W, lane + 1) |
+                  lane_read(W, lane - 2) | lane_read(W, lane + 2);
+  if (P.ny > 1) busy |= vdil(W, 2);
+  if (j <= 1 && !P.bc_quiet[IMEXSW_W]) busy = ~0u;
+  if (j >= P.nx - 2 && !P.bc_quiet[IMEXSW_E]) busy = ~0u;
+  if (P.ny > 1) {
+    if (!P.bc_quiet[IMEXSW_S]) busy |= lo_rows;
+    if (!P.bc_quiet[IMEXSW_N]) busy |= hi_rows;
+  }
+  const unsigned rows16 = ((1u << kTileH) - 1u) << 2;
+  act &= rows16;
+  const unsigned comp = act & busy;
+  if (own) {
 #pragma unroll
-  for (int rr = 0; rr < kTileH; rr++) {
-#pragma unroll
-    for (int i = 0; i < 4; i++) {
-      Sm[i] = Sm[i + 1];
-      Wm[i] = Wm[i + 1];
-      Im[i] = Im[i + 1];
+    for (int rr = 0; rr < kTileH; rr++) {
+      const int k = k0 + rr;
+      if (k < P.ny) {
+        const unsigned bit = 1u << (rr + 2);
+        D.mask[cidx(P, j, k)] =
+            (uint8_t)(((act & bit) ? M_ACTIVE : 0) | ((comp & bit) ? M_COMPUTE : 0));
+      }
     }
-    row_masks(rr + 4, Sm[4], Wm[4], Im[4]);
-    const int k = k0 + rr;
-    if (k >= P.ny) break;
-    // Sm[2] is row k
-    unsigned long long act = dil(Sm[2], 2) | dil(Sm[1], 1) | dil(Sm[3], 1) | Sm[0] | Sm[4];
-    act |= edge_m;
-    if (k == 0 || k == P.ny - 1) act = ~0ull;
-    act &= own_m & ~Im[2];
-    unsigned long long busy = dil(Wm[2], 2);
-    if (P.ny > 1) busy |= Wm[0] | Wm[1] | Wm[3] | Wm[4];
-    if (!P.bc_quiet[IMEXSW_W]) busy |= west_m;
-    if (!P.bc_quiet[IMEXSW_E]) busy |= east_m;
-    if (P.ny > 1 && ((k <= 1 && !P.bc_quiet[IMEXSW_S]) || (k >= P.ny - 2 && !P.bc_quiet[IMEXSW_N])))
-      busy = ~0ull;
-    // quiet candidates: active, dry cross; their own state must be all zero
-    const bool cand = (((act & ~busy) >> lane) & 1ull) != 0;
-    bool nz = false;
-    if (cand) {
-      const size_t c = cidx(P, j, k);
-      for (int i = 1; i < P.nv; i++) nz = nz || !(D.q[i * pl + c] == 0.0);
-    }
-    const unsigned long long comp = act & (busy | __ballot(nz));
-    if (own) {
-      const int a = (int)((act >> lane) & 1ull), cm = (int)((comp >> lane) & 1ull);
-      D.mask[cidx(P, j, k)] = (uint8_t)((a ? M_ACTIVE : 0) | (cm ? M_COMPUTE : 0));
-    }
-    count += __popcll(act);
-    ccount += __popcll(comp);
-    if (comp) rows_m |= 1u << rr;
+  }
+  int count = __popc(act), ccount = __popc(comp);
+  unsigned rows_m = comp >> 2;
+  for (int off = 32; off > 0; off >>= 1) {
+    count += __shfl_xor(count, off, 64);
+    ccount += __shfl_xor(ccount, off, 64);
+    rows_m |= (unsigned)__shfl_xor((int)rows_m, off, 64);
   }
   const int tile = tyy * tiles_x + tx;
   if (lane == 0) {
     if (count) atomicAdd(ctr(D, C_SOLVE_CELLS, tile), (unsigned long long)count);
     if (ccount) atomicAdd(ctr(D, C_COMPUTE_CELLS, tile), (unsigned long long)ccount);
-    const int nrows = __popc(rows_m);
-    D.tile_flag[tile] = nrows ? 1 : 0;
-    if (nrows) {
-      L.tiles[atomicAdd(L.n_tiles, 1ull)] = L.tag | tile;
-      int at = (int)atomicAdd(L.n_rows, (unsigned long long)nrows);
-      for (int r = 0; r < kTileH; r++)
-        if (rows_m & (1u << r)) L.rows[at++] = L.tag | (tile * kTileH + r);
+    D.tile_flag[tile] = rows_m ? 1 : 0;
+  }
+  return rows_m;
+}
+
+// Appends the workgroup's tiles with a row to compute (wavefront w: tile
+// `tile`, compute rows `rows_m`, 0 for none or for an idle wavefront) to the
+// work lists with one reservation per list for the whole workgroup.  Every
+// thread of the workgroup calls it.
+template <int G>
+__device__ __forceinline__ void append_group(const ListSink& L, int tile, unsigned rows_m) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  __shared__ unsigned s_rows[G];
+  __shared__ int s_tb, s_rb;
+  if (lane == 0) s_rows[w] = rows_m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int nt = 0, nr = 0;
+    for (int g = 0; g < G; g++) {
+      nt += s_rows[g] != 0u;
+      nr += __popc(s_rows[g]);
     }
+    s_tb = nt ? (int)atomicAdd(L.n_tiles, (unsigned long long)nt) : 0;
+    s_rb = nr ? (int)atomicAdd(L.n_rows, (unsigned long long)nr) : 0;
+  }
+  __syncthreads();
+  if (rows_m) {
+    int tb = s_tb, rb = s_rb;
+    for (int g = 0; g < w; g++) {
+      tb += s_rows[g] != 0u;
+      rb += __popc(s_rows[g]);
+    }
+    if (lane == 0) L.tiles[tb] = L.tag | tile;
+    if (lane < kTileH && ((rows_m >> lane) & 1u))
+      L.rows[rb + __popc(rows_m & ((1u << lane) - 1u))] = L.tag | (tile * kTileH + lane);
   }
 }
 
-__global__ void __launch_bounds__(64) k_mask(DevParams P, DevPtrs D, int solve_all) {
-  mask_tile(P, D, blockIdx.x, blockIdx.y, gridDim.x, solve_all, own_sink(D));
+// grid (tiles_x, ceil(tiles_y / kMaskGroup)), kMaskGroup wavefronts
+__global__ void __launch_bounds__(64 * kMaskGroup) k_mask(DevParams P, DevPtrs D,
+                                                          int solve_all) {
+  const int tiles_x = (P.nx + kStripW - 1) / kStripW;
+  const int tiles_y = (P.ny + kTileH - 1) / kTileH;
+  const int tyy = blockIdx.y * kMaskGroup + (threadIdx.x >> 6);
+  unsigned rows_m = 0;
+  if (tyy < tiles_y) rows_m = mask_tile(P, D, blockIdx.x, tyy, tiles_x, solve_all);
+  append_group<kMaskGroup>(own_sink(D), tyy * tiles_x + blockIdx.x, rows_m);
 }
 
 // ---------------------------------------------------------------------------
@@ -268,7 +338,19 @@ struct SrcInfo {
   bool src2;       // source_cell == 2 (ring) and RADIAL_SOURCE_FLAG
   bool is_src;     // source_cell == 1 (source interior)
   bool sE, sW, sS, sN;
+  bool dir;        // one of sE, sW, sS, sN
 };
+__device__ __forceinline__ SrcInfo src_info(int b) {
+  SrcInfo S;
+  S.is_src = b & SI_SRC;
+  S.src2 = b & SI_RING;
+  S.sE = b & SI_E;
+  S.sW = b & SI_W;
+  S.sS = b & SI_S;
+  S.sN = b & SI_N;
+  S.dir = b & SI_DIR;
+  return S;
+}
 
 __device__ __forceinline__ void qp2_uv(const double* qp, double& u, double& v) {
   // qp_to_qp2 (src/constitutive_2d.f90:2155-2179), velocity components
@@ -453,37 +535,31 @@ __device__ __forceinline__ double face_flux(const Side<NS>& L, const Side<NS>& R
   return fmax(aP, -aN);
 }
 
-// load the ring information of one cell
-__device__ __forceinline__ SrcInfo load_src(const DevParams& P, const DevPtrs& D,
-                                            size_t c) {
-  SrcInfo S{false, false, false, false, false, false};
-  if (P.radial && D.scell) {
-    const int sc = D.scell[c];
-    S.is_src = (sc == 1);
-    S.src2 = (sc == 2);
-    if (S.src2) {
-      const uint8_t b = D.sbits[c];
-      S.sE = b & IMEXSW_SRC_E;
-      S.sW = b & IMEXSW_SRC_W;
-      S.sS = b & IMEXSW_SRC_S;
-      S.sN = b & IMEXSW_SRC_N;
-    }
+// Source byte and ring vector of cell (j, k) from a clamped, always valid
+// address (zero without a radial source).
+__device__ __forceinline__ void load_src(const DevParams& P, const DevPtrs& D, int j, int k,
+                                         int& b, double& v0, double& v1) {
+  b = 0;
+  v0 = 0.0;
+  v1 = 0.0;
+  if (P.radial && D.sinfo) {
+    const size_t c = cidx(P, min(max(j, 0), P.nx - 1), min(max(k, 0), P.ny - 1));
+    b = D.sinfo[c];
+    v0 = D.sring[c];
+    v1 = D.sring[plane(P) + c];
   }
-  return S;
 }
 
+// boundary state of a ring cell (zero elsewhere), src/solver_2d.f90:3879-3889
 template <int NS>
-__device__ __forceinline__ void ring_state(const DevParams& P, const DevPtrs& D,
-                                           size_t c, const SrcInfo& S, double t,
+__device__ __forceinline__ void ring_state(const DevParams& P, const SrcInfo& S,
+                                           const SrcTime& T, double v0, double v1,
                                            double* sb) {
   constexpr int NV = 4 + NS;
 #pragma unroll
   for (int i = 0; i < NV + 2; i++) sb[i] = 0.0;
-  if (!(P.radial && S.src2)) return;
-  const size_t pl = plane(P);
-  int o = S.sE ? 0 : S.sW ? 2 : S.sS ? 4 : S.sN ? 6 : -1;
-  if (o < 0) return;
-  source_bdry<NS>(P, t, D.svect[o * pl + c], D.svect[(o + 1) * pl + c], sb);
+  if (!(P.radial && S.src2 && S.dir)) return;
+  source_bdry<NS>(P, T, v0, v1, sb);
 }
 
 // ---------------------------------------------------------------------------
@@ -495,6 +571,31 @@ __device__ __forceinline__ void ring_state(const DevParams& P, const DevPtrs& D,
 // only supply states.  x neighbours are exchanged with wave shuffles; rows
 // k0-2 .. k1+1 are read once each.
 // ---------------------------------------------------------------------------
+// Conservative state of cell (j, k) from a clamped, always valid address:
+// loads without a branch around them, so that several rows' loads are in
+// flight together; finish_row turns them into physical variables (a dry
+// state outside the grid).
+template <int NS>
+__device__ __forceinline__ void load_raw(const DevParams& P, const DevPtrs& D, int j, int k,
+                                         double* q) {
+  constexpr int NV = 4 + NS;
+  const size_t c = cidx(P, min(max(j, 0), P.nx - 1), min(max(k, 0), P.ny - 1)), pl = plane(P);
+#pragma unroll
+  for (int i = 0; i < NV; i++) q[i] = D.q[i * pl + c];
+}
+template <int NS>
+__device__ __forceinline__ void finish_row(const DevParams& P, int j, int k, const double* q,
+                                           double* qp) {
+  constexpr int NV = 4 + NS;
+  if (j >= 0 && j < P.nx && k >= 0 && k < P.ny) {
+    phys_or_dry<NS>(P, q, qp);
+  } else {
+#pragma unroll
+    for (int i = 0; i < NV + 2; i++) qp[i] = 0.0;
+    qp[3] = P.T_ambient;
+  }
+}
+
 template <int NS>
 __device__ __forceinline__ void load_row(const DevParams& P, const DevPtrs& D,
                                          int j, int k, double* qp) {
@@ -533,56 +634,237 @@ __device__ __forceinline__ void yface_sides(const DevParams& P, const double* hi
   }
 }
 
-template <int NS, bool HASY>
-__device__ __forceinline__ void stencil_strip(const DevParams& P, const DevPtrs& D, int j,
-                                              int k0, int k1, int compute_dt);
+// Per-cell data of row k that the x pass and the explicit terms read, from
+// clamped (always valid) addresses; loaded one row ahead of its use.
+struct RowData {
+  int m;            // mask byte
+  double gW, gE;    // slope corrections of the W and E faces
+  double bx, by, bxx, bxy, byy, gc, dg;
+};
+__device__ __forceinline__ void load_rowdata(const DevParams& P, const DevPtrs& D, int j, int k,
+                                             RowData& R) {
+  const int jc = min(max(j, 0), P.nx - 1), kc = min(max(k, 0), P.ny - 1);
+  const size_t c = cidx(P, jc, kc);
+  R.m = D.mask[c];
+  R.gW = D.gsx ? D.gsx[(size_t)kc * (P.pitch + 1) + jc] : 1.0;
+  R.gE = D.gsx ? D.gsx[(size_t)kc * (P.pitch + 1) + jc + 1] : 1.0;
+  R.bx = D.Bx[c];
+  R.by = D.By[c];
+  R.bxx = D.Bxx ? D.Bxx[c] : 0.0;
+  R.bxy = D.Bxy ? D.Bxy[c] : 0.0;
+  R.byy = D.Byy ? D.Byy[c] : 0.0;
+  R.gc = D.gcoef ? D.gcoef[c] : 1.0;
+  R.dg = D.dgdx ? D.dgdx[c] : 0.0;
+}
+// y-face slope correction at the S face of row k (k = ny: the N boundary)
+__device__ __forceinline__ double load_gsy(const DevParams& P, const DevPtrs& D, int j, int k) {
+  if (!D.gsy) return 1.0;
+  return D.gsy[cidx(P, min(max(j, 0), P.nx - 1), min(max(k, 0), P.ny))];
+}
 
-// The stencil pass over the listed work tiles.  Each tile is cut into
-// strips of strip_h rows, strip_h = unit * 2^m <= max_strip (a divisor of
-// the tile height): the tallest that still leaves about `target_strips`
-// strips for this step's computed cells (short strips when few cells are
-// computed, so the sequential march stays short; tall strips on a wet grid,
-// so the two halo rows of each strip are amortised).  Workgroups take
-// (tile, strip) items in turn until the list is exhausted.
+// x direction of row k: the W face flux on every lane, the E face by
+// shuffle from the right neighbour (a boundary face on the last column).
+// S is row k's source data, sb its ring state, qk its physical state.
 template <int NS, bool HASY>
-__global__ void __launch_bounds__(64) k_stencil(DevParams P, DevPtrs D, int unit,
-                                                int max_strip, int target_strips,
-                                                int compute_dt) {
-  const int lane = threadIdx.x;
-  if (D.scal[S_DONE] != 0.0) return;
+__device__ __forceinline__ void x_fluxes(const DevParams& P, const SrcInfo& S, const double* sb,
+                                         const double* qk, int j, int k, const RowData& R,
+                                         double* HW, double* HE, double& amaxW,
+                                         double& amaxE) {
+  constexpr int NV = 4 + NS;
+  const int lane = threadIdx.x & 63;
+  const bool injx = (j >= 0 && j < P.nx);
+  double qkm[NV + 2], qkp[NV + 2];
+#pragma unroll
+  for (int i = 0; i < NV + 2; i++) {
+    qkm[i] = shfl_d(qk[i], (lane + 63) & 63);
+    qkp[i] = shfl_d(qk[i], (lane + 1) & 63);
+  }
+  // after a north ring cell the x-slope of v uses y_stag(k+1) as its
+  // third abscissa (src/solver_2d.f90:3616)
+  const bool typo = HASY && injx && P.radial && S.src2 && k > 0 && k < P.ny - 1 &&
+                    !S.sS && S.sN;
+  Side<NS> Wself, Eself;
+  if (injx) {
+    double lo[NV + 2], hi[NV + 2];
+    recon<NS, 1>(P, qk, qkm, qkp, j, P.nx, k, S, sb, typo, lo, hi);
+    if (S.is_src) {
+      zero_side<NS>(Wself);
+      zero_side<NS>(Eself);
+    } else {
+      make_side<NS, 1>(P, lo, R.gW, Wself);
+      make_side<NS, 1>(P, hi, R.gE, Eself);
+    }
+  } else {
+    zero_side<NS>(Wself);
+    zero_side<NS>(Eself);
+  }
+  Side<NS> Eleft;
+  shfl_side<NS>(Eself, Eleft, (lane + 63) & 63);
+  const int flags = (S.is_src ? 1 : 0) | (S.src2 ? 2 : 0) | (S.sE ? 4 : 0) | (S.sW ? 8 : 0);
+  const int flags_left = __shfl(flags, (lane + 63) & 63, kWave);
+  {
+    Side<NS> Ls, Rs;
+    if (j == 0) {
+      Ls = Wself;
+      Rs = Wself;
+    } else {
+      const bool interior_me = (j > 0 && j < P.nx - 1);
+      const bool interior_left = (j - 1 > 0 && j - 1 < P.nx - 1);
+      if (flags_left & 1) {
+        if (P.radial && S.src2 && !S.sE && S.sW && interior_me) Ls = Wself;
+        else zero_side<NS>(Ls);
+      } else {
+        Ls = Eleft;
+      }
+      if (S.is_src) {
+        if (P.radial && (flags_left & 2) && (flags_left & 4) && interior_left) Rs = Eleft;
+        else zero_side<NS>(Rs);
+      } else {
+        Rs = Wself;
+      }
+    }
+    amaxW = face_flux<NS, 1>(Ls, Rs, HW);
+  }
+  {
+    double HEs[NV];
+#pragma unroll
+    for (int i = 0; i < NV; i++) HEs[i] = shfl_d(HW[i], (lane + 1) & 63);
+    double amaxEs = shfl_d(amaxW, (lane + 1) & 63);
+    if (j == P.nx - 1) {
+      amaxE = face_flux<NS, 1>(Eself, Eself, HE);
+    } else {
+#pragma unroll
+      for (int i = 0; i < NV; i++) HE[i] = HEs[i];
+      amaxE = amaxEs;
+    }
+  }
+}
+
+// divergence, explicit terms and the residual D of one computed cell, and
+// its CFL time step folded into dtmin
+template <int NS, bool HASY>
+__device__ __forceinline__ void cell_update(const DevParams& P, const DevPtrs& D, size_t c,
+                                            const double* qk, const RowData& R,
+                                            const double* HW, const double* HE,
+                                            const double* HS, const double* HN, double amaxW,
+                                            double amaxE, double amaxS, double amaxN,
+                                            int compute_dt, double& dtmin) {
+  constexpr int NV = 4 + NS;
+  const size_t pl = plane(P);
+  double ex[NV];
+  expl_terms<NS>(P, R.bx, R.by, R.bxx, R.bxy, R.byy, R.gc, R.dg, qk, ex);
+#pragma unroll
+  for (int i = 0; i < NV; i++) {
+    double dv = 0.0;
+    dv = dv + (HE[i] - HW[i]) * P.one_by_dx;
+    if (HASY) dv = dv + (HN[i] - HS[i]) * P.one_by_dy;
+    D.D[i * pl + c] = (0.0 + (dv - ex[i]) * 1.0) + (0.0 - 0.0) * 0.0;
+  }
+  if (compute_dt) {
+    double ma = fmax(amaxW, amaxE);
+    const double dtx = (ma > 0.0) ? P.cfl * P.dx / ma : P.max_dt;
+    double dty = P.max_dt;
+    if (HASY) {
+      ma = fmax(amaxS, amaxN);
+      dty = (ma > 0.0) ? P.cfl * P.dy / ma : P.max_dt;
+    }
+    dtmin = fmin(dtmin, fmin(dtx, dty));
+  }
+}
+
+// wave minimum of the CFL time step, one atomic per wave (dt > 0: IEEE
+// order == integer order)
+__device__ __forceinline__ void dt_reduce(const DevPtrs& D, const DevParams& P, double dtmin) {
+  const int lane = threadIdx.x & 63;
+  for (int off = 32; off > 0; off >>= 1) dtmin = fmin(dtmin, shfl_d(dtmin, (lane + off) & 63));
+  if (lane == 0 && dtmin < P.max_dt)
+    atomicMin(D.dt_bits, (unsigned long long)__double_as_longlong(dtmin));
+}
+
+template <int NS, bool HASY>
+__device__ __forceinline__ void stencil_strip(const DevParams& P, const DevPtrs& D,
+                                              const SrcTime& T, int j, int k0, int k1,
+                                              bool skip_empty, int compute_dt);
+
+// Height of this step's stencil strips: strip_h = unit * 2^m <= max_strip
+// (a divisor of the tile height), the tallest that still leaves about
+// `target_strips` strips for the step's computed cells (short strips when
+// few cells are computed, so the sequential march stays short; tall strips
+// on a wet grid, so the two halo rows of each strip are amortised).  w_lane
+// is this lane's slot of the computed-cell counter.
+__device__ __forceinline__ int strip_height(const DevParams& P, unsigned long long w_lane,
+                                            int unit, int max_strip, int target_strips) {
   int strip_h = unit;
   if (P.ny > 1) {
-    unsigned long long w = *ctr(D, C_COMPUTE_CELLS, lane);
+    unsigned long long w = w_lane;
     for (int off = 32; off > 0; off >>= 1) w += __shfl_xor(w, off, 64);
     while (2 * strip_h <= max_strip &&
            w / (unsigned long long)(kStripW * 2 * strip_h) >= (unsigned long long)target_strips)
       strip_h *= 2;
   }
-  const int per = kTileH / strip_h;
-  const int ntiles = (int)*ctr(D, C_TILES, 0);
+  return strip_h;
+}
+
+// The stencil pass over the listed work tiles, cut into strips of strip_h
+// rows; workgroups take (tile, strip) items in turn until the list is
+// exhausted.  On a 2-D grid, one-row strips are left to k_stencil_rows; on
+// a 1-D grid every item is a listed row.
+template <int NS, bool HASY>
+__global__ void __launch_bounds__(64) k_stencil(DevParams P, DevPtrs D, int unit,
+                                                int max_strip, int target_strips,
+                                                int compute_dt) {
+  const int lane = threadIdx.x;
+  // the step's scalars and list sizes in one round of loads
+  double done = D.scal[S_DONE], t = D.scal[S_T];
+  unsigned long long w = *ctr(D, C_COMPUTE_CELLS, lane);
+  int ntiles = (int)*ctr(D, C_TILES, 0), nrows = (int)*ctr(D, C_ROWS, 0);
+  pin(done);
+  pin(t);
+  pin(w);
+  pin(ntiles);
+  pin(nrows);
+  if (done != 0.0) return;
+  const int strip_h = strip_height(P, w, unit, max_strip, target_strips);
+  if (HASY && strip_h == 1) return;
+  // the ring cells' time factors are the same for the whole step
+  const SrcTime T = P.radial ? src_time(P, t) : SrcTime{};
   const int tiles_x = (P.nx + kStripW - 1) / kStripW;
+  if (!HASY) {
+    for (int item = blockIdx.x; item < nrows; item += gridDim.x) {
+      const int row = D.row_list[item];
+      const int tile = row / kTileH;
+      const int k0 = (tile / tiles_x) * kTileH + row % kTileH;
+      stencil_strip<NS, HASY>(P, D, T, (tile % tiles_x) * kStripW - 2 + lane, k0, k0 + 1,
+                              false, compute_dt);
+    }
+    return;
+  }
+  const int per = kTileH / strip_h;
   for (int item = blockIdx.x; item < ntiles * per; item += gridDim.x) {
     const int tile = D.tile_list[item / per];
     const int tx = tile % tiles_x, tyy = tile / tiles_x;
     const int k0 = tyy * kTileH + (item % per) * strip_h;
     if (k0 >= P.ny) continue;
     const int k1 = min(k0 + strip_h, P.ny);
-    stencil_strip<NS, HASY>(P, D, tx * kStripW - 2 + lane, k0, k1, compute_dt);
+    stencil_strip<NS, HASY>(P, D, T, tx * kStripW - 2 + lane, k0, k1, true, compute_dt);
   }
 }
 
 // One strip: lane l owns column j = 60 * tile_x - 2 + l, rows k0 .. k1-1.
+// Every load of a row is issued one row ahead of its use (the raw states
+// three rows ahead), from clamped addresses and without a branch around it,
+// so that a strip pays the memory latency about once.
 template <int NS, bool HASY>
-__device__ __forceinline__ void stencil_strip(const DevParams& P, const DevPtrs& D, int j,
-                                              int k0, int k1, int compute_dt) {
+__device__ __forceinline__ void stencil_strip(const DevParams& P, const DevPtrs& D,
+                                              const SrcTime& T, int j, int k0, int k1,
+                                              bool skip_empty, int compute_dt) {
   constexpr int NV = 4 + NS;
   const int lane = threadIdx.x;
   const bool injx = (j >= 0 && j < P.nx);
   const bool own = (lane >= 2 && lane < 2 + kStripW && injx);
-  const double t = D.scal[S_T];
 
   // skip strips without a cell to compute
-  {
+  if (skip_empty) {
     int any = 0;
     if (own)
       for (int k = k0; k < k1; k++) any |= D.mask[cidx(P, j, k)] & M_COMPUTE;
@@ -595,28 +877,49 @@ __device__ __forceinline__ void stencil_strip(const DevParams& P, const DevPtrs&
   double hiy[NV + 2];                          // N value of row k
   double HS[NV];                               // y flux at the S face of row k
   double amaxS = 0.0;
-  SrcInfo S = injx ? load_src(P, D, cidx(P, j, k0)) : SrcInfo{};
+  double rnext[NV];  // raw state of row k+2, loaded one iteration ahead
+  // source data: row k (S, v0, v1) and row k+1 (raw byte bn, vectors), the
+  // slope correction of the S face of row k+1, and row k's per-cell data
+  int bc = 0, bn = 0;
+  double vc0 = 0.0, vc1 = 0.0, vn0 = 0.0, vn1 = 0.0, gn = 1.0;
+  RowData R;
+  SrcInfo S{};
   if (HASY) {
     double qmm[NV + 2], sb[NV + 2], lo[NV + 2];
-    load_row<NS>(P, D, j, k0 - 2, qmm);
-    load_row<NS>(P, D, j, k0 - 1, qm);
-    load_row<NS>(P, D, j, k0, qc);
-    load_row<NS>(P, D, j, k0 + 1, qn);
+    int bm;
+    double vm0, vm1, g0 = 1.0;
+    {
+      double r0[NV], r1[NV], r2[NV], r3[NV];
+      load_raw<NS>(P, D, j, k0 - 2, r0);
+      load_raw<NS>(P, D, j, k0 - 1, r1);
+      load_raw<NS>(P, D, j, k0, r2);
+      load_raw<NS>(P, D, j, k0 + 1, r3);
+      load_raw<NS>(P, D, j, k0 + 2, rnext);
+      load_src(P, D, j, k0 - 1, bm, vm0, vm1);
+      load_src(P, D, j, k0, bc, vc0, vc1);
+      load_src(P, D, j, k0 + 1, bn, vn0, vn1);
+      g0 = load_gsy(P, D, j, k0);
+      gn = load_gsy(P, D, j, k0 + 1);
+      load_rowdata(P, D, j, k0, R);
+      finish_row<NS>(P, j, k0 - 2, r0, qmm);
+      finish_row<NS>(P, j, k0 - 1, r1, qm);
+      finish_row<NS>(P, j, k0, r2, qc);
+      finish_row<NS>(P, j, k0 + 1, r3, qn);
+    }
+    if (injx) S = src_info(bc);
     Side<NS> Bs, Ts;
     if (injx) {
-      const size_t c = cidx(P, j, k0);
-      ring_state<NS>(P, D, c, S, t, sb);
+      ring_state<NS>(P, S, T, vc0, vc1, sb);
       recon<NS, 2>(P, qc, qm, qn, k0, P.ny, j, S, sb, false, lo, hiy);
-      const double g = D.gsy ? D.gsy[c] : 1.0;
+      const double g = g0;
       if (k0 == 0) {
         // boundary face: both sides carry the S value of row 0
         if (S.is_src) zero_side<NS>(Ts); else make_side<NS, 2>(P, lo, g, Ts);
         Bs = Ts;
       } else {
-        const size_t cm = cidx(P, j, k0 - 1);
-        SrcInfo Sm = load_src(P, D, cm);
+        const SrcInfo Sm = src_info(bm);
         double sbm[NV + 2], lom[NV + 2], him[NV + 2];
-        ring_state<NS>(P, D, cm, Sm, t, sbm);
+        ring_state<NS>(P, Sm, T, vm0, vm1, sbm);
         recon<NS, 2>(P, qm, qmm, qc, k0 - 1, P.ny, j, Sm, sbm, false, lom, him);
         yface_sides<NS>(P, him, lo, Sm, S, k0, g, Bs, Ts);
       }
@@ -629,12 +932,19 @@ __device__ __forceinline__ void stencil_strip(const DevParams& P, const DevPtrs&
     amaxS = face_flux<NS, 2>(Bs, Ts, HS);
   } else {
     load_row<NS>(P, D, j, k0, qc);
+    load_src(P, D, j, k0, bc, vc0, vc1);
+    load_rowdata(P, D, j, k0, R);
+    if (injx) S = src_info(bc);
   }
 
   for (int k = k0; k < k1; k++) {
     const size_t c = injx ? cidx(P, j, k) : 0;
+    const bool more = k + 1 < k1;
+    // next row's per-cell data
+    RowData Rn;
+    if (more) load_rowdata(P, D, j, k + 1, Rn);
     double sb[NV + 2];
-    if (injx) ring_state<NS>(P, D, c, S, t, sb);
+    if (injx) ring_state<NS>(P, S, T, vc0, vc1, sb);
     else {
 #pragma unroll
       for (int i = 0; i < NV + 2; i++) sb[i] = 0.0;
@@ -643,24 +953,30 @@ __device__ __forceinline__ void stencil_strip(const DevParams& P, const DevPtrs&
     double HN[NV];
     double amaxN = 0.0;
     SrcInfo Sn{};
+    int bnn = 0;
+    double vnn0 = 0.0, vnn1 = 0.0, gnn = 1.0;
     if (HASY) {
       double qnn[NV + 2], hin[NV + 2];
-      load_row<NS>(P, D, j, k + 2, qnn);
+      finish_row<NS>(P, j, k + 2, rnext, qnn);
+      if (more) {
+        load_raw<NS>(P, D, j, k + 3, rnext);  // next iteration's rows
+        load_src(P, D, j, k + 2, bnn, vnn0, vnn1);
+        gnn = load_gsy(P, D, j, k + 2);
+      }
       Side<NS> Bs, Ts;
       if (injx) {
         if (k == P.ny - 1) {
-          const double g = D.gsy ? D.gsy[cidx(P, j, P.ny)] : 1.0;
+          const double g = gn;  // S face of row ny: the N boundary
           if (S.is_src) zero_side<NS>(Bs); else make_side<NS, 2>(P, hiy, g, Bs);
           Ts = Bs;
 #pragma unroll
           for (int i = 0; i < NV + 2; i++) hin[i] = 0.0;
         } else {
-          const size_t cn = cidx(P, j, k + 1);
-          Sn = load_src(P, D, cn);
+          Sn = src_info(bn);
           double sbn[NV + 2], lon[NV + 2];
-          ring_state<NS>(P, D, cn, Sn, t, sbn);
+          ring_state<NS>(P, Sn, T, vn0, vn1, sbn);
           recon<NS, 2>(P, qn, qc, qnn, k + 1, P.ny, j, Sn, sbn, false, lon, hin);
-          const double g = D.gsy ? D.gsy[cn] : 1.0;
+          const double g = gn;
           yface_sides<NS>(P, hiy, lon, S, Sn, k + 1, g, Bs, Ts);
         }
       } else {
@@ -680,119 +996,199 @@ __device__ __forceinline__ void stencil_strip(const DevParams& P, const DevPtrs&
     }
     // physical state of row k: qm after the shift (2-D) or qc (1-D)
     const double* qk = HASY ? qm : qc;
-    // ---------------- x: W face on every lane, E face by shuffle ----------
-    double qkm[NV + 2], qkp[NV + 2];
-#pragma unroll
-    for (int i = 0; i < NV + 2; i++) {
-      qkm[i] = shfl_d(qk[i], (lane + 63) & 63);
-      qkp[i] = shfl_d(qk[i], (lane + 1) & 63);
-    }
-    // after a north ring cell the x-slope of v uses y_stag(k+1) as its
-    // third abscissa (src/solver_2d.f90:3616)
-    const bool typo = HASY && injx && P.radial && S.src2 && k > 0 && k < P.ny - 1 &&
-                      !S.sS && S.sN;
-    Side<NS> Wself, Eself;
-    if (injx) {
-      double lo[NV + 2], hi[NV + 2];
-      recon<NS, 1>(P, qk, qkm, qkp, j, P.nx, k, S, sb, typo, lo, hi);
-      const double gW = D.gsx ? D.gsx[(size_t)k * (P.pitch + 1) + j] : 1.0;
-      const double gE = D.gsx ? D.gsx[(size_t)k * (P.pitch + 1) + j + 1] : 1.0;
-      if (S.is_src) {
-        zero_side<NS>(Wself);
-        zero_side<NS>(Eself);
-      } else {
-        make_side<NS, 1>(P, lo, gW, Wself);
-        make_side<NS, 1>(P, hi, gE, Eself);
-      }
-    } else {
-      zero_side<NS>(Wself);
-      zero_side<NS>(Eself);
-    }
-    Side<NS> Eleft;
-    shfl_side<NS>(Eself, Eleft, (lane + 63) & 63);
-    const int flags = (S.is_src ? 1 : 0) | (S.src2 ? 2 : 0) | (S.sE ? 4 : 0) |
-                      (S.sW ? 8 : 0);
-    const int flags_left = __shfl(flags, (lane + 63) & 63, kWave);
-    double HW[NV];
-    double amaxW;
-    {
-      Side<NS> Ls, Rs;
-      if (j == 0) {
-        Ls = Wself;
-        Rs = Wself;
-      } else {
-        const bool interior_me = (j > 0 && j < P.nx - 1);
-        const bool interior_left = (j - 1 > 0 && j - 1 < P.nx - 1);
-        if (flags_left & 1) {
-          if (P.radial && S.src2 && !S.sE && S.sW && interior_me) Ls = Wself;
-          else zero_side<NS>(Ls);
-        } else {
-          Ls = Eleft;
-        }
-        if (S.is_src) {
-          if (P.radial && (flags_left & 2) && (flags_left & 4) && interior_left) Rs = Eleft;
-          else zero_side<NS>(Rs);
-        } else {
-          Rs = Wself;
-        }
-      }
-      amaxW = face_flux<NS, 1>(Ls, Rs, HW);
-    }
-    double HE[NV];
-    double amaxE;
-    {
-      double HEs[NV];
-#pragma unroll
-      for (int i = 0; i < NV; i++) HEs[i] = shfl_d(HW[i], (lane + 1) & 63);
-      double amaxEs = shfl_d(amaxW, (lane + 1) & 63);
-      if (j == P.nx - 1) {
-        amaxE = face_flux<NS, 1>(Eself, Eself, HE);
-      } else {
-#pragma unroll
-        for (int i = 0; i < NV; i++) HE[i] = HEs[i];
-        amaxE = amaxEs;
-      }
-    }
-    // ---------------- divergence, explicit terms, D ----------------
-    if (own && (D.mask[c] & M_COMPUTE)) {
-      double ex[NV];
-      const double bx = D.Bx[c], by = D.By[c];
-      const double bxx = D.Bxx ? D.Bxx[c] : 0.0, bxy = D.Bxy ? D.Bxy[c] : 0.0,
-                   byy = D.Byy ? D.Byy[c] : 0.0;
-      const double gc = D.gcoef ? D.gcoef[c] : 1.0;
-      const double dg = D.dgdx ? D.dgdx[c] : 0.0;
-      expl_terms<NS>(P, bx, by, bxx, bxy, byy, gc, dg, qk, ex);
-#pragma unroll
-      for (int i = 0; i < NV; i++) {
-        double dv = 0.0;
-        dv = dv + (HE[i] - HW[i]) * P.one_by_dx;
-        if (HASY) dv = dv + (HN[i] - HS[i]) * P.one_by_dy;
-        D.D[i * pl + c] = (0.0 + (dv - ex[i]) * 1.0) + (0.0 - 0.0) * 0.0;
-      }
-      if (compute_dt) {
-        double ma = fmax(amaxW, amaxE);
-        const double dtx = (ma > 0.0) ? P.cfl * P.dx / ma : P.max_dt;
-        double dty = P.max_dt;
-        if (HASY) {
-          ma = fmax(amaxS, amaxN);
-          dty = (ma > 0.0) ? P.cfl * P.dy / ma : P.max_dt;
-        }
-        dtmin = fmin(dtmin, fmin(dtx, dty));
-      }
-    }
+    double HW[NV], HE[NV], amaxW, amaxE;
+    x_fluxes<NS, HASY>(P, S, sb, qk, j, k, R, HW, HE, amaxW, amaxE);
+    if (own && (R.m & M_COMPUTE))
+      cell_update<NS, HASY>(P, D, c, qk, R, HW, HE, HS, HN, amaxW, amaxE, amaxS, amaxN,
+                            compute_dt, dtmin);
+    if (more) R = Rn;
     if (HASY) {
 #pragma unroll
       for (int i = 0; i < NV; i++) HS[i] = HN[i];
       amaxS = amaxN;
       S = Sn;
+      vc0 = vn0;
+      vc1 = vn1;
+      bn = bnn;
+      vn0 = vnn0;
+      vn1 = vnn1;
+      gn = gnn;
     }
   }
-  if (compute_dt) {
-    // wave minimum, one atomic per wave (dt > 0: IEEE order == integer order)
-    for (int off = 32; off > 0; off >>= 1) dtmin = fmin(dtmin, shfl_d(dtmin, (lane + off) & 63));
-    if (lane == 0 && dtmin < P.max_dt)
-      atomicMin(D.dt_bits, (unsigned long long)__double_as_longlong(dtmin));
+  if (compute_dt) dt_reduce(D, P, dtmin);
+}
+
+// ---------------------------------------------------------------------------
+// k_stencil_rows: the stencil pass when its strips are one row high (few
+// computed cells: the pass is bound by the latency of one row, not by
+// bandwidth).  Three wavefronts share a listed row: wavefront 0 computes
+// the fluxes through the row's S faces, wavefront 1 through its N faces,
+// wavefront 2 the x faces and then, with the y fluxes from LDS, the cell
+// updates.  Each face is computed from the same operands, in the same
+// order, as in the strip march.
+// ---------------------------------------------------------------------------
+// flux through the S face of row k (rows k-2 .. k+1 read)
+template <int NS>
+__device__ __forceinline__ double yflux_S(const DevParams& P, const DevPtrs& D,
+                                          const SrcTime& T, int j, int k, double* H) {
+  constexpr int NV = 4 + NS;
+  const bool injx = (j >= 0 && j < P.nx);
+  double qmm[NV + 2], qm[NV + 2], qc[NV + 2], qn[NV + 2];
+  int bm, bc;
+  double vm0, vm1, vc0, vc1, g;
+  {
+    double r0[NV], r1[NV], r2[NV], r3[NV];
+    load_raw<NS>(P, D, j, k - 2, r0);
+    load_raw<NS>(P, D, j, k - 1, r1);
+    load_raw<NS>(P, D, j, k, r2);
+    load_raw<NS>(P, D, j, k + 1, r3);
+    load_src(P, D, j, k - 1, bm, vm0, vm1);
+    load_src(P, D, j, k, bc, vc0, vc1);
+    g = load_gsy(P, D, j, k);
+    finish_row<NS>(P, j, k - 2, r0, qmm);
+    finish_row<NS>(P, j, k - 1, r1, qm);
+    finish_row<NS>(P, j, k, r2, qc);
+    finish_row<NS>(P, j, k + 1, r3, qn);
   }
+  Side<NS> Bs, Ts;
+  if (injx) {
+    const SrcInfo S = src_info(bc);
+    double sb[NV + 2], lo[NV + 2], hi[NV + 2];
+    ring_state<NS>(P, S, T, vc0, vc1, sb);
+    recon<NS, 2>(P, qc, qm, qn, k, P.ny, j, S, sb, false, lo, hi);
+    if (k == 0) {
+      // boundary face: both sides carry the S value of row 0
+      if (S.is_src) zero_side<NS>(Ts); else make_side<NS, 2>(P, lo, g, Ts);
+      Bs = Ts;
+    } else {
+      const SrcInfo Sm = src_info(bm);
+      double sbm[NV + 2], lom[NV + 2], him[NV + 2];
+      ring_state<NS>(P, Sm, T, vm0, vm1, sbm);
+      recon<NS, 2>(P, qm, qmm, qc, k - 1, P.ny, j, Sm, sbm, false, lom, him);
+      yface_sides<NS>(P, him, lo, Sm, S, k, g, Bs, Ts);
+    }
+  } else {
+    zero_side<NS>(Bs);
+    zero_side<NS>(Ts);
+  }
+  return face_flux<NS, 2>(Bs, Ts, H);
+}
+
+// flux through the N face of row k (rows k-1 .. k+2 read)
+template <int NS>
+__device__ __forceinline__ double yflux_N(const DevParams& P, const DevPtrs& D,
+                                          const SrcTime& T, int j, int k, double* H) {
+  constexpr int NV = 4 + NS;
+  const bool injx = (j >= 0 && j < P.nx);
+  double qm[NV + 2], qc[NV + 2], qn[NV + 2], qnn[NV + 2];
+  int bc, bn;
+  double vc0, vc1, vn0, vn1, g;
+  {
+    double r1[NV], r2[NV], r3[NV], r4[NV];
+    load_raw<NS>(P, D, j, k - 1, r1);
+    load_raw<NS>(P, D, j, k, r2);
+    load_raw<NS>(P, D, j, k + 1, r3);
+    load_raw<NS>(P, D, j, k + 2, r4);
+    load_src(P, D, j, k, bc, vc0, vc1);
+    load_src(P, D, j, k + 1, bn, vn0, vn1);
+    g = load_gsy(P, D, j, k + 1);  // k + 1 == ny: the N boundary
+    finish_row<NS>(P, j, k - 1, r1, qm);
+    finish_row<NS>(P, j, k, r2, qc);
+    finish_row<NS>(P, j, k + 1, r3, qn);
+    finish_row<NS>(P, j, k + 2, r4, qnn);
+  }
+  Side<NS> Bs, Ts;
+  if (injx) {
+    const SrcInfo S = src_info(bc);
+    double sb[NV + 2], lo[NV + 2], hiy[NV + 2];
+    ring_state<NS>(P, S, T, vc0, vc1, sb);
+    recon<NS, 2>(P, qc, qm, qn, k, P.ny, j, S, sb, false, lo, hiy);
+    if (k == P.ny - 1) {
+      if (S.is_src) zero_side<NS>(Bs); else make_side<NS, 2>(P, hiy, g, Bs);
+      Ts = Bs;
+    } else {
+      const SrcInfo Sn = src_info(bn);
+      double sbn[NV + 2], lon[NV + 2], hin[NV + 2];
+      ring_state<NS>(P, Sn, T, vn0, vn1, sbn);
+      recon<NS, 2>(P, qn, qc, qnn, k + 1, P.ny, j, Sn, sbn, false, lon, hin);
+      yface_sides<NS>(P, hiy, lon, S, Sn, k + 1, g, Bs, Ts);
+    }
+  } else {
+    zero_side<NS>(Bs);
+    zero_side<NS>(Ts);
+  }
+  return face_flux<NS, 2>(Bs, Ts, H);
+}
+
+template <int NS>
+__global__ void __launch_bounds__(192) k_stencil_rows(DevParams P, DevPtrs D, int unit,
+                                                      int max_strip, int target_strips,
+                                                      int compute_dt) {
+  constexpr int NV = 4 + NS;
+  __shared__ double yS[NV + 1][64], yN[NV + 1][64];  // fluxes, then the wave speed
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  // the step's scalars, list size and first row in one round of loads (the
+  // grid is never larger than the row list, so row_list[blockIdx.x] exists)
+  double done = D.scal[S_DONE], t = D.scal[S_T];
+  unsigned long long w = *ctr(D, C_COMPUTE_CELLS, lane);
+  int nrows = (int)*ctr(D, C_ROWS, 0), row = D.row_list[blockIdx.x];
+  pin(done);
+  pin(t);
+  pin(w);
+  pin(nrows);
+  pin(row);
+  if (done != 0.0) return;
+  if (strip_height(P, w, unit, max_strip, target_strips) != 1) return;
+  const SrcTime T = P.radial ? src_time(P, t) : SrcTime{};
+  const int tiles_x = (P.nx + kStripW - 1) / kStripW;
+  double dtmin = P.max_dt;
+  for (int item = blockIdx.x; item < nrows; item += gridDim.x) {
+    if (item != blockIdx.x) row = D.row_list[item];
+    const int tile = row / kTileH;
+    const int k = (tile / tiles_x) * kTileH + row % kTileH;
+    const int j = (tile % tiles_x) * kStripW - 2 + lane;
+    const bool own = (lane >= 2 && lane < 2 + kStripW && j < P.nx);
+    double qk[NV + 2], HW[NV], HE[NV], amaxW = 0.0, amaxE = 0.0;
+    RowData R;
+    if (wv < 2) {
+      double H[NV];
+      const double a = (wv == 0) ? yflux_S<NS>(P, D, T, j, k, H) : yflux_N<NS>(P, D, T, j, k, H);
+      double(*y)[64] = (wv == 0) ? yS : yN;
+#pragma unroll
+      for (int i = 0; i < NV; i++) y[i][lane] = H[i];
+      y[NV][lane] = a;
+    } else {
+      const bool injx = (j >= 0 && j < P.nx);
+      double r[NV], vc0, vc1, sb[NV + 2];
+      int bc;
+      load_raw<NS>(P, D, j, k, r);
+      load_src(P, D, j, k, bc, vc0, vc1);
+      load_rowdata(P, D, j, k, R);
+      finish_row<NS>(P, j, k, r, qk);
+      SrcInfo S{};
+      if (injx) {
+        S = src_info(bc);
+        ring_state<NS>(P, S, T, vc0, vc1, sb);
+      } else {
+#pragma unroll
+        for (int i = 0; i < NV + 2; i++) sb[i] = 0.0;
+      }
+      x_fluxes<NS, true>(P, S, sb, qk, j, k, R, HW, HE, amaxW, amaxE);
+    }
+    __syncthreads();  // y fluxes in LDS
+    if (wv == 2 && own && (R.m & M_COMPUTE)) {
+      double HS[NV], HN[NV];
+#pragma unroll
+      for (int i = 0; i < NV; i++) {
+        HS[i] = yS[i][lane];
+        HN[i] = yN[i][lane];
+      }
+      cell_update<NS, true>(P, D, cidx(P, j, k), qk, R, HW, HE, HS, HN, amaxW, amaxE,
+                            yS[NV][lane], yN[NV][lane], compute_dt, dtmin);
+    }
+    __syncthreads();  // LDS free for the next row
+  }
+  if (wv == 2 && compute_dt) dt_reduce(D, P, dtmin);
 }
 
 // ---------------------------------------------------------------------------
@@ -868,28 +1264,89 @@ __device__ __forceinline__ void local_cell(const DevParams& P, const DevPtrs& D,
                                            double& s_mass, double& s_vol, double& s_area,
                                            double& s_sol) {
   constexpr int NV = 4 + NS;
+  constexpr int NSX = NS > 0 ? NS : 1;
   const size_t pl = plane(P);
   const double dt = D.scal[S_DT];
+  const bool rk = (PH == 0 || PH == 2) && (ops & OP_RK);
+  const bool ero = PH == 1 && (ops & OP_EROSION) &&
+                   !((P.erosion_coeff == 0.0) && !P.settling && !P.entrain);
+  const bool post = PH == 1 && (ops & OP_POST);
+  // every load of the cell first, the mask byte among them, so that the
+  // cell waits for memory once
   double q[NV], qp[NV + 2];
+  double Dr[NV], bx = 0.0, by = 0.0, bxx = 0.0, bxy = 0.0, byy = 0.0, gc = 1.0, wg[2];
+  double erod[NSX], depo[NSX], eros[NSX], bcell = 0.0, hm = 0.0, pm = 0.0, mvel = 0.0;
+  uint32_t vbits = 0;
+  const int mk = D.mask[c];
 #pragma unroll
   for (int i = 0; i < NV; i++) q[i] = D.q[i * pl + c];
-  if ((PH == 0 || PH == 2) && (ops & OP_RK)) {
-    double q0[NV], Dr[NV];
+  if (rk) {
 #pragma unroll
-    for (int i = 0; i < NV; i++) {
-      q0[i] = q[i];
-      Dr[i] = D.D[i * pl + c];
-    }
-    const double bx = D.Bx[c], by = D.By[c];
-    const double bxx = D.Bxx ? D.Bxx[c] : 0.0, bxy = D.Bxy ? D.Bxy[c] : 0.0,
-                 byy = D.Byy ? D.Byy[c] : 0.0;
-    const double gc = D.gcoef ? D.gcoef[c] : 1.0;
-    int nonconv = 0, ls = 0, wu = -1;
-    double wg[2];
+    for (int i = 0; i < NV; i++) Dr[i] = D.D[i * pl + c];
+    bxx = D.Bxx ? D.Bxx[c] : 0.0;
+    bxy = D.Bxy ? D.Bxy[c] : 0.0;
+    byy = D.Byy ? D.Byy[c] : 0.0;
+    gc = D.gcoef ? D.gcoef[c] : 1.0;
     if (PH == 2 && (ops & OP_WARM)) {
       wg[0] = D.guess[c];
       wg[1] = D.guess[pl + c];
     }
+  }
+  if (rk || ero) {
+    bx = D.Bx[c];
+    by = D.By[c];
+  }
+  if (ero) {
+#pragma unroll
+    for (int i = 0; i < NS; i++) {
+      erod[i] = D.erodible[i * pl + c];
+      depo[i] = D.deposit[i * pl + c];
+      eros[i] = D.erosion[i * pl + c];
+    }
+    if (P.topo) bcell = D.B[c];
+  }
+  if (post) {
+    hm = D.hmax[c];
+    pm = D.pdynmax[c];
+    mvel = D.modvel[c];
+    if (D.vuln) vbits = D.vuln[c];
+  }
+  // the loaded values are needed in registers here: no load is moved into
+  // the branches below
+#pragma unroll
+  for (int i = 0; i < NV; i++) pin(q[i]);
+  if (rk) {
+#pragma unroll
+    for (int i = 0; i < NV; i++) pin(Dr[i]);
+    pin(bxx);
+    pin(bxy);
+    pin(byy);
+    pin(gc);
+  }
+  if (rk || ero) {
+    pin(bx);
+    pin(by);
+  }
+  if (ero) {
+#pragma unroll
+    for (int i = 0; i < NS; i++) {
+      pin(erod[i]);
+      pin(depo[i]);
+      pin(eros[i]);
+    }
+    pin(bcell);
+  }
+  if (post) {
+    pin(hm);
+    pin(pm);
+    pin(mvel);
+  }
+  if (!(mk & M_COMPUTE)) return;
+  if (rk) {
+    double q0[NV];
+#pragma unroll
+    for (int i = 0; i < NV; i++) q0[i] = q[i];
+    int nonconv = 0, ls = 0, wu = -1;
     float hv[kHarvestWidth];
     bool hok = false;
     int it = stage2<NS>(P, q0, Dr, dt, bx, by, bxx, bxy, byy, gc, q, nonconv, ls,
@@ -967,27 +1424,23 @@ __device__ __forceinline__ void local_cell(const DevParams& P, const DevPtrs& D,
       for (int i = 0; i < NV; i++) q[i] = 0.0;
     }
   }
-  if (PH == 1 && (ops & OP_EROSION) &&
-      !((P.erosion_coeff == 0.0) && !P.settling && !P.entrain)) {
+  if (ero) {
     phys_or_dry<NS>(P, q, qp);
-    double er[NS > 0 ? NS : 1], dep[NS > 0 ? NS : 1], eq[NV], topo;
-    double erod[NS > 0 ? NS : 1];
-#pragma unroll
-    for (int i = 0; i < NS; i++) erod[i] = D.erodible[i * pl + c];
-    mass_exchange<NS>(P, qp, D.Bx[c], D.By[c], erod, dt, er, dep, eq, topo);
+    double er[NSX], dep[NSX], eq[NV], topo;
+    mass_exchange<NS>(P, qp, bx, by, erod, dt, er, dep, eq, topo);
 #pragma unroll
     for (int i = 0; i < NV; i++) q[i] = q[i] + dt * eq[i];
 #pragma unroll
     for (int i = 0; i < NS; i++) q[4 + i] = fmax(0.0, q[4 + i]);
 #pragma unroll
     for (int i = 0; i < NS; i++) {
-      D.deposit[i * pl + c] = D.deposit[i * pl + c] + dt * dep[i];
-      D.erosion[i * pl + c] = D.erosion[i * pl + c] + dt * er[i];
+      D.deposit[i * pl + c] = depo[i] + dt * dep[i];
+      D.erosion[i * pl + c] = eros[i] + dt * er[i];
       double ev = erod[i] - dt * er[i];
       if (P.erod_dep) ev = ev + dt * dep[i];
       D.erodible[i * pl + c] = ev;
     }
-    if (P.topo) D.B[c] = D.B[c] + dt * topo;
+    if (P.topo) D.B[c] = bcell + dt * topo;
     if (q[0] <= 0.0 && q[0] > -1.0e-10) {
 #pragma unroll
       for (int i = 0; i < NV; i++) q[i] = 0.0;
@@ -1010,20 +1463,25 @@ __device__ __forceinline__ void local_cell(const DevParams& P, const DevPtrs& D,
     }
   }
   if (ops & (OP_RK | OP_EROSION)) {
+    bool zero = true;
 #pragma unroll
-    for (int i = 0; i < NV; i++) D.q[i * pl + c] = q[i];
+    for (int i = 0; i < NV; i++) {
+      D.q[i * pl + c] = q[i];
+      zero = zero && (q[i] == 0.0);
+    }
+    D.sflag[c] = state_flags(q[0], zero);
   }
-  if (PH == 1 && (ops & OP_POST)) {
+  if (post) {
     double pd = phys_or_dry<NS>(P, q, qp);
     if (q[0] > 0.0) {
-      D.hmax[c] = fmax(D.hmax[c], qp[0]);
+      D.hmax[c] = fmax(hm, qp[0]);
       const double mv = sqrt(qp[NV] * qp[NV] + qp[NV + 1] * qp[NV + 1]);
       if (qp[0] > 0.001) {
-        D.pdynmax[c] = fmax(D.pdynmax[c], pd);
-        D.modvel[c] = fmax(D.modvel[c], mv);
+        D.pdynmax[c] = fmax(pm, pd);
+        D.modvel[c] = fmax(mvel, mv);
       }
       if (D.vuln) {
-        uint32_t bits = D.vuln[c];
+        uint32_t bits = vbits;
         int it = 0;
         for (int a = 0; a < P.n_thk; a++) {
           const bool th = qp[0] >= P.thickness_levels[a];
@@ -1058,11 +1516,7 @@ __device__ __forceinline__ void local_row(const DevParams& P, const DevPtrs& D, 
   const int k = (tile / tiles_x) * kTileH + row % kTileH;
   const bool own = lane >= 2 && lane < 2 + kStripW && j < P.nx && k < P.ny;
   double s_mass = 0.0, s_vol = 0.0, s_area = 0.0, s_sol = 0.0;
-  if (own) {
-    const size_t c = cidx(P, j, k);
-    if (D.mask[c] & M_COMPUTE)
-      local_cell<NS, PH>(P, D, c, ops, e, imax, s_mass, s_vol, s_area, s_sol);
-  }
+  if (own) local_cell<NS, PH>(P, D, cidx(P, j, k), ops, e, imax, s_mass, s_vol, s_area, s_sol);
   if (PH == 1 && (ops & OP_SUMS)) {
     double v4[4] = {s_mass, s_vol, s_area, s_sol};
 #pragma unroll
@@ -1293,6 +1747,17 @@ __global__ void k_aos_to_planes(DevParams P, const double* __restrict__ src, int
   if (j >= P.nx || k >= P.ny) return;
   const size_t pl = plane(P), s = ((size_t)k * P.nx + j) * nvar;
   for (int i = 0; i < nvar; i++) dst[i * pl + cidx(P, j, k)] = src[s + i];
+}
+
+// state flags of the current state (after an upload)
+__global__ void k_sflag(DevParams P, DevPtrs D) {
+  const int j = blockIdx.x * kTileX + (threadIdx.x & 63);
+  const int k = blockIdx.y * kTileY + (threadIdx.x >> 6);
+  if (j >= P.nx || k >= P.ny) return;
+  const size_t c = cidx(P, j, k), pl = plane(P);
+  bool zero = true;
+  for (int i = 0; i < P.nv; i++) zero = zero && (D.q[i * pl + c] == 0.0);
+  D.sflag[c] = state_flags(D.q[c], zero);
 }
 
 __global__ void k_planes_to_aos(DevParams P, const double* __restrict__ src, int nvar,

@@ -464,13 +464,17 @@ __device__ __forceinline__ void mass_exchange(const DevParams& P, const double* 
   double r_T = qp[3];
   Mixt m = mixt<NS, true>(P, qp);
   double inv_kin_visc = 1.0 / P.kin_visc_c;
+  if (P.settling) {
+    // hindered settling factor, the same for every class
+    const double hind = det_pow(1.0 - fmin(1.0, at / alpha_max), hind_exp);
 #pragma unroll
-  for (int i = 0; i < NS; i++) {
-    if ((al[i] > 0.0) && P.settling) {
-      double sv = settling_velocity(P, P.diam_s[i], P.rho_s[i], m.rho_c, inv_kin_visc);
-      double d = al[i] * sv * 1.0;
-      d = d * det_pow(1.0 - fmin(1.0, at / alpha_max), hind_exp);
-      dep[i] = fmin(d, r_h * al[i] / dt);
+    for (int i = 0; i < NS; i++) {
+      if (al[i] > 0.0) {
+        const double sv = settling_velocity(P, P.diam_s[i], P.rho_s[i], m.rho_c, inv_kin_visc);
+        double d = al[i] * sv * 1.0;
+        d = d * hind;
+        dep[i] = fmin(d, r_h * al[i] / dt);
+      }
     }
   }
   double sdep = 0.0;

@@ -569,17 +569,20 @@ __device__ __forceinline__ double settling_velocity(const DevParams& P, double d
   return sv;
 }
 
-// eval_source_bdry (src/constitutive_2d.f90:5024-5130)
-template <int NS>
-__device__ __forceinline__ void source_bdry(const DevParams& P, double time,
-                                            double vx, double vy, double* sb) {
-  constexpr int NV = 4 + NS;
+// eval_source_bdry (src/constitutive_2d.f90:5024-5130), split into the
+// time factors (the same for every ring cell of a step) and the per-cell
+// part that scales them by the cell's direction vector
+struct SrcTime {
+  bool off;           // time >= time_param(4): the source is switched off
+  double tc, p15, p05;  // time coefficient, tc**1.5, tc**0.5
+};
+
+__device__ __forceinline__ SrcTime src_time(const DevParams& P, double time) {
   const double* tp = P.time_param;
+  SrcTime r{false, 0.0, 0.0, 0.0};
   if (time >= tp[3]) {
-#pragma unroll
-    for (int i = 0; i < NV + 2; i++) sb[i] = 0.0;
-    sb[3] = P.T_source;
-    return;
+    r.off = true;
+    return r;
   }
   double t_rem = fmod(time, tp[0]);
   double tc = 0.0;
@@ -593,15 +596,32 @@ __device__ __forceinline__ void source_bdry(const DevParams& P, double time,
     else if (t_rem <= tp[1])
       tc = 0.5 * (1.0 + det_cos(P.pi_g * ((t_rem - tp[1]) / tp[2] + 1.0)));
   }
+  r.tc = tc;
+  r.p15 = det_pow(tc, 1.5);
+  r.p05 = det_pow(tc, 0.5);
+  return r;
+}
+
+template <int NS>
+__device__ __forceinline__ void source_bdry(const DevParams& P, const SrcTime& T, double vx,
+                                            double vy, double* sb) {
+  constexpr int NV = 4 + NS;
+  if (T.off) {
+#pragma unroll
+    for (int i = 0; i < NV + 2; i++) sb[i] = 0.0;
+    sb[3] = P.T_source;
+    return;
+  }
+  const double tc = T.tc;
   sb[0] = tc * P.h_source;
-  sb[1] = det_pow(tc, 1.5) * P.h_source * P.vel_source * vx;
-  sb[2] = det_pow(tc, 1.5) * P.h_source * P.vel_source * vy;
+  sb[1] = T.p15 * P.h_source * P.vel_source * vx;
+  sb[2] = T.p15 * P.h_source * P.vel_source * vy;
   sb[3] = P.T_source;
 #pragma unroll
   for (int i = 0; i < NS; i++)
     sb[4 + i] = P.alpha ? P.alphas_source[i] : tc * P.h_source * P.alphas_source[i];
-  sb[NV] = det_pow(tc, 0.5) * P.vel_source * vx;
-  sb[NV + 1] = det_pow(tc, 0.5) * P.vel_source * vy;
+  sb[NV] = T.p05 * P.vel_source * vx;
+  sb[NV + 1] = T.p05 * P.vel_source * vy;
 }
 
 }  // namespace imexsw
