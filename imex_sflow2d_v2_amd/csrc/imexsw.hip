@@ -55,9 +55,17 @@ struct imexsw_ctx {
   // learned warm start (guess planes in D.guess) and harvest buffer
   bool warm = false;
   MlpDev mlp{};
-  // captured kGraphSteps-step graph of imexsw_advance and what it captured
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t exec = nullptr;
+  // sparse steps (few computed cells): the row stencil and the fused local
+  // kernel; chosen on the host from the computed-cell count of an earlier
+  // step (both ways give the same bits)
+  bool sparse = false;
+  int64_t sparse_steps = 0;
+  unsigned long long* h_cnt = nullptr;  // pinned: 2 x kSlots computed-cell counters
+  hipEvent_t cnt_ev[2] = {nullptr, nullptr};
+  // captured kGraphSteps-step graphs of imexsw_advance (dense, sparse) and
+  // what they captured
+  hipGraph_t graph[2] = {nullptr, nullptr};
+  hipGraphExec_t exec[2] = {nullptr, nullptr};
   DevParams graph_P{};
   DevPtrs graph_D{};
   long graph_shape = 0;
@@ -161,22 +169,22 @@ int resident_grid(K kernel, int threads, int n_cu, int items) {
   return std::max(1, std::min(items, per_cu * n_cu));
 }
 
+// rows: the three-wavefront row kernel (sparse steps); else strips
 template <int NS>
-void launch_stencil(imexsw_ctx* c, int compute_dt) {
+void launch_stencil(imexsw_ctx* c, int compute_dt, bool rows) {
   // work items: (listed tile, strip) pairs, at most tiles * kTileH / row_unit
   const int items = c->n_tiles * (kTileH / c->row_unit);
-  if (c->P.ny > 1) {
-    // strips taller than one row, and one-row strips (the kernel that finds
-    // the step's strip height is not its own returns at once)
-    static int res = 0, resr = 0;
+  if (c->P.ny > 1 && rows) {
+    static int res = 0;
+    if (!res) res = resident_grid(k_stencil_rows<NS>, 192, c->n_cu, 1 << 30);
+    hipLaunchKernelGGL((k_stencil_rows<NS>), dim3(std::min(c->n_tiles * kTileH, res)),
+                       dim3(192), 0, c->stream, c->P, c->D, compute_dt);
+  } else if (c->P.ny > 1) {
+    static int res = 0;
     if (!res) res = resident_grid(k_stencil<NS, true>, 64, c->n_cu, 1 << 30);
-    if (!resr) resr = resident_grid(k_stencil_rows<NS>, 192, c->n_cu, 1 << 30);
     hipLaunchKernelGGL((k_stencil<NS, true>), dim3(std::min(items, res)), dim3(64), 0,
                        c->stream, c->P, c->D, c->row_unit, c->strip_h, c->target_strips,
                        compute_dt);
-    hipLaunchKernelGGL((k_stencil_rows<NS>), dim3(std::min(c->n_tiles * kTileH, resr)),
-                       dim3(192), 0, c->stream, c->P, c->D, c->row_unit, c->strip_h,
-                       c->target_strips, compute_dt);
   } else {
     static int res = 0;
     if (!res) res = resident_grid(k_stencil<NS, false>, 64, c->n_cu, 1 << 30);
@@ -185,11 +193,19 @@ void launch_stencil(imexsw_ctx* c, int compute_dt) {
                        compute_dt);
   }
 }
+// fused: stage update, erosion and diagnostics of a row in one pass (sparse
+// steps without the learned warm start); else up to two kernels
 template <int NS>
-void launch_local(imexsw_ctx* c, int ops) {
-  // work items: listed tile rows, four per workgroup; the stage update and
-  // the erosion/diagnostics part are two kernels
+void launch_local(imexsw_ctx* c, int ops, bool fused) {
+  // work items: listed tile rows, four per workgroup
   const int items = (c->n_tiles * kTileH + 3) / 4;
+  if (fused) {
+    static int res3 = 0;
+    if (!res3) res3 = resident_grid(k_local<NS, 3>, 256, c->n_cu, 1 << 30);
+    hipLaunchKernelGGL((k_local<NS, 3>), dim3(std::min(items, res3)), dim3(256), 0, c->stream,
+                       c->P, c->D, ops);
+    return;
+  }
   static int res0 = 0, res1 = 0, res2 = 0;
   if (!res0) res0 = resident_grid(k_local<NS, 0>, 256, c->n_cu, 1 << 30);
   if (!res1) res1 = resident_grid(k_local<NS, 1>, 256, c->n_cu, 1 << 30);
@@ -231,10 +247,10 @@ void launch_qp(imexsw_ctx* c, double* dst) {
     default: CALL(6); break;       \
   }
 
-int run_stencil(imexsw_ctx* c, int compute_dt) {
+int run_stencil(imexsw_ctx* c, int compute_dt, bool rows = false) {
   {
     Timed tm(c, K_STENCIL);
-#define CALL(N) launch_stencil<N>(c, compute_dt)
+#define CALL(N) launch_stencil<N>(c, compute_dt, rows)
     NS_SWITCH(c->ns, CALL)
 #undef CALL
   }
@@ -242,10 +258,10 @@ int run_stencil(imexsw_ctx* c, int compute_dt) {
   return check_launch(c, "k_stencil");
 }
 
-int run_local(imexsw_ctx* c, int ops, int kid) {
+int run_local(imexsw_ctx* c, int ops, int kid, bool fused = false) {
   {
     Timed tm(c, kid);
-#define CALL(N) launch_local<N>(c, ops)
+#define CALL(N) launch_local<N>(c, ops, fused)
     NS_SWITCH(c->ns, CALL)
 #undef CALL
   }
@@ -277,22 +293,39 @@ int run_topo(imexsw_ctx* c, int sparse) {
 int enqueue_step(imexsw_ctx* c) {
   hipLaunchKernelGGL(k_reset_step, dim3(1), dim3(1), 0, c->stream, c->P, c->D);
   if (run_mask(c, 0)) return -1;
-  if (run_stencil(c, c->P.cfl != -1.0)) return -1;
+  if (run_stencil(c, c->P.cfl != -1.0, c->sparse)) return -1;
   {
     Timed tm(c, K_DTPOLICY);
     hipLaunchKernelGGL(k_dtpolicy, dim3(1), dim3(1), 0, c->stream, c->P, c->D);
   }
-  if (run_local(c, OP_RK | OP_EROSION | OP_POST, K_LOCAL)) return -1;
+  if (run_local(c, OP_RK | OP_EROSION | OP_POST, K_LOCAL, c->sparse)) return -1;
   if (c->P.topo && run_topo(c, 1)) return -1;
   hipLaunchKernelGGL(k_advance_time, dim3(1), dim3(1), 0, c->stream, c->D);
   return check_launch(c, "k_advance_time");
 }
 
 void drop_ctx_graph(imexsw_ctx* c) {
-  if (c->exec) (void)hipGraphExecDestroy(c->exec);
-  if (c->graph) (void)hipGraphDestroy(c->graph);
-  c->exec = nullptr;
-  c->graph = nullptr;
+  for (int m = 0; m < 2; m++) {
+    if (c->exec[m]) (void)hipGraphExecDestroy(c->exec[m]);
+    if (c->graph[m]) (void)hipGraphDestroy(c->graph[m]);
+    c->exec[m] = nullptr;
+    c->graph[m] = nullptr;
+  }
+}
+
+// Sparse steps when the stencil strips would be one row high (the
+// strip-height rule of k_stencil applied to w computed cells) and the
+// learned warm start is off.
+bool want_sparse(const imexsw_ctx* c, unsigned long long w) {
+  if (c->warm || c->D.harvest || c->P.ny < 2) return false;
+  return c->strip_h < 2 ||
+         w / (unsigned long long)(kStripW * 2) < (unsigned long long)c->target_strips;
+}
+
+unsigned long long sum_slots(const unsigned long long* v) {
+  unsigned long long s = 0;
+  for (int i = 0; i < kSlots; i++) s += v[i];
+  return s;
 }
 
 int set_scalar(imexsw_ctx* c, int slot, double v) {
@@ -546,6 +579,9 @@ void imexsw_destroy(imexsw_ctx* c) {
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   collect_timing(c);
   drop_ctx_graph(c);
+  for (hipEvent_t e : c->cnt_ev)
+    if (e) (void)hipEventDestroy(e);
+  if (c->h_cnt) (void)hipHostFree(c->h_cnt);
   if (c->D.harvest) (void)hipFree(c->D.harvest);
   for (void* p : c->allocs) (void)hipFree(p);
   if (c->stage) (void)hipFree(c->stage);
@@ -772,44 +808,69 @@ int imexsw_advance(imexsw_ctx* c, int32_t nsteps, double dt0, double* t_io, doub
     c->dt_hist = true;
   }
   HIPCHK(c, hipMemcpyAsync(c->D.scal, s, sizeof s, hipMemcpyHostToDevice, c->stream));
+  if (!c->h_cnt) {
+    HIPCHK(c, hipHostMalloc((void**)&c->h_cnt, 2 * kSlots * sizeof(unsigned long long)));
+    for (hipEvent_t& e : c->cnt_ev) HIPCHK(c, hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  }
+  const unsigned long long* d_cnt = c->D.counters + C_COMPUTE_CELLS * kSlots;
   unsigned long long steps0 = 0;
   HIPCHK(c, hipMemcpyAsync(&steps0, c->D.counters + C_STEPS * kSlots, sizeof steps0, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->h_cnt, d_cnt, kSlots * sizeof(unsigned long long),
+                           hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
+  c->sparse = want_sparse(c, sum_slots(c->h_cnt));
   // Without per-kernel timing the steps run as replays of a captured graph
   // of kGraphSteps steps (one host call per replay instead of eight kernel
   // launches per step); the rest, and every step when timing, is launched
-  // directly.  The graph is recaptured when the context's constants or
-  // pointers changed since the capture.
+  // directly.  The graphs are recaptured when the context's constants or
+  // pointers changed since the capture.  Each replay is followed by a copy of
+  // its last step's computed-cell count, which decides between sparse and
+  // dense steps two replays later (so the device always has a replay queued).
   int n = 0;
   if (!c->timing && nsteps >= kGraphSteps) {
-    if (!c->exec || std::memcmp(&c->graph_P, &c->P, sizeof(DevParams)) ||
+    if (std::memcmp(&c->graph_P, &c->P, sizeof(DevParams)) ||
         std::memcmp(&c->graph_D, &c->D, sizeof(DevPtrs)) ||
         c->graph_shape != c->strip_h * 1000003 + c->target_strips || c->graph_warm != c->warm ||
         std::memcmp(&c->graph_mlp, &c->mlp, sizeof(MlpDev))) {
       drop_ctx_graph(c);
-      HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-      int rc = 0;
-      for (int i = 0; i < kGraphSteps && !rc; i++) rc = enqueue_step(c);
-      hipGraph_t g = nullptr;
-      hipError_t err = hipStreamEndCapture(c->stream, &g);
-      if (rc) {
-        if (g) (void)hipGraphDestroy(g);
-        return -1;
-      }
-      if (err != hipSuccess) return fail(c, "hipStreamEndCapture: %s", hipGetErrorString(err));
-      c->graph = g;
-      HIPCHK(c, hipGraphInstantiate(&c->exec, g, nullptr, nullptr, 0));
       c->graph_P = c->P;
       c->graph_D = c->D;
       c->graph_shape = c->strip_h * 1000003 + c->target_strips;
       c->graph_warm = c->warm;
       c->graph_mlp = c->mlp;
     }
-    for (; n + kGraphSteps <= nsteps; n += kGraphSteps)
-      HIPCHK(c, hipGraphLaunch(c->exec, c->stream));
+    for (int r = 0; n + kGraphSteps <= nsteps; n += kGraphSteps, r++) {
+      if (r >= 2) {
+        HIPCHK(c, hipEventSynchronize(c->cnt_ev[r & 1]));
+        c->sparse = want_sparse(c, sum_slots(c->h_cnt + (r & 1) * kSlots));
+      }
+      const int m = c->sparse ? 1 : 0;
+      if (!c->exec[m]) {
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        int rc = 0;
+        for (int i = 0; i < kGraphSteps && !rc; i++) rc = enqueue_step(c);
+        hipGraph_t g = nullptr;
+        hipError_t err = hipStreamEndCapture(c->stream, &g);
+        if (rc) {
+          if (g) (void)hipGraphDestroy(g);
+          return -1;
+        }
+        if (err != hipSuccess) return fail(c, "hipStreamEndCapture: %s", hipGetErrorString(err));
+        c->graph[m] = g;
+        HIPCHK(c, hipGraphInstantiate(&c->exec[m], g, nullptr, nullptr, 0));
+      }
+      HIPCHK(c, hipGraphLaunch(c->exec[m], c->stream));
+      if (c->sparse) c->sparse_steps += kGraphSteps;
+      HIPCHK(c, hipMemcpyAsync(c->h_cnt + (r & 1) * kSlots, d_cnt, kSlots * sizeof(unsigned long long),
+                               hipMemcpyDeviceToHost, c->stream));
+      HIPCHK(c, hipEventRecord(c->cnt_ev[r & 1], c->stream));
+    }
   }
-  for (; n < nsteps; n++)
+  for (; n < nsteps; n++) {
     if (enqueue_step(c)) return -1;
+    if (c->sparse) c->sparse_steps++;
+  }
   c->d_valid = false;
   if (get_scalars(c, s)) return -1;
   *t_io = s[S_T];
@@ -912,9 +973,11 @@ int imexsw_stats_get(imexsw_ctx* c, imexsw_stats* st, int reset) {
   st->newton_iter_max = (int64_t)v[C_ITMAX];
   st->warm_cells = (int64_t)v[C_WARM_USED];
   st->warm_fallbacks = (int64_t)v[C_WARM_FALLBACK];
+  st->sparse_steps = c->sparse_steps;
   st->solve_cells = (int32_t)v[C_SOLVE_CELLS];
   st->compute_cells = (int32_t)v[C_COMPUTE_CELLS];
   if (reset) {
+    c->sparse_steps = 0;
     // keep the current solve_cells / compute_cells slots, clear the rest
     for (size_t i = 0; i < raw.size(); i++)
       if (i / kSlots != C_SOLVE_CELLS && i / kSlots != C_COMPUTE_CELLS) raw[i] = 0;

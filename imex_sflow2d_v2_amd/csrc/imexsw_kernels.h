@@ -807,8 +807,7 @@ __device__ __forceinline__ int strip_height(const DevParams& P, unsigned long lo
 
 // The stencil pass over the listed work tiles, cut into strips of strip_h
 // rows; workgroups take (tile, strip) items in turn until the list is
-// exhausted.  On a 2-D grid, one-row strips are left to k_stencil_rows; on
-// a 1-D grid every item is a listed row.
+// exhausted.  With one-row strips every item is a listed row.
 template <int NS, bool HASY>
 __global__ void __launch_bounds__(64) k_stencil(DevParams P, DevPtrs D, int unit,
                                                 int max_strip, int target_strips,
@@ -825,11 +824,10 @@ __global__ void __launch_bounds__(64) k_stencil(DevParams P, DevPtrs D, int unit
   pin(nrows);
   if (done != 0.0) return;
   const int strip_h = strip_height(P, w, unit, max_strip, target_strips);
-  if (HASY && strip_h == 1) return;
   // the ring cells' time factors are the same for the whole step
   const SrcTime T = P.radial ? src_time(P, t) : SrcTime{};
   const int tiles_x = (P.nx + kStripW - 1) / kStripW;
-  if (!HASY) {
+  if (strip_h == 1) {
     for (int item = blockIdx.x; item < nrows; item += gridDim.x) {
       const int row = D.row_list[item];
       const int tile = row / kTileH;
@@ -1019,8 +1017,8 @@ __device__ __forceinline__ void stencil_strip(const DevParams& P, const DevPtrs&
 }
 
 // ---------------------------------------------------------------------------
-// k_stencil_rows: the stencil pass when its strips are one row high (few
-// computed cells: the pass is bound by the latency of one row, not by
+// k_stencil_rows: the stencil pass over the listed rows when few cells are
+// computed (the pass is then bound by the latency of one row, not by
 // bandwidth).  Three wavefronts share a listed row: wavefront 0 computes
 // the fluxes through the row's S faces, wavefront 1 through its N faces,
 // wavefront 2 the x faces and then, with the y fluxes from LDS, the cell
@@ -1121,24 +1119,19 @@ __device__ __forceinline__ double yflux_N(const DevParams& P, const DevPtrs& D,
 }
 
 template <int NS>
-__global__ void __launch_bounds__(192) k_stencil_rows(DevParams P, DevPtrs D, int unit,
-                                                      int max_strip, int target_strips,
-                                                      int compute_dt) {
+__global__ void __launch_bounds__(192) k_stencil_rows(DevParams P, DevPtrs D, int compute_dt) {
   constexpr int NV = 4 + NS;
   __shared__ double yS[NV + 1][64], yN[NV + 1][64];  // fluxes, then the wave speed
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   // the step's scalars, list size and first row in one round of loads (the
   // grid is never larger than the row list, so row_list[blockIdx.x] exists)
   double done = D.scal[S_DONE], t = D.scal[S_T];
-  unsigned long long w = *ctr(D, C_COMPUTE_CELLS, lane);
   int nrows = (int)*ctr(D, C_ROWS, 0), row = D.row_list[blockIdx.x];
   pin(done);
   pin(t);
-  pin(w);
   pin(nrows);
   pin(row);
   if (done != 0.0) return;
-  if (strip_height(P, w, unit, max_strip, target_strips) != 1) return;
   const SrcTime T = P.radial ? src_time(P, t) : SrcTime{};
   const int tiles_x = (P.nx + kStripW - 1) / kStripW;
   double dtmin = P.max_dt;
@@ -1256,8 +1249,11 @@ enum LocalOps { OP_RK = 1, OP_EROSION = 2, OP_POST = 4, OP_SUMS = 8, OP_WARM = 1
 // PH selects the part compiled in: 0 = stage update and assembly checks
 // (OP_RK), 1 = erosion/deposition and diagnostics (OP_EROSION, OP_POST,
 // OP_SUMS), 2 = the stage update with the learned warm start and the pair
-// harvest (OP_WARM, OP_HARVEST).  Separate kernels keep the plain stage
-// update's register footprint free of the learned part.
+// harvest (OP_WARM, OP_HARVEST), 3 = parts 0 and 1 in one pass (few
+// computed cells: a row goes through both without waiting for every other
+// row's stage update).  Separate kernels keep the plain stage update's
+// register footprint free of the learned part, and on a wet grid the stage
+// update's free of the erosion part.
 template <int NS, int PH>
 __device__ __forceinline__ void local_cell(const DevParams& P, const DevPtrs& D, size_t c,
                                            int ops, unsigned long long* e, int& imax,
@@ -1267,10 +1263,10 @@ __device__ __forceinline__ void local_cell(const DevParams& P, const DevPtrs& D,
   constexpr int NSX = NS > 0 ? NS : 1;
   const size_t pl = plane(P);
   const double dt = D.scal[S_DT];
-  const bool rk = (PH == 0 || PH == 2) && (ops & OP_RK);
-  const bool ero = PH == 1 && (ops & OP_EROSION) &&
+  const bool rk = (PH == 0 || PH == 2 || PH == 3) && (ops & OP_RK);
+  const bool ero = (PH == 1 || PH == 3) && (ops & OP_EROSION) &&
                    !((P.erosion_coeff == 0.0) && !P.settling && !P.entrain);
-  const bool post = PH == 1 && (ops & OP_POST);
+  const bool post = (PH == 1 || PH == 3) && (ops & OP_POST);
   // every load of the cell first, the mask byte among them, so that the
   // cell waits for memory once
   double q[NV], qp[NV + 2];
@@ -1517,7 +1513,7 @@ __device__ __forceinline__ void local_row(const DevParams& P, const DevPtrs& D, 
   const bool own = lane >= 2 && lane < 2 + kStripW && j < P.nx && k < P.ny;
   double s_mass = 0.0, s_vol = 0.0, s_area = 0.0, s_sol = 0.0;
   if (own) local_cell<NS, PH>(P, D, cidx(P, j, k), ops, e, imax, s_mass, s_vol, s_area, s_sol);
-  if (PH == 1 && (ops & OP_SUMS)) {
+  if ((PH == 1 || PH == 3) && (ops & OP_SUMS)) {
     double v4[4] = {s_mass, s_vol, s_area, s_sol};
 #pragma unroll
     for (int m = 0; m < 4; m++) {

@@ -34,6 +34,7 @@ class ImexswStats(C.Structure):
         "line_search_evals")] + [
         ("newton_iter_hist", C.c_int64 * 8), ("newton_iter_max", C.c_int64),
         ("warm_cells", C.c_int64), ("warm_fallbacks", C.c_int64),
+        ("sparse_steps", C.c_int64),
         ("solve_cells", C.c_int32), ("compute_cells", C.c_int32)]
 
 

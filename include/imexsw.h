@@ -147,6 +147,9 @@ typedef struct imexsw_stats {
   int64_t newton_iter_max;      /* most iterations any cell took */
   int64_t warm_cells;           /* Newton solves started from the learned guess */
   int64_t warm_fallbacks;       /* ... that fell back to the plain predictor */
+  int64_t sparse_steps;         /* steps run in the few-computed-cells form
+                                   (row stencil, one pass per row for the
+                                   stage update and erosion); same results */
   int32_t solve_cells;          /* active cells of the last check_solve */
   int32_t compute_cells;        /* of those, cells whose update is computed
                                    (the rest provably keep their state) */

@@ -174,6 +174,36 @@ def test_strip_shapes_bitwise_identical(tuning):
     assert np.array_equal(ref.q(), alt.q())
 
 
+@pytest.mark.parametrize("curv", [False, True])
+def test_sparse_and_dense_steps_bitwise_identical(curv):
+    """Sparse steps (the three-wavefront row stencil and one pass per row for
+    the stage update, erosion and diagnostics) and dense steps (stencil
+    strips, separate stage-update and erosion kernels) are two schedules of
+    the same arithmetic: state, erosion fields, maxima and time agree to the
+    bit, over graph replays and direct launches."""
+    d = decks.small_2d(n=70, ny=50, seed=10, erosion=True, topo_change=True, source=True,
+                       slope_correction=curv, curvature=curv, wet_floor=0.05)
+    runs = []
+    for tuning in ((16, 1), (1, 2048)):
+        g = Solver(d.params)
+        g.set_tuning(*tuning)
+        g.load_deck(d)
+        g.advance(60, dt0=d.dt0)
+        g.advance(7, dt0=d.dt0)
+        runs.append((g, g.stats()))
+    (a, sa), (b, sb) = runs
+    print("sparse steps: dense run %d, sparse run %d" % (sa["sparse_steps"], sb["sparse_steps"]))
+    assert sa["sparse_steps"] == 0 and sb["sparse_steps"] == 67
+    assert a.t == b.t
+    assert np.array_equal(a.q(), b.q())
+    for x, y in zip(a.post(), b.post()):
+        assert np.array_equal(x, y)
+    for x, y in zip(a.erosion_state(), b.erosion_state()):
+        assert np.array_equal(x, y)
+    assert sa["newton_iters"] == sb["newton_iters"]
+    assert sa["line_search_evals"] == sb["line_search_evals"]
+
+
 def test_graph_replays_match_direct_launches():
     """imexsw_advance replays a captured graph of 20 steps when per-kernel
     timing is off; every replay, with host downloads and synchronisations in
