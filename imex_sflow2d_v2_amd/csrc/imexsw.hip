@@ -200,10 +200,11 @@ void launch_local(imexsw_ctx* c, int ops, bool fused) {
   // work items: listed tile rows, four per workgroup
   const int items = (c->n_tiles * kTileH + 3) / 4;
   if (fused) {
+    // half rows, four per workgroup
     static int res3 = 0;
     if (!res3) res3 = resident_grid(k_local<NS, 3>, 256, c->n_cu, 1 << 30);
-    hipLaunchKernelGGL((k_local<NS, 3>), dim3(std::min(items, res3)), dim3(256), 0, c->stream,
-                       c->P, c->D, ops);
+    hipLaunchKernelGGL((k_local<NS, 3>), dim3(std::min(2 * items, res3)), dim3(256), 0,
+                       c->stream, c->P, c->D, ops & ~(int)OP_SUMS);
     return;
   }
   static int res0 = 0, res1 = 0, res2 = 0;
@@ -211,16 +212,16 @@ void launch_local(imexsw_ctx* c, int ops, bool fused) {
   if (!res1) res1 = resident_grid(k_local<NS, 1>, 256, c->n_cu, 1 << 30);
   if (!res2) res2 = resident_grid(k_local<NS, 2>, 256, c->n_cu, 1 << 30);
   if (ops & OP_RK) {
-    int rk = OP_RK;
+    int rk = OP_RK | (ops & OP_POLICY);
     if (c->warm) {
       static int resg = 0;
       if (!resg) resg = resident_grid(k_mlp_guess<NS>, 64, c->n_cu, 1 << 30);
       hipLaunchKernelGGL((k_mlp_guess<NS>), dim3(std::min(c->n_tiles * kTileH, resg)), dim3(64),
-                         0, c->stream, c->P, c->D, c->mlp);
+                         0, c->stream, c->P, c->D, c->mlp, (ops & OP_POLICY) ? 1 : 0);
       rk |= OP_WARM;
     }
     if (c->D.harvest) rk |= OP_HARVEST;
-    if (rk == OP_RK)
+    if (!(rk & (OP_WARM | OP_HARVEST)))
       hipLaunchKernelGGL((k_local<NS, 0>), dim3(std::min(items, res0)), dim3(256), 0, c->stream,
                          c->P, c->D, rk);
     else
@@ -289,19 +290,24 @@ int run_topo(imexsw_ctx* c, int sparse) {
 }
 
 // one time step of the main loop (src/IMEX_SfloW2D.f90:337-531), all on the
-// context's stream
-int enqueue_step(imexsw_ctx* c) {
-  hipLaunchKernelGGL(k_reset_step, dim3(1), dim3(1), 0, c->stream, c->P, c->D);
-  if (run_mask(c, 0)) return -1;
-  if (run_stencil(c, c->P.cfl != -1.0, c->sparse)) return -1;
+// context's stream.  The stage kernel applies the time-step policy itself;
+// k_step_begin of the next step (commit_prev) or end_steps commits it and
+// advances t.
+int enqueue_step(imexsw_ctx* c, bool commit_prev) {
   {
     Timed tm(c, K_DTPOLICY);
-    hipLaunchKernelGGL(k_dtpolicy, dim3(1), dim3(1), 0, c->stream, c->P, c->D);
+    hipLaunchKernelGGL(k_step_begin, dim3(1), dim3(1), 0, c->stream, c->P, c->D,
+                       commit_prev ? 1 : 0, 1);
   }
-  if (run_local(c, OP_RK | OP_EROSION | OP_POST, K_LOCAL, c->sparse)) return -1;
+  if (run_mask(c, 0)) return -1;
+  if (run_stencil(c, c->P.cfl != -1.0, c->sparse)) return -1;
+  if (run_local(c, OP_RK | OP_EROSION | OP_POST | OP_POLICY, K_LOCAL, c->sparse)) return -1;
   if (c->P.topo && run_topo(c, 1)) return -1;
-  hipLaunchKernelGGL(k_advance_time, dim3(1), dim3(1), 0, c->stream, c->D);
-  return check_launch(c, "k_advance_time");
+  return check_launch(c, "k_step_begin");
+}
+int end_steps(imexsw_ctx* c) {
+  hipLaunchKernelGGL(k_step_begin, dim3(1), dim3(1), 0, c->stream, c->P, c->D, 1, 0);
+  return check_launch(c, "k_step_begin");
 }
 
 void drop_ctx_graph(imexsw_ctx* c) {
@@ -849,7 +855,8 @@ int imexsw_advance(imexsw_ctx* c, int32_t nsteps, double dt0, double* t_io, doub
         HIPCHK(c, hipStreamSynchronize(c->stream));
         HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
         int rc = 0;
-        for (int i = 0; i < kGraphSteps && !rc; i++) rc = enqueue_step(c);
+        for (int i = 0; i < kGraphSteps && !rc; i++) rc = enqueue_step(c, i > 0);
+        if (!rc) rc = end_steps(c);
         hipGraph_t g = nullptr;
         hipError_t err = hipStreamEndCapture(c->stream, &g);
         if (rc) {
@@ -868,7 +875,7 @@ int imexsw_advance(imexsw_ctx* c, int32_t nsteps, double dt0, double* t_io, doub
     }
   }
   for (; n < nsteps; n++) {
-    if (enqueue_step(c)) return -1;
+    if (enqueue_step(c, false) || end_steps(c)) return -1;
     if (c->sparse) c->sparse_steps++;
   }
   c->d_valid = false;

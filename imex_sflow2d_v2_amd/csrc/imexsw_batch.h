@@ -104,7 +104,7 @@ __global__ void __launch_bounds__(256) kb_local(const DevParams* __restrict__ Ps
     for (int i = lane; i < kLocalEvents; i += 64) cnt[w][i] = 0;
     __builtin_amdgcn_wave_barrier();
     int imax = 0;
-    local_row<NS, PH>(P, D, e & kIdMask, ops, cnt[w], imax);
+    local_row<NS, PH>(P, D, e & kIdMask, D.scal[S_DT], ops, cnt[w], imax);
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
     __builtin_amdgcn_wave_barrier();
     if (lane < kLocalEvents && cnt[w][lane])

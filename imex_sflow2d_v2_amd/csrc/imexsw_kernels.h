@@ -1185,26 +1185,46 @@ __global__ void __launch_bounds__(192) k_stencil_rows(DevParams P, DevPtrs D, in
 }
 
 // ---------------------------------------------------------------------------
-// k_dtpolicy: src/IMEX_SfloW2D.f90:357-381 on the device (one thread)
+// the host loop's time-step policy, src/IMEX_SfloW2D.f90:357-381, on the
+// device: step_dt evaluates it from the step's scalars and the CFL minimum
+// without writing anything (every workgroup of the stage kernel does), and
+// dtpolicy_member commits the result
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ void dtpolicy_member(const DevParams& P, const DevPtrs& D) {
-  double* s = D.scal;
-  if (s[S_DONE] != 0.0) return;
-  double t = s[S_T], t_end = s[S_T_END], t_out = s[S_T_OUT];
-  if (!(t < t_end)) {
-    s[S_DONE] = 1.0;
-    return;
-  }
+struct StepDt {
+  bool done;       // the run was over, or t >= t_end: no update this step
+  double dt, dt_cfl, t_out;
+};
+__device__ __forceinline__ StepDt step_dt(const DevParams& P, const DevPtrs& D) {
+  const double* s = D.scal;
+  StepDt r{true, 0.0, 0.0, 0.0};
+  if (s[S_DONE] != 0.0) return r;
+  const double t = s[S_T], t_end = s[S_T_END];
+  double t_out = s[S_T_OUT];
+  if (!(t < t_end)) return r;
   double dt = (P.cfl != -1.0) ? __longlong_as_double((long long)*D.dt_bits) : P.max_dt;
-  s[S_DT_CFL] = dt;
+  r.dt_cfl = dt;
   if (t_end - t_out < 1.0e-7) t_out = t_end;
   if (t + dt > t_end) dt = t_end - t;
   if (t + dt > t_out) dt = t_out - t;
   dt = fmin(dt, 1.1 * 0.5 * (s[S_DT_OLD] + s[S_DT_OLD_OLD]));
+  r.done = false;
+  r.dt = dt;
+  r.t_out = t_out;
+  return r;
+}
+__device__ __forceinline__ void dtpolicy_member(const DevParams& P, const DevPtrs& D) {
+  double* s = D.scal;
+  if (s[S_DONE] != 0.0) return;
+  const StepDt r = step_dt(P, D);
+  if (r.done) {
+    s[S_DONE] = 1.0;
+    return;
+  }
+  s[S_DT_CFL] = r.dt_cfl;
   s[S_DT_OLD_OLD] = s[S_DT_OLD];
-  s[S_DT_OLD] = dt;
-  s[S_DT] = dt;
-  s[S_T_OUT] = t_out;
+  s[S_DT_OLD] = r.dt;
+  s[S_DT] = r.dt;
+  s[S_T_OUT] = r.t_out;
 }
 __global__ void k_dtpolicy(DevParams P, DevPtrs D) { dtpolicy_member(P, D); }
 
@@ -1236,13 +1256,27 @@ __device__ __forceinline__ void reset_member(const DevParams& P, const DevPtrs& 
 }
 __global__ void k_reset_step(DevParams P, DevPtrs D) { reset_member(P, D); }
 
+// Between two steps of imexsw_advance: commit the previous step (its time
+// step policy, t = t + dt, counters) when `commit`, then reset the per-step
+// counters for the next one.  With commit and without reset: the end of the
+// last step.
+__global__ void k_step_begin(DevParams P, DevPtrs D, int commit, int reset) {
+  if (commit) {
+    dtpolicy_member(P, D);
+    advance_member(D);
+  }
+  if (reset) reset_member(P, D);
+}
+
 // ---------------------------------------------------------------------------
 // k_local: stage 2 + assembly (imex_RK_solver :1105-1626), erosion /
 // deposition (update_erosion_deposition_cell :2380-2595) and the post-step
 // loop (src/IMEX_SfloW2D.f90:391-447), fused per cell.
 // ---------------------------------------------------------------------------
+// OP_POLICY: the step's dt is step_dt (the policy is committed after the
+// step), else the scalar S_DT
 enum LocalOps { OP_RK = 1, OP_EROSION = 2, OP_POST = 4, OP_SUMS = 8, OP_WARM = 16,
-                OP_HARVEST = 32 };
+                OP_HARVEST = 32, OP_POLICY = 64 };
 
 // one cell of k_local; e (the wavefront's counters in LDS) accumulates
 // counter C_NEWTON_CELLS + m in e[m]
@@ -1254,15 +1288,16 @@ enum LocalOps { OP_RK = 1, OP_EROSION = 2, OP_POST = 4, OP_SUMS = 8, OP_WARM = 1
 // row's stage update).  Separate kernels keep the plain stage update's
 // register footprint free of the learned part, and on a wet grid the stage
 // update's free of the erosion part.
+// (PH 3 runs each cell on a lane pair: both lanes compute, `counts` is set
+// on one of them)
 template <int NS, int PH>
 __device__ __forceinline__ void local_cell(const DevParams& P, const DevPtrs& D, size_t c,
-                                           int ops, unsigned long long* e, int& imax,
-                                           double& s_mass, double& s_vol, double& s_area,
-                                           double& s_sol) {
+                                           double dt, int ops, unsigned long long* e,
+                                           int& imax, double& s_mass, double& s_vol,
+                                           double& s_area, double& s_sol, bool counts = true) {
   constexpr int NV = 4 + NS;
   constexpr int NSX = NS > 0 ? NS : 1;
   const size_t pl = plane(P);
-  const double dt = D.scal[S_DT];
   const bool rk = (PH == 0 || PH == 2 || PH == 3) && (ops & OP_RK);
   const bool ero = (PH == 1 || PH == 3) && (ops & OP_EROSION) &&
                    !((P.erosion_coeff == 0.0) && !P.settling && !P.entrain);
@@ -1345,11 +1380,11 @@ __device__ __forceinline__ void local_cell(const DevParams& P, const DevPtrs& D,
     int nonconv = 0, ls = 0, wu = -1;
     float hv[kHarvestWidth];
     bool hok = false;
-    int it = stage2<NS>(P, q0, Dr, dt, bx, by, bxx, bxy, byy, gc, q, nonconv, ls,
+    int it = stage2<NS, PH == 3>(P, q0, Dr, dt, bx, by, bxx, bxy, byy, gc, q, nonconv, ls,
                         (PH == 2 && (ops & OP_WARM)) ? wg : nullptr, &wu,
                         (PH == 2 && (ops & OP_HARVEST)) ? hv : nullptr, &hok);
-    if (wu == 1) atomicAdd(&e[C_WARM_USED - C_NEWTON_CELLS], 1ull);
-    if (wu == 0) atomicAdd(&e[C_WARM_FALLBACK - C_NEWTON_CELLS], 1ull);
+    if (counts && wu == 1) atomicAdd(&e[C_WARM_USED - C_NEWTON_CELLS], 1ull);
+    if (counts && wu == 0) atomicAdd(&e[C_WARM_FALLBACK - C_NEWTON_CELLS], 1ull);
     if (PH == 2 && (ops & OP_HARVEST)) {
       // wave-aggregated append of the converged pairs
       const unsigned long long b = __ballot(hok);
@@ -1364,7 +1399,7 @@ __device__ __forceinline__ void local_cell(const DevParams& P, const DevPtrs& D,
           for (int i = 0; i < kHarvestWidth; i++) D.harvest[at * kHarvestWidth + i] = hv[i];
       }
     }
-    if (it > 0) {
+    if (it > 0 && counts) {
       atomicAdd(&e[C_NEWTON_CELLS - C_NEWTON_CELLS], 1ull);
       atomicAdd(&e[C_NEWTON_ITERS - C_NEWTON_CELLS], (unsigned long long)it);
       if (nonconv) atomicAdd(&e[C_NONCONV - C_NEWTON_CELLS], 1ull);
@@ -1377,13 +1412,13 @@ __device__ __forceinline__ void local_cell(const DevParams& P, const DevPtrs& D,
     bool isn = false;
 #pragma unroll
     for (int i = 0; i < NV; i++) isn |= isnan(q[i]);
-    if (isn) atomicAdd(&e[C_NAN - C_NEWTON_CELLS], 1ull);
+    if (counts && isn) atomicAdd(&e[C_NAN - C_NEWTON_CELLS], 1ull);
     if (q[0] < kEps) {
       if (q[0] > -1.0e-7) {
 #pragma unroll
         for (int i = 0; i < NV; i++) q[i] = 0.0;
       } else {
-        atomicAdd(&e[C_NEGH - C_NEWTON_CELLS], 1ull);
+        if (counts) atomicAdd(&e[C_NEGH - C_NEWTON_CELLS], 1ull);
       }
     }
     {
@@ -1394,7 +1429,7 @@ __device__ __forceinline__ void local_cell(const DevParams& P, const DevPtrs& D,
         anybig |= q[4 + i] <= -1.0e-7;
       }
       if (anyneg) {
-        if (anybig) atomicAdd(&e[C_NEGS - C_NEWTON_CELLS], 1ull);
+        if (anybig && counts) atomicAdd(&e[C_NEGS - C_NEWTON_CELLS], 1ull);
         else {
 #pragma unroll
           for (int i = 0; i < NS; i++)
@@ -1403,7 +1438,7 @@ __device__ __forceinline__ void local_cell(const DevParams& P, const DevPtrs& D,
       }
     }
     phys<NS>(P, q, qp);
-    if (qp[3] < 273.0) atomicAdd(&e[C_LOWT - C_NEWTON_CELLS], 1ull);
+    if (counts && qp[3] < 273.0) atomicAdd(&e[C_LOWT - C_NEWTON_CELLS], 1ull);
     double ssol = 0.0;
 #pragma unroll
     for (int i = 0; i < NS; i++) ssol += q[4 + i];
@@ -1412,7 +1447,7 @@ __device__ __forceinline__ void local_cell(const DevParams& P, const DevPtrs& D,
 #pragma unroll
         for (int i = 0; i < NS; i++) q[4 + i] = q[4 + i] / ssol * q[0];
       } else {
-        atomicAdd(&e[C_SUMS - C_NEWTON_CELLS], 1ull);
+        if (counts) atomicAdd(&e[C_SUMS - C_NEWTON_CELLS], 1ull);
       }
     }
     if (D.nodata && D.nodata[c]) {
@@ -1423,7 +1458,7 @@ __device__ __forceinline__ void local_cell(const DevParams& P, const DevPtrs& D,
   if (ero) {
     phys_or_dry<NS>(P, q, qp);
     double er[NSX], dep[NSX], eq[NV], topo;
-    mass_exchange<NS>(P, qp, bx, by, erod, dt, er, dep, eq, topo);
+    mass_exchange<NS, PH == 3>(P, qp, bx, by, erod, dt, er, dep, eq, topo);
 #pragma unroll
     for (int i = 0; i < NV; i++) q[i] = q[i] + dt * eq[i];
 #pragma unroll
@@ -1504,7 +1539,8 @@ __device__ __forceinline__ void local_cell(const DevParams& P, const DevPtrs& D,
 // sums go to tile_sums[4 * row ..] when OP_SUMS.
 template <int NS, int PH>
 __device__ __forceinline__ void local_row(const DevParams& P, const DevPtrs& D, int row,
-                                          int ops, unsigned long long* e, int& imax) {
+                                          double dt, int ops, unsigned long long* e,
+                                          int& imax) {
   const int lane = threadIdx.x & 63;
   const int tiles_x = (P.nx + kStripW - 1) / kStripW;
   const int tile = row / kTileH;
@@ -1512,7 +1548,8 @@ __device__ __forceinline__ void local_row(const DevParams& P, const DevPtrs& D, 
   const int k = (tile / tiles_x) * kTileH + row % kTileH;
   const bool own = lane >= 2 && lane < 2 + kStripW && j < P.nx && k < P.ny;
   double s_mass = 0.0, s_vol = 0.0, s_area = 0.0, s_sol = 0.0;
-  if (own) local_cell<NS, PH>(P, D, cidx(P, j, k), ops, e, imax, s_mass, s_vol, s_area, s_sol);
+  if (own)
+    local_cell<NS, PH>(P, D, cidx(P, j, k), dt, ops, e, imax, s_mass, s_vol, s_area, s_sol);
   if ((PH == 1 || PH == 3) && (ops & OP_SUMS)) {
     double v4[4] = {s_mass, s_vol, s_area, s_sol};
 #pragma unroll
@@ -1524,22 +1561,47 @@ __device__ __forceinline__ void local_row(const DevParams& P, const DevPtrs& D, 
   }
 }
 
+// Half a listed tile row (30 cells) for PH 3, one cell per lane pair (lanes
+// 2i and 2i+1 take cell i; lanes 60..63 idle).  No per-row sums.
+template <int NS>
+__device__ __forceinline__ void local_half_row(const DevParams& P, const DevPtrs& D, int row,
+                                               int half, double dt, int ops,
+                                               unsigned long long* e, int& imax) {
+  const int lane = threadIdx.x & 63, i = lane >> 1;
+  const int tiles_x = (P.nx + kStripW - 1) / kStripW;
+  const int tile = row / kTileH;
+  const int j = (tile % tiles_x) * kStripW + half * (kStripW / 2) + i;
+  const int k = (tile / tiles_x) * kTileH + row % kTileH;
+  double s_mass = 0.0, s_vol = 0.0, s_area = 0.0, s_sol = 0.0;
+  if (i < kStripW / 2 && j < P.nx && k < P.ny)
+    local_cell<NS, 3>(P, D, cidx(P, j, k), dt, ops & ~(int)OP_SUMS, e, imax, s_mass, s_vol,
+                      s_area, s_sol, (lane & 1) == 0);
+}
+
 // Learned Newton starting points for this step: for every computed cell of
 // the listed rows, the stage-2 predictor, its features, and the MLP on the
 // matrix cores (the 64 lanes of a wavefront are the 64 samples); the
 // starting momenta q_si * (1 + prediction) go to the guess planes (NaN where
 // the cell does not enter Newton).
 template <int NS>
-__global__ void __launch_bounds__(64) k_mlp_guess(DevParams P, DevPtrs D, MlpDev M) {
+__global__ void __launch_bounds__(64) k_mlp_guess(DevParams P, DevPtrs D, MlpDev M,
+                                                   int policy) {
   constexpr int NV = 4 + NS;
   __shared__ short xs[64][kMlpInPad];
   __shared__ float ys[64][kMlpOut];
   const int lane = threadIdx.x, i = lane & 15, g = lane >> 4;
-  if (D.scal[S_DONE] != 0.0) return;
+  double dt;
+  if (policy) {
+    const StepDt r = step_dt(P, D);
+    if (r.done) return;
+    dt = r.dt;
+  } else {
+    if (D.scal[S_DONE] != 0.0) return;
+    dt = D.scal[S_DT];
+  }
   const int nrows = (int)*ctr(D, C_ROWS, 0);
   const int tiles_x = (P.nx + kStripW - 1) / kStripW;
   const size_t pl = plane(P);
-  const double dt = D.scal[S_DT];
   for (int it = blockIdx.x; it < nrows; it += gridDim.x) {
     const int row = D.row_list[it];
     const int tile = row / kTileH;
@@ -1602,8 +1664,8 @@ __global__ void __launch_bounds__(64) k_mlp_guess(DevParams P, DevPtrs D, MlpDev
   }
 }
 
-// One wavefront per listed tile row; the four wavefronts of a workgroup
-// take rows in turn until the list is exhausted.
+// One wavefront per listed tile row (per half row for PH 3); the four
+// wavefronts of a workgroup take rows in turn until the list is exhausted.
 template <int NS, int PH>
 __global__ void __launch_bounds__(256) k_local(DevParams P, DevPtrs D, int ops) {
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -1612,10 +1674,25 @@ __global__ void __launch_bounds__(256) k_local(DevParams P, DevPtrs D, int ops) 
   for (int m = lane; m < kLocalEvents; m += 64) cnt[w][m] = 0;
   __syncthreads();
   int imax = 0;
-  const bool live = (D.scal[S_DONE] == 0.0);
+  bool live;
+  double dt;
+  if (ops & OP_POLICY) {
+    const StepDt r = step_dt(P, D);
+    live = !r.done;
+    dt = r.dt;
+  } else {
+    live = (D.scal[S_DONE] == 0.0);
+    dt = D.scal[S_DT];
+  }
   const int nrows = (int)*ctr(D, C_ROWS, 0);
-  for (int it = blockIdx.x * 4 + w; live && it < nrows; it += gridDim.x * 4)
-    local_row<NS, PH>(P, D, D.row_list[it], ops, cnt[w], imax);
+  // PH 3: two items (half rows) per listed row
+  const int items = (PH == 3) ? 2 * nrows : nrows;
+  for (int it = blockIdx.x * 4 + w; live && it < items; it += gridDim.x * 4) {
+    if (PH == 3)
+      local_half_row<NS>(P, D, D.row_list[it >> 1], it & 1, dt, ops, cnt[w], imax);
+    else
+      local_row<NS, PH>(P, D, D.row_list[it], dt, ops, cnt[w], imax);
+  }
   for (int off = 32; off > 0; off >>= 1) imax = max(imax, __shfl_xor(imax, off, 64));
   if (lane == 0) itmax[w] = imax;
   __syncthreads();

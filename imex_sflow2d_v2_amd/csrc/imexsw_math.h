@@ -22,73 +22,63 @@ constexpr double kLn2Hi = 6.93147180369123816490e-01;  // 0x3fe62e42fee00000
 constexpr double kLn2Lo = 1.90821492927058770002e-10;
 constexpr double kInvLn2 = 1.44269504088896338700e+00;
 
+// The special cases are selected after the fact rather than branched
+// around, so that the straight-line code of independent calls can overlap.
 __device__ __forceinline__ double det_log(double x) {
   // x > 0, finite
-  int64_t bits = __double_as_longlong(x);
-  int e = (int)((bits >> 52) & 0x7ff);
-  if (e == 0) {  // subnormal: scale by 2^54
-    x = x * 18014398509481984.0;
-    bits = __double_as_longlong(x);
-    e = (int)((bits >> 52) & 0x7ff) - 54;
-  }
-  e -= 1022;  // x = m * 2^e, m in [0.5, 1)
+  const int64_t bits0 = __double_as_longlong(x);
+  const bool sub = ((bits0 >> 52) & 0x7ff) == 0;  // subnormal: scale by 2^54
+  const double xs = sub ? x * 18014398509481984.0 : x;
+  const int64_t bits = __double_as_longlong(xs);
+  int e = (int)((bits >> 52) & 0x7ff) - (sub ? 54 : 0) - 1022;  // x = m * 2^e, m in [0.5, 1)
   double m = __longlong_as_double((bits & 0x000fffffffffffffLL) | 0x3fe0000000000000LL);
-  if (m < 0.70710678118654752440) {
-    m = m * 2.0;
-    e -= 1;
-  }
+  const bool lo = m < 0.70710678118654752440;
+  m = lo ? m * 2.0 : m;
+  e -= lo ? 1 : 0;
   const double s = (m - 1.0) / (m + 1.0);
   const double s2 = s * s;
-  double p = 1.0 / 23.0;
-  p = p * s2 + 1.0 / 21.0;
-  p = p * s2 + 1.0 / 19.0;
-  p = p * s2 + 1.0 / 17.0;
-  p = p * s2 + 1.0 / 15.0;
-  p = p * s2 + 1.0 / 13.0;
-  p = p * s2 + 1.0 / 11.0;
-  p = p * s2 + 1.0 / 9.0;
-  p = p * s2 + 1.0 / 7.0;
-  p = p * s2 + 1.0 / 5.0;
-  p = p * s2 + 1.0 / 3.0;
+  // 1/3 + s2/5 + ... + s2^10/23 by Estrin's scheme (depth 7, not 20)
+  const double x2 = s2 * s2, x4 = x2 * x2, x8 = x4 * x4;
+  const double a0 = 1.0 / 3.0 + 1.0 / 5.0 * s2, a1 = 1.0 / 7.0 + 1.0 / 9.0 * s2;
+  const double a2 = 1.0 / 11.0 + 1.0 / 13.0 * s2, a3 = 1.0 / 15.0 + 1.0 / 17.0 * s2;
+  const double a4 = 1.0 / 19.0 + 1.0 / 21.0 * s2;
+  const double b0 = a0 + a1 * x2, b1 = a2 + a3 * x2, b2 = a4 + 1.0 / 23.0 * x2;
+  const double p = (b0 + b1 * x4) + b2 * x8;
   const double logm = 2.0 * s + 2.0 * s * (s2 * p);
   const double de = (double)e;
   return (de * kLn2Hi + logm) + de * kLn2Lo;
 }
 
 __device__ __forceinline__ double det_exp(double y) {
-  if (y > 709.0) return __longlong_as_double(0x7ff0000000000000LL);
-  if (y < -745.0) return 0.0;
-  const double n = rint(y * kInvLn2);
-  const double r = (y - n * kLn2Hi) - n * kLn2Lo;
-  double p = 1.0 / 87178291200.0;  // 1/14!
-  p = p * r + 1.0 / 6227020800.0;
-  p = p * r + 1.0 / 479001600.0;
-  p = p * r + 1.0 / 39916800.0;
-  p = p * r + 1.0 / 3628800.0;
-  p = p * r + 1.0 / 362880.0;
-  p = p * r + 1.0 / 40320.0;
-  p = p * r + 1.0 / 5040.0;
-  p = p * r + 1.0 / 720.0;
-  p = p * r + 1.0 / 120.0;
-  p = p * r + 1.0 / 24.0;
-  p = p * r + 1.0 / 6.0;
-  p = p * r + 0.5;
-  p = p * r + 1.0;
-  p = p * r + 1.0;
+  const bool hi = y > 709.0, lo = y < -745.0;
+  const double yc = (hi || lo) ? 0.0 : y;
+  const double n = rint(yc * kInvLn2);
+  const double r = (yc - n * kLn2Hi) - n * kLn2Lo;
+  // sum of r^k / k!, k = 0..14, by Estrin's scheme (depth 8, not 28)
+  const double r2 = r * r, r4 = r2 * r2, r8 = r4 * r4;
+  const double a0 = 1.0 + r, a1 = 0.5 + 1.0 / 6.0 * r;
+  const double a2 = 1.0 / 24.0 + 1.0 / 120.0 * r, a3 = 1.0 / 720.0 + 1.0 / 5040.0 * r;
+  const double a4 = 1.0 / 40320.0 + 1.0 / 362880.0 * r;
+  const double a5 = 1.0 / 3628800.0 + 1.0 / 39916800.0 * r;
+  const double a6 = 1.0 / 479001600.0 + 1.0 / 6227020800.0 * r;
+  const double b0 = a0 + a1 * r2, b1 = a2 + a3 * r2, b2 = a4 + a5 * r2;
+  const double b3 = a6 + 1.0 / 87178291200.0 * r2;
+  const double p = (b0 + b1 * r4) + (b2 + b3 * r4) * r8;
   // scale by 2^n in two steps to stay in range near the limits
   const int ni = (int)n;
   const int n1 = ni / 2, n2 = ni - n1;
   const double s1 = __longlong_as_double((int64_t)(1023 + n1) << 52);
   const double s2 = __longlong_as_double((int64_t)(1023 + n2) << 52);
-  return (p * s1) * s2;
+  const double v = (p * s1) * s2;
+  return hi ? __longlong_as_double(0x7ff0000000000000LL) : lo ? 0.0 : v;
 }
 
 // x**y for x >= 0 (the reference only raises non-negative quantities)
 __device__ __forceinline__ double det_pow(double x, double y) {
-  if (x == 0.0) return (y > 0.0) ? 0.0 : (y == 0.0 ? 1.0 : __longlong_as_double(0x7ff0000000000000LL));
-  if (x == 1.0 || y == 0.0) return 1.0;
-  if (x != x || y != y) return x + y;
-  return det_exp(y * det_log(x));
+  const bool z = x == 0.0, one = (x == 1.0 || y == 0.0), nan = (x != x || y != y);
+  const double v = det_exp(y * det_log((z || one || nan) ? 2.0 : x));
+  return z ? ((y > 0.0) ? 0.0 : (y == 0.0 ? 1.0 : __longlong_as_double(0x7ff0000000000000LL)))
+           : one ? 1.0 : nan ? x + y : v;
 }
 
 __device__ __forceinline__ double det_cos(double x) {

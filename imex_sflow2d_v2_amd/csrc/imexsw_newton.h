@@ -53,6 +53,14 @@ __device__ __forceinline__ double norm2_(const double* x) {
   return scale * sqrt(res);
 }
 
+// the value of the other lane of this lane's pair (lanes 2i, 2i+1)
+__device__ __forceinline__ double pair_other(double v) {
+  int lo = __double2loint(v), hi = __double2hiint(v);
+  lo = __shfl_xor(lo, 1, 64);
+  hi = __shfl_xor(hi, 1, 64);
+  return __hiloint2double(hi, lo);
+}
+
 struct NewtonOut {
   int iters;
   int conv;
@@ -64,7 +72,10 @@ struct NewtonOut {
 // src/constitutive_2d.f90:478-480): the Jacobian has the 2x2 momentum block
 // plus the diagonal scaling of the other unknowns, which is what the
 // reference's packed partitioned solve reduces to.
-template <int NS>
+// PAIR: two lanes (2i, 2i+1) solve the same cell with the same operands;
+// each evaluates one of the two complex-step Jacobian columns and takes the
+// other from its partner, so an iteration waits for one column, not two.
+template <int NS, bool PAIR = false>
 __device__ __forceinline__ NewtonOut solve_rk_step(const DevParams& P, double* qj,
                                                   const double* qold,
                                                   const double* Rj, double a_diag,
@@ -107,6 +118,23 @@ __device__ __forceinline__ NewtonOut solve_rk_step(const DevParams& P, double* q
     // --- Jacobian: complex step on the two momentum columns
     const double dta = dt * a_diag;
     double J1[3], J2[3];  // d nh_{1,2} / d rel_{1}, d nh_{1,2} / d rel_{2}
+    if (PAIR) {
+      const int col = 1 + (threadIdx.x & 1);
+      cpx cq[NV];
+#pragma unroll
+      for (int m = 0; m < NV; m++) cq[m] = mk(rel[m], 0.0);
+      cq[col] = mk(rel[col], P.h_cs);
+#pragma unroll
+      for (int m = 0; m < NV; m++) cq[m] = cmulr(cq[m], org[m]);
+      cpx n1, n2;
+      implicit_c<NS>(P, cq, Bx, By, n1, n2);
+      const double a = 1.0 * n1.im * P.one_by_h, b = 1.0 * n2.im * P.one_by_h;
+      const double ao = pair_other(a), bo = pair_other(b);
+      J1[1] = (col == 1) ? a : ao;
+      J1[2] = (col == 1) ? b : bo;
+      J2[1] = (col == 2) ? a : ao;
+      J2[2] = (col == 2) ? b : bo;
+    } else {
     {
       cpx cq[NV];
 #pragma unroll
@@ -130,6 +158,7 @@ __device__ __forceinline__ NewtonOut solve_rk_step(const DevParams& P, double* q
       implicit_c<NS>(P, cq, Bx, By, n1, n2);
       J2[1] = 1.0 * n1.im * P.one_by_h;
       J2[2] = 1.0 * n2.im * P.one_by_h;
+    }
     }
     // left_matrix entries (column 1 and 2 of the momentum block); rows of
     // the other equations of those columns are 0 - dta*(1.0*0*one_by_h) = 0
@@ -335,7 +364,7 @@ __device__ __forceinline__ bool predictor(const DevParams& P, const double* q0,
 // implicit term refresh of :1234-1236 is not evaluated: for the stiffly
 // accurate N_RK = 2 tableau it is overwritten at :1271-1277 and never enters
 // q.  Returns the Newton iteration count (0: no Newton).
-template <int NS>
+template <int NS, bool PAIR = false>
 __device__ __forceinline__ int stage2(const DevParams& P, const double* q0,
                                       const double* D, double dt, double Bx,
                                       double By, double Bxx, double Bxy,
@@ -377,10 +406,10 @@ __device__ __forceinline__ int stage2(const DevParams& P, const double* q0,
 #pragma unroll
       for (int i = 0; i < NV; i++) q[i] = qw[i];
     }
-    no = solve_rk_step<NS>(P, q, q0, Rj, 1.0, dt, Bx, By, use ? rtw : rtp, use ? sw : sp);
+    no = solve_rk_step<NS, PAIR>(P, q, q0, Rj, 1.0, dt, Bx, By, use ? rtw : rtp, use ? sw : sp);
   } else {
     if (warm_used) *warm_used = -1;
-    no = solve_rk_step<NS>(P, q, q0, Rj, 1.0, dt, Bx, By);
+    no = solve_rk_step<NS, PAIR>(P, q, q0, Rj, 1.0, dt, Bx, By);
   }
   nonconv = !no.conv;
   ls_evals = no.ls_evals;
@@ -421,7 +450,9 @@ __device__ __forceinline__ int stage2(const DevParams& P, const double* q0,
 
 // eval_mass_exchange_terms, supported branches (src/constitutive_2d.f90:
 // 4424-5003).  Returns false when every term is zero.
-template <int NS>
+// LOCKSTEP: the classes' settling iterations side by side (see
+// settling_velocities)
+template <int NS, bool LOCKSTEP>
 __device__ __forceinline__ void mass_exchange(const DevParams& P, const double* qp,
                                               double Bx, double By,
                                               const double* erodible, double dt,
@@ -467,10 +498,14 @@ __device__ __forceinline__ void mass_exchange(const DevParams& P, const double* 
   if (P.settling) {
     // hindered settling factor, the same for every class
     const double hind = det_pow(1.0 - fmin(1.0, at / alpha_max), hind_exp);
+    double svl[NS > 0 ? NS : 1];
+    if (LOCKSTEP) settling_velocities<NS>(P, m.rho_c, inv_kin_visc, al, svl);
 #pragma unroll
     for (int i = 0; i < NS; i++) {
       if (al[i] > 0.0) {
-        const double sv = settling_velocity(P, P.diam_s[i], P.rho_s[i], m.rho_c, inv_kin_visc);
+        const double sv = LOCKSTEP ? svl[i]
+                                   : settling_velocity(P, P.diam_s[i], P.rho_s[i], m.rho_c,
+                                                       inv_kin_visc);
         double d = al[i] * sv * 1.0;
         d = d * hind;
         dep[i] = fmin(d, r_h * al[i] / dt);
