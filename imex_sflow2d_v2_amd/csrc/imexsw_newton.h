@@ -450,9 +450,12 @@ __device__ __forceinline__ int stage2(const DevParams& P, const double* q0,
 
 // eval_mass_exchange_terms, supported branches (src/constitutive_2d.f90:
 // 4424-5003).  Returns false when every term is zero.
-// LOCKSTEP: the classes' settling iterations side by side (see
-// settling_velocities)
-template <int NS, bool LOCKSTEP>
+// PAIR: the cell runs on a lane pair (lanes 2i, 2i+1 with the same
+// operands); the solid classes' settling velocities are split between the
+// two lanes (class 2s on the even lane, 2s+1 on the odd one, the same
+// instructions on both) and exchanged, so the pair waits for about half of
+// them.
+template <int NS, bool PAIR = false>
 __device__ __forceinline__ void mass_exchange(const DevParams& P, const double* qp,
                                               double Bx, double By,
                                               const double* erodible, double dt,
@@ -498,14 +501,31 @@ __device__ __forceinline__ void mass_exchange(const DevParams& P, const double* 
   if (P.settling) {
     // hindered settling factor, the same for every class
     const double hind = det_pow(1.0 - fmin(1.0, at / alpha_max), hind_exp);
-    double svl[NS > 0 ? NS : 1];
-    if (LOCKSTEP) settling_velocities<NS>(P, m.rho_c, inv_kin_visc, al, svl);
+    double svs[NS > 0 ? NS : 1];
+    if (PAIR) {
+      const int p = threadIdx.x & 1;
+      double mine[(NS + 1) / 2 > 0 ? (NS + 1) / 2 : 1];
+#pragma unroll
+      for (int sl = 0; sl < (NS + 1) / 2; sl++) {
+        const int c0 = 2 * sl, c1 = (2 * sl + 1 < NS) ? 2 * sl + 1 : 2 * sl;
+        const bool has = (p == 0) || (2 * sl + 1 < NS);
+        const double a = p ? al[c1] : al[c0];
+        const double diam = p ? P.diam_s[c1] : P.diam_s[c0];
+        const double rhos = p ? P.rho_s[c1] : P.rho_s[c0];
+        mine[sl] = (has && a > 0.0) ? settling_velocity(P, diam, rhos, m.rho_c, inv_kin_visc)
+                                    : 0.0;
+      }
+#pragma unroll
+      for (int i = 0; i < NS; i++) {
+        const double o = pair_other(mine[i >> 1]);
+        svs[i] = ((i & 1) == p) ? mine[i >> 1] : o;
+      }
+    }
 #pragma unroll
     for (int i = 0; i < NS; i++) {
       if (al[i] > 0.0) {
-        const double sv = LOCKSTEP ? svl[i]
-                                   : settling_velocity(P, P.diam_s[i], P.rho_s[i], m.rho_c,
-                                                       inv_kin_visc);
+        const double sv =
+            PAIR ? svs[i] : settling_velocity(P, P.diam_s[i], P.rho_s[i], m.rho_c, inv_kin_visc);
         double d = al[i] * sv * 1.0;
         d = d * hind;
         dep[i] = fmin(d, r_h * al[i] / dt);

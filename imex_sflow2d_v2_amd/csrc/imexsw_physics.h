@@ -569,47 +569,6 @@ __device__ __forceinline__ double settling_velocity(const DevParams& P, double d
   return sv;
 }
 
-// The settling velocities of every class with al > 0, the classes'
-// fixed-point iterations run side by side: each class takes exactly its own
-// sequence of iterates (settling_velocity's operations in the same order)
-// and keeps its value from its own convergence on, but the independent
-// chains overlap instead of following one another.  Shorter on the
-// critical path of a cell, more work in total (every class iterates until
-// the slowest has converged): for steps with few computed cells.
-template <int NS>
-__device__ __forceinline__ void settling_velocities(const DevParams& P, double rhoc,
-                                                    double inv_kin_visc, const double* al,
-                                                    double* sv) {
-  constexpr int NSX = NS > 0 ? NS : 1;
-  double cp[NSX], Rey[NSX];
-  bool it[NSX];
-#pragma unroll
-  for (int i = 0; i < NS; i++) {
-    const bool on = al[i] > 0.0 && P.rho_s[i] > rhoc;
-    cp[i] = sqrt(0.75 * (P.rho_s[i] / rhoc - 1.0) * P.diam_s[i] * P.grav);
-    sv[i] = on ? cp[i] * 1.0 : 0.0;
-    Rey[i] = P.diam_s[i] * sv[i] * inv_kin_visc;
-    it[i] = on && Rey[i] <= 1000.0;
-  }
-  for (int n = 1; n <= 20; n++) {
-    bool any = false;
-#pragma unroll
-    for (int i = 0; i < NS; i++) any = any || it[i];
-    if (!any) break;
-#pragma unroll
-    for (int i = 0; i < NS; i++) {
-      const double old = sv[i];
-      const double isc = sqrt(Rey[i] / (24.0 * (1.0 + 0.15 * det_pow(Rey[i], 0.687))));
-      const double nsv = cp[i] * isc;
-      const bool conv = fabs(old - nsv) / old < 1.0e-6;
-      const double nrey = P.diam_s[i] * nsv * inv_kin_visc;
-      sv[i] = it[i] ? nsv : sv[i];
-      Rey[i] = it[i] ? nrey : Rey[i];
-      it[i] = it[i] && !conv;
-    }
-  }
-}
-
 // eval_source_bdry (src/constitutive_2d.f90:5024-5130), split into the
 // time factors (the same for every ring cell of a step) and the per-cell
 // part that scales them by the cell's direction vector
