@@ -43,7 +43,8 @@ HBM_PEAK_GBS = 8000.0
 #            (read+write, NS each) when erosion or settling is on; B_cent
 #            read+write when the topography changes; hmax, pdynmax,
 #            mod_vel_max, vulnerability word read+write
-#   mask:    h plane in, mask byte out (every grid cell)
+#   mask:    state-flag byte in (+ source byte with a radial source), mask
+#            byte out (every grid cell)
 #   topo:    B_cent in, B'x, B'y out (cells near the flow)
 
 
@@ -66,7 +67,7 @@ def kernel_bytes(p, name):
         b += 3 * 16 + 8
         return b
     if name == "mask":
-        return 9
+        return 2 + (1 if p.radial_source_flag else 0)
     if name == "topo":
         return 24
     return 0
