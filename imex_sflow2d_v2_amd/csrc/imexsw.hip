@@ -176,9 +176,9 @@ void launch_stencil(imexsw_ctx* c, int compute_dt, bool rows) {
   const int items = c->n_tiles * (kTileH / c->row_unit);
   if (c->P.ny > 1 && rows) {
     static int res = 0;
-    if (!res) res = resident_grid(k_stencil_rows<NS>, 192, c->n_cu, 1 << 30);
+    if (!res) res = resident_grid(k_stencil_rows<NS>, 384, c->n_cu, 1 << 30);
     hipLaunchKernelGGL((k_stencil_rows<NS>), dim3(std::min(c->n_tiles * kTileH, res)),
-                       dim3(192), 0, c->stream, c->P, c->D, compute_dt);
+                       dim3(384), 0, c->stream, c->P, c->D, compute_dt);
   } else if (c->P.ny > 1) {
     static int res = 0;
     if (!res) res = resident_grid(k_stencil<NS, true>, 64, c->n_cu, 1 << 30);
@@ -279,10 +279,19 @@ int run_mask(imexsw_ctx* c, int solve_all) {
   return check_launch(c, "k_mask");
 }
 
+// sparse: 0 every tile, 1 tiles next to a computed tile (flag test per
+// tile), 2 the same tiles found from the tile list (few computed tiles)
 int run_topo(imexsw_ctx* c, int sparse) {
   {
     Timed tm(c, K_TOPO);
-    hipLaunchKernelGGL(k_topo, c->grid_mask, dim3(256), 0, c->stream, c->P, c->D, sparse);
+    if (sparse == 2) {
+      static int res = 0;
+      if (!res) res = resident_grid(k_topo_list, 256, c->n_cu, 1 << 30);
+      hipLaunchKernelGGL(k_topo_list, dim3(std::min(9 * c->n_tiles, res)), dim3(256), 0,
+                         c->stream, c->P, c->D);
+    } else {
+      hipLaunchKernelGGL(k_topo, c->grid_mask, dim3(256), 0, c->stream, c->P, c->D, sparse);
+    }
     if (c->D.gcoef)
       hipLaunchKernelGGL(k_gstag, c->grid_stag, dim3(256), 0, c->stream, c->P, c->D);
   }
@@ -302,7 +311,7 @@ int enqueue_step(imexsw_ctx* c, bool commit_prev) {
   if (run_mask(c, 0)) return -1;
   if (run_stencil(c, c->P.cfl != -1.0, c->sparse)) return -1;
   if (run_local(c, OP_RK | OP_EROSION | OP_POST | OP_POLICY, K_LOCAL, c->sparse)) return -1;
-  if (c->P.topo && run_topo(c, 1)) return -1;
+  if (c->P.topo && run_topo(c, c->sparse ? 2 : 1)) return -1;
   return check_launch(c, "k_step_begin");
 }
 int end_steps(imexsw_ctx* c) {

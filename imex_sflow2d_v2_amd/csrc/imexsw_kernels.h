@@ -616,22 +616,34 @@ __device__ __forceinline__ void load_row(const DevParams& P, const DevPtrs& D,
 // the two sides of the y face between rows k-1 (B, owner Sb) and k (T,
 // owner St), with the ring overrides of src/solver_2d.f90:3879-3889
 template <int NS>
-__device__ __forceinline__ void yface_sides(const DevParams& P, const double* hib,
-                                            const double* lot, const SrcInfo& Sb,
-                                            const SrcInfo& St, int k, double g,
-                                            Side<NS>& Bs, Side<NS>& Ts) {
+__device__ __forceinline__ void yface_bottom(const DevParams& P, const double* hib,
+                                             const double* lot, const SrcInfo& Sb,
+                                             const SrcInfo& St, int k, double g, Side<NS>& Bs) {
   if (Sb.is_src) {
     if (P.radial && St.src2 && St.sS && k < P.ny - 1) make_side<NS, 2>(P, lot, g, Bs);
     else zero_side<NS>(Bs);
   } else {
     make_side<NS, 2>(P, hib, g, Bs);
   }
+}
+template <int NS>
+__device__ __forceinline__ void yface_top(const DevParams& P, const double* hib,
+                                          const double* lot, const SrcInfo& Sb,
+                                          const SrcInfo& St, int k, double g, Side<NS>& Ts) {
   if (St.is_src) {
     if (P.radial && Sb.src2 && !Sb.sS && Sb.sN && k - 1 > 0) make_side<NS, 2>(P, hib, g, Ts);
     else zero_side<NS>(Ts);
   } else {
     make_side<NS, 2>(P, lot, g, Ts);
   }
+}
+template <int NS>
+__device__ __forceinline__ void yface_sides(const DevParams& P, const double* hib,
+                                            const double* lot, const SrcInfo& Sb,
+                                            const SrcInfo& St, int k, double g,
+                                            Side<NS>& Bs, Side<NS>& Ts) {
+  yface_bottom<NS>(P, hib, lot, Sb, St, k, g, Bs);
+  yface_top<NS>(P, hib, lot, Sb, St, k, g, Ts);
 }
 
 // Per-cell data of row k that the x pass and the explicit terms read, from
@@ -661,6 +673,12 @@ __device__ __forceinline__ double load_gsy(const DevParams& P, const DevPtrs& D,
   if (!D.gsy) return 1.0;
   return D.gsy[cidx(P, min(max(j, 0), P.nx - 1), min(max(k, 0), P.ny))];
 }
+
+template <int NS>
+__device__ __forceinline__ void x_face_fluxes(const DevParams& P, const SrcInfo& S, int j,
+                                              const Side<NS>& Wself, const Side<NS>& Eself,
+                                              const Side<NS>& Eleft, double* HW, double* HE,
+                                              double& amaxW, double& amaxE);
 
 // x direction of row k: the W face flux on every lane, the E face by
 // shuffle from the right neighbour (a boundary face on the last column).
@@ -700,6 +718,19 @@ __device__ __forceinline__ void x_fluxes(const DevParams& P, const SrcInfo& S, c
   }
   Side<NS> Eleft;
   shfl_side<NS>(Eself, Eleft, (lane + 63) & 63);
+  x_face_fluxes<NS>(P, S, j, Wself, Eself, Eleft, HW, HE, amaxW, amaxE);
+}
+
+// The x face fluxes of row k from its W and E sides (Eleft: the E side of
+// the left neighbour): the W face on every lane, the E face by shuffle from
+// the right neighbour (a boundary face on the last column).
+template <int NS>
+__device__ __forceinline__ void x_face_fluxes(const DevParams& P, const SrcInfo& S, int j,
+                                              const Side<NS>& Wself, const Side<NS>& Eself,
+                                              const Side<NS>& Eleft, double* HW, double* HE,
+                                              double& amaxW, double& amaxE) {
+  constexpr int NV = 4 + NS;
+  const int lane = threadIdx.x & 63;
   const int flags = (S.is_src ? 1 : 0) | (S.src2 ? 2 : 0) | (S.sE ? 4 : 0) | (S.sW ? 8 : 0);
   const int flags_left = __shfl(flags, (lane + 63) & 63, kWave);
   {
@@ -748,11 +779,17 @@ __device__ __forceinline__ void cell_update(const DevParams& P, const DevPtrs& D
                                             const double* HW, const double* HE,
                                             const double* HS, const double* HN, double amaxW,
                                             double amaxE, double amaxS, double amaxN,
-                                            int compute_dt, double& dtmin) {
+                                            int compute_dt, double& dtmin,
+                                            const double* ex_in = nullptr) {
   constexpr int NV = 4 + NS;
   const size_t pl = plane(P);
   double ex[NV];
-  expl_terms<NS>(P, R.bx, R.by, R.bxx, R.bxy, R.byy, R.gc, R.dg, qk, ex);
+  if (ex_in) {
+#pragma unroll
+    for (int i = 0; i < NV; i++) ex[i] = ex_in[i];
+  } else {
+    expl_terms<NS>(P, R.bx, R.by, R.bxx, R.bxy, R.byy, R.gc, R.dg, qk, ex);
+  }
 #pragma unroll
   for (int i = 0; i < NV; i++) {
     double dv = 0.0;
@@ -1019,109 +1056,48 @@ __device__ __forceinline__ void stencil_strip(const DevParams& P, const DevPtrs&
 // ---------------------------------------------------------------------------
 // k_stencil_rows: the stencil pass over the listed rows when few cells are
 // computed (the pass is then bound by the latency of one row, not by
-// bandwidth).  Three wavefronts share a listed row: wavefront 0 computes
-// the fluxes through the row's S faces, wavefront 1 through its N faces,
-// wavefront 2 the x faces and then, with the y fluxes from LDS, the cell
-// updates.  Each face is computed from the same operands, in the same
-// order, as in the strip march.
+// bandwidth).  Six wavefronts share a listed row k and hand their results on
+// through LDS in four phases:
+//   A  y reconstructions of rows k-1 (w0), k (w1), k+1 (w2), the x
+//      reconstruction of row k (w3), row k's explicit terms (w4)
+//   B  the sides of the S face (w0 bottom, w1 top), of the N face (w2, w3)
+//      and of the x faces (w4 W, w5 E)
+//   C  the S (w0), N (w2) and x (w4) face fluxes
+//   D  the cell updates and the CFL time step (w4)
+// Each face is computed from the same operands, in the same order, as in
+// the strip march.
 // ---------------------------------------------------------------------------
-// flux through the S face of row k (rows k-2 .. k+1 read)
 template <int NS>
-__device__ __forceinline__ double yflux_S(const DevParams& P, const DevPtrs& D,
-                                          const SrcTime& T, int j, int k, double* H) {
-  constexpr int NV = 4 + NS;
-  const bool injx = (j >= 0 && j < P.nx);
-  double qmm[NV + 2], qm[NV + 2], qc[NV + 2], qn[NV + 2];
-  int bm, bc;
-  double vm0, vm1, vc0, vc1, g;
-  {
-    double r0[NV], r1[NV], r2[NV], r3[NV];
-    load_raw<NS>(P, D, j, k - 2, r0);
-    load_raw<NS>(P, D, j, k - 1, r1);
-    load_raw<NS>(P, D, j, k, r2);
-    load_raw<NS>(P, D, j, k + 1, r3);
-    load_src(P, D, j, k - 1, bm, vm0, vm1);
-    load_src(P, D, j, k, bc, vc0, vc1);
-    g = load_gsy(P, D, j, k);
-    finish_row<NS>(P, j, k - 2, r0, qmm);
-    finish_row<NS>(P, j, k - 1, r1, qm);
-    finish_row<NS>(P, j, k, r2, qc);
-    finish_row<NS>(P, j, k + 1, r3, qn);
+__device__ __forceinline__ void side_store(double (*buf)[64], const Side<NS>& s, int lane) {
+#pragma unroll
+  for (int i = 0; i < 4 + NS; i++) {
+    buf[i][lane] = s.f[i];
+    buf[4 + NS + i][lane] = s.qc[i];
   }
-  Side<NS> Bs, Ts;
-  if (injx) {
-    const SrcInfo S = src_info(bc);
-    double sb[NV + 2], lo[NV + 2], hi[NV + 2];
-    ring_state<NS>(P, S, T, vc0, vc1, sb);
-    recon<NS, 2>(P, qc, qm, qn, k, P.ny, j, S, sb, false, lo, hi);
-    if (k == 0) {
-      // boundary face: both sides carry the S value of row 0
-      if (S.is_src) zero_side<NS>(Ts); else make_side<NS, 2>(P, lo, g, Ts);
-      Bs = Ts;
-    } else {
-      const SrcInfo Sm = src_info(bm);
-      double sbm[NV + 2], lom[NV + 2], him[NV + 2];
-      ring_state<NS>(P, Sm, T, vm0, vm1, sbm);
-      recon<NS, 2>(P, qm, qmm, qc, k - 1, P.ny, j, Sm, sbm, false, lom, him);
-      yface_sides<NS>(P, him, lo, Sm, S, k, g, Bs, Ts);
-    }
-  } else {
-    zero_side<NS>(Bs);
-    zero_side<NS>(Ts);
-  }
-  return face_flux<NS, 2>(Bs, Ts, H);
+  buf[8 + 2 * NS][lane] = s.vmin;
+  buf[9 + 2 * NS][lane] = s.vmax;
+  buf[10 + 2 * NS][lane] = s.zv;
 }
-
-// flux through the N face of row k (rows k-1 .. k+2 read)
 template <int NS>
-__device__ __forceinline__ double yflux_N(const DevParams& P, const DevPtrs& D,
-                                          const SrcTime& T, int j, int k, double* H) {
-  constexpr int NV = 4 + NS;
-  const bool injx = (j >= 0 && j < P.nx);
-  double qm[NV + 2], qc[NV + 2], qn[NV + 2], qnn[NV + 2];
-  int bc, bn;
-  double vc0, vc1, vn0, vn1, g;
-  {
-    double r1[NV], r2[NV], r3[NV], r4[NV];
-    load_raw<NS>(P, D, j, k - 1, r1);
-    load_raw<NS>(P, D, j, k, r2);
-    load_raw<NS>(P, D, j, k + 1, r3);
-    load_raw<NS>(P, D, j, k + 2, r4);
-    load_src(P, D, j, k, bc, vc0, vc1);
-    load_src(P, D, j, k + 1, bn, vn0, vn1);
-    g = load_gsy(P, D, j, k + 1);  // k + 1 == ny: the N boundary
-    finish_row<NS>(P, j, k - 1, r1, qm);
-    finish_row<NS>(P, j, k, r2, qc);
-    finish_row<NS>(P, j, k + 1, r3, qn);
-    finish_row<NS>(P, j, k + 2, r4, qnn);
+__device__ __forceinline__ void side_load(double (*buf)[64], Side<NS>& s, int lane) {
+#pragma unroll
+  for (int i = 0; i < 4 + NS; i++) {
+    s.f[i] = buf[i][lane];
+    s.qc[i] = buf[4 + NS + i][lane];
   }
-  Side<NS> Bs, Ts;
-  if (injx) {
-    const SrcInfo S = src_info(bc);
-    double sb[NV + 2], lo[NV + 2], hiy[NV + 2];
-    ring_state<NS>(P, S, T, vc0, vc1, sb);
-    recon<NS, 2>(P, qc, qm, qn, k, P.ny, j, S, sb, false, lo, hiy);
-    if (k == P.ny - 1) {
-      if (S.is_src) zero_side<NS>(Bs); else make_side<NS, 2>(P, hiy, g, Bs);
-      Ts = Bs;
-    } else {
-      const SrcInfo Sn = src_info(bn);
-      double sbn[NV + 2], lon[NV + 2], hin[NV + 2];
-      ring_state<NS>(P, Sn, T, vn0, vn1, sbn);
-      recon<NS, 2>(P, qn, qc, qnn, k + 1, P.ny, j, Sn, sbn, false, lon, hin);
-      yface_sides<NS>(P, hiy, lon, S, Sn, k + 1, g, Bs, Ts);
-    }
-  } else {
-    zero_side<NS>(Bs);
-    zero_side<NS>(Ts);
-  }
-  return face_flux<NS, 2>(Bs, Ts, H);
+  s.vmin = buf[8 + 2 * NS][lane];
+  s.vmax = buf[9 + 2 * NS][lane];
+  s.zv = buf[10 + 2 * NS][lane];
 }
 
 template <int NS>
-__global__ void __launch_bounds__(192) k_stencil_rows(DevParams P, DevPtrs D, int compute_dt) {
+__global__ void __launch_bounds__(384) k_stencil_rows(DevParams P, DevPtrs D, int compute_dt) {
   constexpr int NV = 4 + NS;
-  __shared__ double yS[NV + 1][64], yN[NV + 1][64];  // fluxes, then the wave speed
+  constexpr int NQ = NV + 2;       // reconstructed values per side
+  constexpr int NSD = 2 * NV + 3;  // doubles per face side
+  // phase A values (then the y fluxes of phase C), face sides
+  __shared__ double V[6][NQ][64];
+  __shared__ double SD[5][NSD][64];  // S bottom, S top, N bottom, N top, E of x
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   // the step's scalars, list size and first row in one round of loads (the
   // grid is never larger than the row list, so row_list[blockIdx.x] exists)
@@ -1140,48 +1116,163 @@ __global__ void __launch_bounds__(192) k_stencil_rows(DevParams P, DevPtrs D, in
     const int tile = row / kTileH;
     const int k = (tile / tiles_x) * kTileH + row % kTileH;
     const int j = (tile % tiles_x) * kStripW - 2 + lane;
-    const bool own = (lane >= 2 && lane < 2 + kStripW && j < P.nx);
-    double qk[NV + 2], HW[NV], HE[NV], amaxW = 0.0, amaxE = 0.0;
+    const bool injx = (j >= 0 && j < P.nx);
+    const bool own = (lane >= 2 && lane < 2 + kStripW && injx);
+    // row k's source data, the neighbour rows' (k-1 for the S face, k+1
+    // for the N face), and row k's per-cell data
+    int bc, bo;
+    double vc0, vc1, vo0, vo1;
+    load_src(P, D, j, k, bc, vc0, vc1);
+    load_src(P, D, j, (wv < 2) ? k - 1 : k + 1, bo, vo0, vo1);
     RowData R;
-    if (wv < 2) {
-      double H[NV];
-      const double a = (wv == 0) ? yflux_S<NS>(P, D, T, j, k, H) : yflux_N<NS>(P, D, T, j, k, H);
-      double(*y)[64] = (wv == 0) ? yS : yN;
-#pragma unroll
-      for (int i = 0; i < NV; i++) y[i][lane] = H[i];
-      y[NV][lane] = a;
-    } else {
-      const bool injx = (j >= 0 && j < P.nx);
-      double r[NV], vc0, vc1, sb[NV + 2];
-      int bc;
-      load_raw<NS>(P, D, j, k, r);
-      load_src(P, D, j, k, bc, vc0, vc1);
-      load_rowdata(P, D, j, k, R);
-      finish_row<NS>(P, j, k, r, qk);
-      SrcInfo S{};
-      if (injx) {
-        S = src_info(bc);
-        ring_state<NS>(P, S, T, vc0, vc1, sb);
+    load_rowdata(P, D, j, k, R);
+    const double g = load_gsy(P, D, j, (wv < 2) ? k : k + 1);
+    const SrcInfo S = injx ? src_info(bc) : SrcInfo{};
+    const SrcInfo So = injx ? src_info(bo) : SrcInfo{};
+    double qk[NV + 2], ex[NV];
+    // ---- phase A
+    if (wv <= 2) {
+      // y reconstruction of row kr = k - 1 + wv from rows kr-1, kr, kr+1
+      const int kr = k - 1 + wv;
+      double r0[NV], r1[NV], r2[NV], qa[NV + 2], qb[NV + 2], qc[NV + 2];
+      load_raw<NS>(P, D, j, kr - 1, r0);
+      load_raw<NS>(P, D, j, kr, r1);
+      load_raw<NS>(P, D, j, kr + 1, r2);
+      finish_row<NS>(P, j, kr - 1, r0, qa);
+      finish_row<NS>(P, j, kr, r1, qb);
+      finish_row<NS>(P, j, kr + 1, r2, qc);
+      const bool need = injx && (wv == 1 || (wv == 0 && k > 0) || (wv == 2 && k < P.ny - 1));
+      double lo[NV + 2], hi[NV + 2];
+      if (need) {
+        const SrcInfo& Sr = (wv == 1) ? S : So;
+        double sb[NV + 2];
+        ring_state<NS>(P, Sr, T, (wv == 1) ? vc0 : vo0, (wv == 1) ? vc1 : vo1, sb);
+        recon<NS, 2>(P, qb, qa, qc, kr, P.ny, j, Sr, sb, false, lo, hi);
       } else {
 #pragma unroll
-        for (int i = 0; i < NV + 2; i++) sb[i] = 0.0;
+        for (int i = 0; i < NV + 2; i++) lo[i] = hi[i] = 0.0;
       }
-      x_fluxes<NS, true>(P, S, sb, qk, j, k, R, HW, HE, amaxW, amaxE);
+      if (wv == 0) {
+#pragma unroll
+        for (int i = 0; i < NQ; i++) V[0][i][lane] = hi[i];  // hib of the S face
+      } else if (wv == 1) {
+#pragma unroll
+        for (int i = 0; i < NQ; i++) {
+          V[1][i][lane] = lo[i];  // lot of the S face
+          V[2][i][lane] = hi[i];  // hib of the N face
+        }
+      } else {
+#pragma unroll
+        for (int i = 0; i < NQ; i++) V[3][i][lane] = lo[i];  // lot of the N face
+      }
+    } else {
+      double r[NV];
+      load_raw<NS>(P, D, j, k, r);
+      finish_row<NS>(P, j, k, r, qk);
+      if (wv == 3) {
+        double sb[NV + 2], qkm[NV + 2], qkp[NV + 2], lo[NV + 2], hi[NV + 2];
+#pragma unroll
+        for (int i = 0; i < NV + 2; i++) {
+          qkm[i] = shfl_d(qk[i], (lane + 63) & 63);
+          qkp[i] = shfl_d(qk[i], (lane + 1) & 63);
+        }
+        const bool typo = injx && P.radial && S.src2 && k > 0 && k < P.ny - 1 && !S.sS && S.sN;
+        if (injx) {
+          ring_state<NS>(P, S, T, vc0, vc1, sb);
+          recon<NS, 1>(P, qk, qkm, qkp, j, P.nx, k, S, sb, typo, lo, hi);
+        } else {
+#pragma unroll
+          for (int i = 0; i < NV + 2; i++) lo[i] = hi[i] = 0.0;
+        }
+#pragma unroll
+        for (int i = 0; i < NQ; i++) {
+          V[4][i][lane] = lo[i];
+          V[5][i][lane] = hi[i];
+        }
+      } else if (wv == 4 && own && (R.m & M_COMPUTE)) {
+        expl_terms<NS>(P, R.bx, R.by, R.bxx, R.bxy, R.byy, R.gc, R.dg, qk, ex);
+      }
     }
-    __syncthreads();  // y fluxes in LDS
-    if (wv == 2 && own && (R.m & M_COMPUTE)) {
+    __syncthreads();
+    // ---- phase B
+    Side<NS> Wself;
+    {
+      double va[NQ], vb[NQ];
+      const int ia = (wv < 2) ? 0 : (wv < 4) ? 2 : 4 + (wv - 4);
+#pragma unroll
+      for (int i = 0; i < NQ; i++) va[i] = V[ia][i][lane];
+      if (wv < 4) {
+#pragma unroll
+        for (int i = 0; i < NQ; i++) vb[i] = V[ia + 1][i][lane];
+      }
+      Side<NS> sd;
+      if (wv < 2) {
+        // S face of row k: bottom = row k-1 (So), top = row k (S)
+        if (!injx) {
+          zero_side<NS>(sd);
+        } else if (k == 0) {
+          // boundary face: both sides carry the S value of row 0
+          if (S.is_src) zero_side<NS>(sd); else make_side<NS, 2>(P, vb, g, sd);
+        } else if (wv == 0) {
+          yface_bottom<NS>(P, va, vb, So, S, k, g, sd);
+        } else {
+          yface_top<NS>(P, va, vb, So, S, k, g, sd);
+        }
+        side_store<NS>(SD[wv], sd, lane);
+      } else if (wv < 4) {
+        // N face of row k: bottom = row k (S), top = row k+1 (So)
+        if (!injx) {
+          zero_side<NS>(sd);
+        } else if (k == P.ny - 1) {
+          if (S.is_src) zero_side<NS>(sd); else make_side<NS, 2>(P, va, g, sd);
+        } else if (wv == 2) {
+          yface_bottom<NS>(P, va, vb, S, So, k + 1, g, sd);
+        } else {
+          yface_top<NS>(P, va, vb, S, So, k + 1, g, sd);
+        }
+        side_store<NS>(SD[wv], sd, lane);
+      } else {
+        // x faces: W side (w4, kept), E side (w5, to LDS)
+        if (!injx || S.is_src) zero_side<NS>(sd);
+        else make_side<NS, 1>(P, va, (wv == 4) ? R.gW : R.gE, sd);
+        if (wv == 4) Wself = sd;
+        else side_store<NS>(SD[4], sd, lane);
+      }
+    }
+    __syncthreads();
+    // ---- phase C
+    double HW[NV], HE[NV], amaxW = 0.0, amaxE = 0.0;
+    if (wv == 0 || wv == 2) {
+      Side<NS> Bs, Ts;
+      side_load<NS>(SD[wv], Bs, lane);
+      side_load<NS>(SD[wv + 1], Ts, lane);
+      double H[NV];
+      const double a = face_flux<NS, 2>(Bs, Ts, H);
+      const int o = (wv == 0) ? 0 : 1;
+#pragma unroll
+      for (int i = 0; i < NV; i++) V[o][i][lane] = H[i];
+      V[o][NV][lane] = a;
+    } else if (wv == 4) {
+      Side<NS> Eself, Eleft;
+      side_load<NS>(SD[4], Eself, lane);
+      side_load<NS>(SD[4], Eleft, (lane + 63) & 63);
+      x_face_fluxes<NS>(P, S, j, Wself, Eself, Eleft, HW, HE, amaxW, amaxE);
+    }
+    __syncthreads();
+    // ---- phase D
+    if (wv == 4 && own && (R.m & M_COMPUTE)) {
       double HS[NV], HN[NV];
 #pragma unroll
       for (int i = 0; i < NV; i++) {
-        HS[i] = yS[i][lane];
-        HN[i] = yN[i][lane];
+        HS[i] = V[0][i][lane];
+        HN[i] = V[1][i][lane];
       }
       cell_update<NS, true>(P, D, cidx(P, j, k), qk, R, HW, HE, HS, HN, amaxW, amaxE,
-                            yS[NV][lane], yN[NV][lane], compute_dt, dtmin);
+                            V[0][NV][lane], V[1][NV][lane], compute_dt, dtmin, ex);
     }
     __syncthreads();  // LDS free for the next row
   }
-  if (wv == 2 && compute_dt) dt_reduce(D, P, dtmin);
+  if (wv == 4 && compute_dt) dt_reduce(D, P, dtmin);
 }
 
 // ---------------------------------------------------------------------------
@@ -1787,6 +1878,41 @@ __device__ __forceinline__ void topo_cell(const DevParams& P, const DevPtrs& D, 
     const double g = 1.0 / (1.0 + bx * bx + by * by);
     D.gcoef[c] = g;
     D.dgdx[c] = -(2.0 * (g * g) * (bx * bxx + by * bxy));
+  }
+}
+
+// The same recomputation driven by this step's tile list (few computed
+// tiles): item (listed tile t, neighbour d of 9) recomputes tile t + d.
+// Every tile of the 3x3 neighbourhoods is done once: a listed tile by
+// itself, any other by the smallest listed tile next to it.  Persistent
+// grid, four wavefronts per workgroup as in topo_tile.
+__global__ void __launch_bounds__(256) k_topo_list(DevParams P, DevPtrs D) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int tiles_x = (P.nx + kStripW - 1) / kStripW;
+  const int tiles_y = (P.ny + kTileH - 1) / kTileH;
+  const int n = (int)*ctr(D, C_TILES, 0);
+  for (int item = blockIdx.x; item < 9 * n; item += gridDim.x) {
+    const int t = D.tile_list[item / 9], dd = item % 9;
+    const int tx = t % tiles_x + dd % 3 - 1, ty = t / tiles_x + dd / 3 - 1;
+    if (tx < 0 || tx >= tiles_x || ty < 0 || ty >= tiles_y) continue;
+    const int nt = ty * tiles_x + tx;
+    if (nt != t) {
+      if (D.tile_flag[nt]) continue;  // listed: done by itself
+      int first = t;
+      for (int dy = -1; dy <= 1; dy++)
+        for (int dx = -1; dx <= 1; dx++) {
+          const int x = tx + dx, y = ty + dy;
+          if (x >= 0 && x < tiles_x && y >= 0 && y < tiles_y && D.tile_flag[y * tiles_x + x])
+            first = min(first, y * tiles_x + x);
+        }
+      if (first != t) continue;
+    }
+    const int j = tx * kStripW - 2 + lane;
+    if (lane < 2 || lane >= 2 + kStripW || j >= P.nx) continue;
+    for (int rr = w; rr < kTileH; rr += 4) {
+      const int k = ty * kTileH + rr;
+      if (k < P.ny) topo_cell(P, D, j, k);
+    }
   }
 }
 
