@@ -200,10 +200,10 @@ void launch_local(imexsw_ctx* c, int ops, bool fused) {
   // work items: listed tile rows, four per workgroup
   const int items = (c->n_tiles * kTileH + 3) / 4;
   if (fused) {
-    // half rows, four per workgroup
+    // row parts, four per workgroup
     static int res3 = 0;
     if (!res3) res3 = resident_grid(k_local<NS, 3>, 256, c->n_cu, 1 << 30);
-    hipLaunchKernelGGL((k_local<NS, 3>), dim3(std::min(2 * items, res3)), dim3(256), 0,
+    hipLaunchKernelGGL((k_local<NS, 3>), dim3(std::min(kSparseLanes * items, res3)), dim3(256), 0,
                        c->stream, c->P, c->D, ops & ~(int)OP_SUMS);
     return;
   }

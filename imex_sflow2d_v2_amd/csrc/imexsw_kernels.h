@@ -29,6 +29,9 @@ constexpr int kTileX = 64, kTileY = 4;  // layout-conversion kernels
 // cells, lanes 0, 1, 62, 63 its two-cell halo), so k_mask, k_stencil,
 // k_local and k_topo all see a tile the same way.
 constexpr int kTileH = 16;
+// lanes per cell in the one-pass local kernel of sparse steps (a divisor of
+// kStripW and of the wavefront)
+constexpr int kSparseLanes = 4;
 
 // ---------------------------------------------------------------------------
 // pointers of one context (device addresses)
@@ -1379,8 +1382,8 @@ enum LocalOps { OP_RK = 1, OP_EROSION = 2, OP_POST = 4, OP_SUMS = 8, OP_WARM = 1
 // row's stage update).  Separate kernels keep the plain stage update's
 // register footprint free of the learned part, and on a wet grid the stage
 // update's free of the erosion part.
-// (PH 3 runs each cell on a lane pair: both lanes compute, `counts` is set
-// on one of them)
+// (PH 3 runs each cell on kSparseLanes lanes: every lane computes, `counts`
+// is set on one of them)
 template <int NS, int PH>
 __device__ __forceinline__ void local_cell(const DevParams& P, const DevPtrs& D, size_t c,
                                            double dt, int ops, unsigned long long* e,
@@ -1471,7 +1474,8 @@ __device__ __forceinline__ void local_cell(const DevParams& P, const DevPtrs& D,
     int nonconv = 0, ls = 0, wu = -1;
     float hv[kHarvestWidth];
     bool hok = false;
-    int it = stage2<NS, PH == 3>(P, q0, Dr, dt, bx, by, bxx, bxy, byy, gc, q, nonconv, ls,
+    int it = stage2<NS, (PH == 3) ? kSparseLanes : 1>(P, q0, Dr, dt, bx, by, bxx, bxy, byy, gc,
+                                                       q, nonconv, ls,
                         (PH == 2 && (ops & OP_WARM)) ? wg : nullptr, &wu,
                         (PH == 2 && (ops & OP_HARVEST)) ? hv : nullptr, &hok);
     if (counts && wu == 1) atomicAdd(&e[C_WARM_USED - C_NEWTON_CELLS], 1ull);
@@ -1549,7 +1553,7 @@ __device__ __forceinline__ void local_cell(const DevParams& P, const DevPtrs& D,
   if (ero) {
     phys_or_dry<NS>(P, q, qp);
     double er[NSX], dep[NSX], eq[NV], topo;
-    mass_exchange<NS, PH == 3>(P, qp, bx, by, erod, dt, er, dep, eq, topo);
+    mass_exchange<NS, (PH == 3) ? kSparseLanes : 1>(P, qp, bx, by, erod, dt, er, dep, eq, topo);
 #pragma unroll
     for (int i = 0; i < NV; i++) q[i] = q[i] + dt * eq[i];
 #pragma unroll
@@ -1652,21 +1656,24 @@ __device__ __forceinline__ void local_row(const DevParams& P, const DevPtrs& D, 
   }
 }
 
-// Half a listed tile row (30 cells) for PH 3, one cell per lane pair (lanes
-// 2i and 2i+1 take cell i; lanes 60..63 idle).  No per-row sums.
+// One part of a listed tile row for PH 3: the row's 60 cells in
+// kSparseLanes parts of 60 / kSparseLanes, cell i of the part on lanes
+// kSparseLanes*i .. kSparseLanes*i + kSparseLanes-1 (lanes 60..63 idle).  No
+// per-row sums.
 template <int NS>
-__device__ __forceinline__ void local_half_row(const DevParams& P, const DevPtrs& D, int row,
-                                               int half, double dt, int ops,
+__device__ __forceinline__ void local_part_row(const DevParams& P, const DevPtrs& D, int row,
+                                               int part, double dt, int ops,
                                                unsigned long long* e, int& imax) {
-  const int lane = threadIdx.x & 63, i = lane >> 1;
+  constexpr int kPart = kStripW / kSparseLanes;
+  const int lane = threadIdx.x & 63, i = lane / kSparseLanes;
   const int tiles_x = (P.nx + kStripW - 1) / kStripW;
   const int tile = row / kTileH;
-  const int j = (tile % tiles_x) * kStripW + half * (kStripW / 2) + i;
+  const int j = (tile % tiles_x) * kStripW + part * kPart + i;
   const int k = (tile / tiles_x) * kTileH + row % kTileH;
   double s_mass = 0.0, s_vol = 0.0, s_area = 0.0, s_sol = 0.0;
-  if (i < kStripW / 2 && j < P.nx && k < P.ny)
+  if (i < kPart && j < P.nx && k < P.ny)
     local_cell<NS, 3>(P, D, cidx(P, j, k), dt, ops & ~(int)OP_SUMS, e, imax, s_mass, s_vol,
-                      s_area, s_sol, (lane & 1) == 0);
+                      s_area, s_sol, (lane % kSparseLanes) == 0);
 }
 
 // Learned Newton starting points for this step: for every computed cell of
@@ -1755,7 +1762,7 @@ __global__ void __launch_bounds__(64) k_mlp_guess(DevParams P, DevPtrs D, MlpDev
   }
 }
 
-// One wavefront per listed tile row (per half row for PH 3); the four
+// One wavefront per listed tile row (per row part for PH 3); the four
 // wavefronts of a workgroup take rows in turn until the list is exhausted.
 template <int NS, int PH>
 __global__ void __launch_bounds__(256) k_local(DevParams P, DevPtrs D, int ops) {
@@ -1776,11 +1783,12 @@ __global__ void __launch_bounds__(256) k_local(DevParams P, DevPtrs D, int ops) 
     dt = D.scal[S_DT];
   }
   const int nrows = (int)*ctr(D, C_ROWS, 0);
-  // PH 3: two items (half rows) per listed row
-  const int items = (PH == 3) ? 2 * nrows : nrows;
+  // PH 3: kSparseLanes items (row parts) per listed row
+  const int items = (PH == 3) ? kSparseLanes * nrows : nrows;
   for (int it = blockIdx.x * 4 + w; live && it < items; it += gridDim.x * 4) {
     if (PH == 3)
-      local_half_row<NS>(P, D, D.row_list[it >> 1], it & 1, dt, ops, cnt[w], imax);
+      local_part_row<NS>(P, D, D.row_list[it / kSparseLanes], it % kSparseLanes, dt, ops,
+                         cnt[w], imax);
     else
       local_row<NS, PH>(P, D, D.row_list[it], dt, ops, cnt[w], imax);
   }

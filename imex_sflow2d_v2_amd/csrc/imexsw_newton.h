@@ -8,7 +8,24 @@
 
 namespace imexsw {
 
-// eval_f (src/solver_2d.f90:2243-2282)
+// eval_f (src/solver_2d.f90:2243-2282) from the implicit terms' real parts
+// nh1, nh2 at qj
+template <int NS>
+__device__ __forceinline__ void finish_f(const double* qj, const double* qold,
+                                         const double* Rj, double a_diag, double dt,
+                                         double nh1, double nh2, double* f, double& scal) {
+  constexpr int NV = 4 + NS;
+  double s = 0.0;
+#pragma unroll
+  for (int i = 0; i < NV; i++) {
+    double nh = (i == 1) ? nh1 : (i == 2) ? nh2 : 0.0;
+    double R = Rj[i] - a_diag * nh;
+    f[i] = 1.0 * (qj[i] - qold[i] + dt * R);
+  }
+#pragma unroll
+  for (int i = 0; i < NV; i++) s += f[i] * f[i];
+  scal = 0.5 * s;
+}
 template <int NS>
 __device__ __forceinline__ void eval_f(const DevParams& P, const double* qj,
                                        const double* qold, const double* Rj,
@@ -20,16 +37,7 @@ __device__ __forceinline__ void eval_f(const DevParams& P, const double* qj,
   for (int i = 0; i < NV; i++) cq[i] = mk(qj[i], 0.0);
   cpx n1, n2;
   implicit_c<NS>(P, cq, Bx, By, n1, n2);
-  double s = 0.0;
-#pragma unroll
-  for (int i = 0; i < NV; i++) {
-    double nh = (i == 1) ? n1.re : (i == 2) ? n2.re : 0.0;
-    double R = Rj[i] - a_diag * nh;
-    f[i] = 1.0 * (qj[i] - qold[i] + dt * R);
-  }
-#pragma unroll
-  for (int i = 0; i < NV; i++) s += f[i] * f[i];
-  scal = 0.5 * s;
+  finish_f<NS>(qj, qold, Rj, a_diag, dt, n1.re, n2.re, f, scal);
 }
 
 // gfortran NORM2 (scaled sum of squares)
@@ -53,14 +61,6 @@ __device__ __forceinline__ double norm2_(const double* x) {
   return scale * sqrt(res);
 }
 
-// the value of the other lane of this lane's pair (lanes 2i, 2i+1)
-__device__ __forceinline__ double pair_other(double v) {
-  int lo = __double2loint(v), hi = __double2hiint(v);
-  lo = __shfl_xor(lo, 1, 64);
-  hi = __shfl_xor(hi, 1, 64);
-  return __hiloint2double(hi, lo);
-}
-
 struct NewtonOut {
   int iters;
   int conv;
@@ -72,10 +72,7 @@ struct NewtonOut {
 // src/constitutive_2d.f90:478-480): the Jacobian has the 2x2 momentum block
 // plus the diagonal scaling of the other unknowns, which is what the
 // reference's packed partitioned solve reduces to.
-// PAIR: two lanes (2i, 2i+1) solve the same cell with the same operands;
-// each evaluates one of the two complex-step Jacobian columns and takes the
-// other from its partner, so an iteration waits for one column, not two.
-template <int NS, bool PAIR = false>
+template <int NS>
 __device__ __forceinline__ NewtonOut solve_rk_step(const DevParams& P, double* qj,
                                                   const double* qold,
                                                   const double* Rj, double a_diag,
@@ -118,23 +115,6 @@ __device__ __forceinline__ NewtonOut solve_rk_step(const DevParams& P, double* q
     // --- Jacobian: complex step on the two momentum columns
     const double dta = dt * a_diag;
     double J1[3], J2[3];  // d nh_{1,2} / d rel_{1}, d nh_{1,2} / d rel_{2}
-    if (PAIR) {
-      const int col = 1 + (threadIdx.x & 1);
-      cpx cq[NV];
-#pragma unroll
-      for (int m = 0; m < NV; m++) cq[m] = mk(rel[m], 0.0);
-      cq[col] = mk(rel[col], P.h_cs);
-#pragma unroll
-      for (int m = 0; m < NV; m++) cq[m] = cmulr(cq[m], org[m]);
-      cpx n1, n2;
-      implicit_c<NS>(P, cq, Bx, By, n1, n2);
-      const double a = 1.0 * n1.im * P.one_by_h, b = 1.0 * n2.im * P.one_by_h;
-      const double ao = pair_other(a), bo = pair_other(b);
-      J1[1] = (col == 1) ? a : ao;
-      J1[2] = (col == 1) ? b : bo;
-      J2[1] = (col == 2) ? a : ao;
-      J2[2] = (col == 2) ? b : bo;
-    } else {
     {
       cpx cq[NV];
 #pragma unroll
@@ -158,7 +138,6 @@ __device__ __forceinline__ NewtonOut solve_rk_step(const DevParams& P, double* q
       implicit_c<NS>(P, cq, Bx, By, n1, n2);
       J2[1] = 1.0 * n1.im * P.one_by_h;
       J2[2] = 1.0 * n2.im * P.one_by_h;
-    }
     }
     // left_matrix entries (column 1 and 2 of the momentum block); rows of
     // the other equations of those columns are 0 - dta*(1.0*0*one_by_h) = 0
@@ -303,6 +282,240 @@ __device__ __forceinline__ NewtonOut solve_rk_step(const DevParams& P, double* q
   return out;
 }
 
+// solve_rk_step on four lanes (4i .. 4i+3, the same cell and operands on
+// each).  One implicit-term evaluation per lane and round: lanes 0 and 1 of
+// the quad the two complex-step Jacobian columns, lane 2 the residual.  A
+// full Newton step (the first iteration) and the line search's first trial
+// (alam = 1) evaluate the residual at the trial point and, at the same time,
+// the Jacobian there, which the next iteration uses when the trial is
+// accepted: the same operands the plain solver would give its Jacobian
+// evaluation, so the same bits, one evaluation earlier.  Every value the
+// control flow depends on is taken from one lane of the quad, so the four
+// lanes stay together.
+template <int NS>
+__device__ __forceinline__ double quad_from(double v, int role) {
+  const int src = (threadIdx.x & 60) | role;
+  int lo = __double2loint(v), hi = __double2hiint(v);
+  lo = __shfl(lo, src, 64);
+  hi = __shfl(hi, src, 64);
+  return __hiloint2double(hi, lo);
+}
+
+// One quad round at the point q = rel * org: J1/J2 (columns 1, 2) and the
+// residual rt, scal (when `want_f`; else lanes 2, 3 duplicate column 1)
+template <int NS>
+__device__ __forceinline__ void quad_round(const DevParams& P, const double* q, const double* org,
+                                           const double* qold, const double* Rj, double a_diag,
+                                           double dt, double Bx, double By, bool want_f,
+                                           double* J1, double* J2, double* rt, double& scal) {
+  constexpr int NV = 4 + NS;
+  const int role = threadIdx.x & 3;
+  const int pcol = (role == 1) ? 2 : (role == 0 || !want_f) ? 1 : -1;  // -1: none
+  cpx cq[NV];
+#pragma unroll
+  for (int m = 0; m < NV; m++) cq[m] = mk(q[m], (m == pcol) ? P.h_cs * org[m] : 0.0);
+  cpx n1, n2;
+  implicit_c<NS>(P, cq, Bx, By, n1, n2);
+  const double a = 1.0 * n1.im * P.one_by_h, b = 1.0 * n2.im * P.one_by_h;
+  J1[1] = quad_from<NS>(a, 0);
+  J1[2] = quad_from<NS>(b, 0);
+  J2[1] = quad_from<NS>(a, 1);
+  J2[2] = quad_from<NS>(b, 1);
+  if (want_f) {
+    const double nh1 = quad_from<NS>(n1.re, 2), nh2 = quad_from<NS>(n2.re, 2);
+    finish_f<NS>(q, qold, Rj, a_diag, dt, nh1, nh2, rt, scal);
+  }
+}
+
+template <int NS>
+__device__ __forceinline__ NewtonOut solve_rk_step_quad(const DevParams& P, double* qj,
+                                                       const double* qold, const double* Rj,
+                                                       double a_diag, double dt, double Bx,
+                                                       double By) {
+  constexpr int NV = 4 + NS;
+  const double TOLF = 1.0e-10;
+  const double TOLX = kEps;
+  double org[NV], rel[NV], rt[NV], old[NV], desc[NV];
+  double scal, scal_old;
+#pragma unroll
+  for (int i = 0; i < NV; i++) org[i] = fmax(fabs(qj[i]), 1.0e-3);
+#pragma unroll
+  for (int i = 0; i < NV; i++) rel[i] = qj[i] / org[i];
+  NewtonOut out{0, 0, 0};
+  int it;
+  bool fresh = false;
+  bool jnext = false;       // J1n/J2n hold the Jacobian at the current point
+  double J1n[3], J2n[3];
+  for (it = 1; it <= 100; it++) {
+    if (!fresh) eval_f<NS>(P, qj, qold, Rj, a_diag, dt, Bx, By, rt, scal);
+    bool restored = false;
+    double mf = 0.0;
+#pragma unroll
+    for (int i = 0; i < NV; i++) mf = fmax(mf, fabs(rt[i]));
+    if (mf < TOLF) {
+      out.conv = 1;
+      break;
+    }
+    const double dta = dt * a_diag;
+    double J1[3], J2[3];
+    if (jnext) {
+      J1[1] = J1n[1];
+      J1[2] = J1n[2];
+      J2[1] = J2n[1];
+      J2[2] = J2n[2];
+    } else {
+      double q[NV], rdum[NV], sdum;
+#pragma unroll
+      for (int m = 0; m < NV; m++) q[m] = rel[m] * org[m];
+      quad_round<NS>(P, q, org, qold, Rj, a_diag, dt, Bx, By, false, J1, J2, rdum, sdum);
+    }
+    jnext = false;
+    const double zJ = 1.0 * 0.0 * P.one_by_h;
+    const double L11 = 1.0 * org[1] - dta * J1[1];
+    const double L21 = 0.0 - dta * J1[2];
+    const double L12 = 0.0 - dta * J2[1];
+    const double L22 = 1.0 * org[2] - dta * J2[2];
+    const double Lz = 0.0 - dta * zJ;
+    double d1[NV];
+#pragma unroll
+    for (int i = 0; i < NV; i++) d1[i] = (i == 1 || i == 2) ? 0.0 : rt[i] / (1.0 * org[i]);
+    double s = 0.0;
+#pragma unroll
+    for (int i = 0; i < NV; i++)
+      if (i != 1 && i != 2) s += d1[i] * 0.0;
+    double b1 = rt[1] - s, b2 = rt[2] - s;
+    {
+      double inv_det = 1.0 / (L11 * L22 - L21 * L12);
+      double x1 = (b1 * L22 - b2 * L12) * inv_det;
+      double x2 = (L11 * b2 - L21 * b1) * inv_det;
+#pragma unroll
+      for (int i = 0; i < NV; i++) {
+        if (i == 1) desc[i] = -x1 + 0.0;
+        else if (i == 2) desc[i] = -x2 + 0.0;
+        else desc[i] = 0.0 + -d1[i];
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < NV; i++) old[i] = rel[i];
+    scal_old = scal;
+    if (it > 1) {
+      double dd = 0.0;
+#pragma unroll
+      for (int i = 0; i < NV; i++) dd += rel[i] * rel[i];
+      double stpmax = 100.0 * fmax(sqrt(dd), (double)NV);
+      double grad[NV];
+#pragma unroll
+      for (int i = 0; i < NV; i++) {
+        double g = 0.0;
+#pragma unroll
+        for (int r = 0; r < NV; r++) {
+          double Lri;
+          if (i == 1) Lri = (r == 1) ? L11 : (r == 2) ? L21 : Lz;
+          else if (i == 2) Lri = (r == 1) ? L12 : (r == 2) ? L22 : Lz;
+          else Lri = (r == i) ? 1.0 * org[i] : 0.0;
+          g += rt[r] * Lri;
+        }
+        grad[i] = g;
+      }
+      double dabs = norm2_<NV>(desc);
+      if (dabs > stpmax) {
+#pragma unroll
+        for (int i = 0; i < NV; i++) desc[i] = desc[i] * stpmax / dabs;
+      }
+      double slope = 0.0;
+#pragma unroll
+      for (int i = 0; i < NV; i++) slope += grad[i] * desc[i];
+      double mx = -INFINITY;
+#pragma unroll
+      for (int i = 0; i < NV; i++) {
+        double v = fabs(desc[i]) / fmax(fabs(old[i]), 1.0);
+        if (v > mx) mx = v;
+      }
+      double alamin = TOLX / mx;
+      if (alamin == 0.0) {
+#pragma unroll
+        for (int i = 0; i < NV; i++) rel[i] = old[i];
+      } else {
+        double alam = 1.0, alam2 = 0.0, f2 = 0.0, tmplam = 0.0;
+        for (int ls = 0; ls < 256; ls++) {
+          double q[NV];
+#pragma unroll
+          for (int i = 0; i < NV; i++) rel[i] = old[i] + alam * desc[i];
+#pragma unroll
+          for (int i = 0; i < NV; i++) q[i] = rel[i] * org[i];
+          if (ls == 0) {
+            // the residual and, speculatively, the Jacobian at the full step
+            quad_round<NS>(P, q, org, qold, Rj, a_diag, dt, Bx, By, true, J1n, J2n, rt, scal);
+          } else {
+            eval_f<NS>(P, q, qold, Rj, a_diag, dt, Bx, By, rt, scal);
+          }
+          out.ls_evals++;
+          if (scal <= 0.9 * scal_old) {
+            jnext = (ls == 0);
+            break;
+          }
+          if (alam < alamin) {
+#pragma unroll
+            for (int i = 0; i < NV; i++) rel[i] = old[i];
+            scal = scal_old;
+            restored = true;
+            break;
+          }
+          if (alam == 1.0) {
+            tmplam = -(slope / (2.0 * (scal - scal_old - slope)));
+          } else {
+            double rhs1 = scal - scal_old - alam * slope;
+            double rhs2 = f2 - scal_old - alam2 * slope;
+            double a = (rhs1 / (alam * alam) - rhs2 / (alam2 * alam2)) / (alam - alam2);
+            double b = (-(alam2 * rhs1 / (alam * alam)) + alam * rhs2 / (alam2 * alam2)) /
+                       (alam - alam2);
+            if (a == 0.0) {
+              tmplam = -(slope / (2.0 * b));
+            } else {
+              double disc = b * b - 3.0 * a * slope;
+              if (disc < 0.0) tmplam = 0.5 * alam;
+              else if (b <= 0.0) tmplam = (-b + sqrt(disc)) / (3.0 * a);
+              else tmplam = -(slope / (b + sqrt(disc)));
+            }
+            if (tmplam > 0.5 * alam) tmplam = 0.5 * alam;
+          }
+          alam2 = alam;
+          f2 = scal;
+          alam = fmax(tmplam, 0.5 * alam);
+        }
+      }
+    } else {
+#pragma unroll
+      for (int i = 0; i < NV; i++) rel[i] = old[i] + desc[i];
+#pragma unroll
+      for (int i = 0; i < NV; i++) qj[i] = rel[i] * org[i];
+      // the residual and, speculatively, the Jacobian at the full step
+      quad_round<NS>(P, qj, org, qold, Rj, a_diag, dt, Bx, By, true, J1n, J2n, rt, scal);
+      jnext = true;
+    }
+#pragma unroll
+    for (int i = 0; i < NV; i++) qj[i] = rel[i] * org[i];
+    fresh = !restored;
+    if (restored) jnext = false;
+    mf = 0.0;
+#pragma unroll
+    for (int i = 0; i < NV; i++) mf = fmax(mf, fabs(rt[i]));
+    if (mf < TOLF) {
+      out.conv = 1;
+      break;
+    }
+    double mx = 0.0;
+#pragma unroll
+    for (int i = 0; i < NV; i++) mx = fmax(mx, fabs(rel[i] - old[i]) / fmax(fabs(rel[i]), 1.0));
+    if (mx < TOLX) {
+      out.conv = 1;
+      break;
+    }
+  }
+  out.iters = it > 100 ? 100 : it;
+  return out;
+}
+
 // Raw warm-start features of one cell entering Newton: log h, the
 // predictor's velocity (u, v), T, three solid volume fractions, log dt and
 // the bed slope magnitude.
@@ -364,7 +577,8 @@ __device__ __forceinline__ bool predictor(const DevParams& P, const double* q0,
 // implicit term refresh of :1234-1236 is not evaluated: for the stiffly
 // accurate N_RK = 2 tableau it is overwritten at :1271-1277 and never enters
 // q.  Returns the Newton iteration count (0: no Newton).
-template <int NS, bool PAIR = false>
+// LANES: lanes per cell, 1 or 4 (the quad solver)
+template <int NS, int LANES = 1>
 __device__ __forceinline__ int stage2(const DevParams& P, const double* q0,
                                       const double* D, double dt, double Bx,
                                       double By, double Bxx, double Bxy,
@@ -406,10 +620,11 @@ __device__ __forceinline__ int stage2(const DevParams& P, const double* q0,
 #pragma unroll
       for (int i = 0; i < NV; i++) q[i] = qw[i];
     }
-    no = solve_rk_step<NS, PAIR>(P, q, q0, Rj, 1.0, dt, Bx, By, use ? rtw : rtp, use ? sw : sp);
+    no = solve_rk_step<NS>(P, q, q0, Rj, 1.0, dt, Bx, By, use ? rtw : rtp, use ? sw : sp);
   } else {
     if (warm_used) *warm_used = -1;
-    no = solve_rk_step<NS, PAIR>(P, q, q0, Rj, 1.0, dt, Bx, By);
+    no = (LANES == 4) ? solve_rk_step_quad<NS>(P, q, q0, Rj, 1.0, dt, Bx, By)
+                      : solve_rk_step<NS>(P, q, q0, Rj, 1.0, dt, Bx, By);
   }
   nonconv = !no.conv;
   ls_evals = no.ls_evals;
@@ -450,12 +665,11 @@ __device__ __forceinline__ int stage2(const DevParams& P, const double* q0,
 
 // eval_mass_exchange_terms, supported branches (src/constitutive_2d.f90:
 // 4424-5003).  Returns false when every term is zero.
-// PAIR: the cell runs on a lane pair (lanes 2i, 2i+1 with the same
-// operands); the solid classes' settling velocities are split between the
-// two lanes (class 2s on the even lane, 2s+1 on the odd one, the same
-// instructions on both) and exchanged, so the pair waits for about half of
-// them.
-template <int NS, bool PAIR = false>
+// LANES > 1: the cell runs on LANES lanes (the same operands on each); the
+// solid classes' settling velocities are split between them (class
+// LANES*s + r on lane r, the same instructions on every lane) and
+// exchanged, so the cell waits for about 1/LANES of them.
+template <int NS, int LANES = 1>
 __device__ __forceinline__ void mass_exchange(const DevParams& P, const double* qp,
                                               double Bx, double By,
                                               const double* erodible, double dt,
@@ -502,30 +716,42 @@ __device__ __forceinline__ void mass_exchange(const DevParams& P, const double* 
     // hindered settling factor, the same for every class
     const double hind = det_pow(1.0 - fmin(1.0, at / alpha_max), hind_exp);
     double svs[NS > 0 ? NS : 1];
-    if (PAIR) {
-      const int p = threadIdx.x & 1;
-      double mine[(NS + 1) / 2 > 0 ? (NS + 1) / 2 : 1];
+    if (LANES > 1) {
+      constexpr int NSL = (NS + LANES - 1) / LANES;
+      const int p = threadIdx.x & (LANES - 1);
+      double mine[NSL > 0 ? NSL : 1];
 #pragma unroll
-      for (int sl = 0; sl < (NS + 1) / 2; sl++) {
-        const int c0 = 2 * sl, c1 = (2 * sl + 1 < NS) ? 2 * sl + 1 : 2 * sl;
-        const bool has = (p == 0) || (2 * sl + 1 < NS);
-        const double a = p ? al[c1] : al[c0];
-        const double diam = p ? P.diam_s[c1] : P.diam_s[c0];
-        const double rhos = p ? P.rho_s[c1] : P.rho_s[c0];
+      for (int sl = 0; sl < NSL; sl++) {
+        double a = 0.0, diam = 0.0, rhos = 0.0;
+        bool has = false;
+#pragma unroll
+        for (int r = 0; r < LANES; r++) {
+          const int c = sl * LANES + r;
+          if (c < NS && p == r) {
+            a = al[c];
+            diam = P.diam_s[c];
+            rhos = P.rho_s[c];
+            has = true;
+          }
+        }
         mine[sl] = (has && a > 0.0) ? settling_velocity(P, diam, rhos, m.rho_c, inv_kin_visc)
                                     : 0.0;
       }
 #pragma unroll
       for (int i = 0; i < NS; i++) {
-        const double o = pair_other(mine[i >> 1]);
-        svs[i] = ((i & 1) == p) ? mine[i >> 1] : o;
+        const int src = (threadIdx.x & (64 - LANES)) | (i % LANES);
+        int lo = __double2loint(mine[i / LANES]), hi = __double2hiint(mine[i / LANES]);
+        lo = __shfl(lo, src, 64);
+        hi = __shfl(hi, src, 64);
+        svs[i] = __hiloint2double(hi, lo);
       }
     }
 #pragma unroll
     for (int i = 0; i < NS; i++) {
       if (al[i] > 0.0) {
-        const double sv =
-            PAIR ? svs[i] : settling_velocity(P, P.diam_s[i], P.rho_s[i], m.rho_c, inv_kin_visc);
+        const double sv = (LANES > 1) ? svs[i]
+                                      : settling_velocity(P, P.diam_s[i], P.rho_s[i], m.rho_c,
+                                                          inv_kin_visc);
         double d = al[i] * sv * 1.0;
         d = d * hind;
         dep[i] = fmin(d, r_h * al[i] / dt);
