@@ -436,52 +436,89 @@ __device__ __forceinline__ NewtonOut solve_rk_step_quad(const DevParams& P, doub
 #pragma unroll
         for (int i = 0; i < NV; i++) rel[i] = old[i];
       } else {
-        double alam = 1.0, alam2 = 0.0, f2 = 0.0, tmplam = 0.0;
-        for (int ls = 0; ls < 256; ls++) {
+        // lnsrch's trials: 1, then a2 = max(quadratic model, 0.5), then
+        // halvings (the cubic model is capped at half the last step, so
+        // max(tmplam, 0.5 * alam) is always 0.5 * alam).  Lane r of a round
+        // evaluates one trial; the trials are then taken in order with the
+        // serial rules, so the accepted point and the count are the serial
+        // ones.  Round 1: the Jacobian at trial 1 (lanes 0, 1), trial 1
+        // (lane 2), and the trial 0.5 that follows whenever a2 = 0.5 (lane 3).
+        const int role = threadIdx.x & 3;
+        double n1a, n2a;  // this lane's implicit terms (real parts)
+        {
+          const double al = (role == 3) ? 0.5 : 1.0;
+          const int pcol = (role == 0) ? 1 : (role == 1) ? 2 : -1;
+          cpx cq[NV];
+#pragma unroll
+          for (int m = 0; m < NV; m++) {
+            const double rm = old[m] + al * desc[m];
+            cq[m] = mk(rm * org[m], (m == pcol) ? P.h_cs * org[m] : 0.0);
+          }
+          cpx n1, n2;
+          implicit_c<NS>(P, cq, Bx, By, n1, n2);
+          const double a = 1.0 * n1.im * P.one_by_h, b = 1.0 * n2.im * P.one_by_h;
+          J1n[1] = quad_from<NS>(a, 0);
+          J1n[2] = quad_from<NS>(b, 0);
+          J2n[1] = quad_from<NS>(a, 1);
+          J2n[2] = quad_from<NS>(b, 1);
+          n1a = n1.re;
+          n2a = n2.re;
+        }
+        // trial `alam` from the implicit terms of lane `src`; returns true
+        // when the search ends with it (accepted, or below alamin)
+        auto take = [&](double alam, int src, bool& accepted) -> bool {
+          const double h1 = quad_from<NS>(n1a, src), h2 = quad_from<NS>(n2a, src);
           double q[NV];
 #pragma unroll
           for (int i = 0; i < NV; i++) rel[i] = old[i] + alam * desc[i];
 #pragma unroll
           for (int i = 0; i < NV; i++) q[i] = rel[i] * org[i];
-          if (ls == 0) {
-            // the residual and, speculatively, the Jacobian at the full step
-            quad_round<NS>(P, q, org, qold, Rj, a_diag, dt, Bx, By, true, J1n, J2n, rt, scal);
-          } else {
-            eval_f<NS>(P, q, qold, Rj, a_diag, dt, Bx, By, rt, scal);
-          }
+          finish_f<NS>(q, qold, Rj, a_diag, dt, h1, h2, rt, scal);
           out.ls_evals++;
-          if (scal <= 0.9 * scal_old) {
-            jnext = (ls == 0);
-            break;
-          }
+          accepted = scal <= 0.9 * scal_old;
+          if (accepted) return true;
           if (alam < alamin) {
 #pragma unroll
             for (int i = 0; i < NV; i++) rel[i] = old[i];
             scal = scal_old;
             restored = true;
-            break;
+            return true;
           }
-          if (alam == 1.0) {
-            tmplam = -(slope / (2.0 * (scal - scal_old - slope)));
-          } else {
-            double rhs1 = scal - scal_old - alam * slope;
-            double rhs2 = f2 - scal_old - alam2 * slope;
-            double a = (rhs1 / (alam * alam) - rhs2 / (alam2 * alam2)) / (alam - alam2);
-            double b = (-(alam2 * rhs1 / (alam * alam)) + alam * rhs2 / (alam2 * alam2)) /
-                       (alam - alam2);
-            if (a == 0.0) {
-              tmplam = -(slope / (2.0 * b));
-            } else {
-              double disc = b * b - 3.0 * a * slope;
-              if (disc < 0.0) tmplam = 0.5 * alam;
-              else if (b <= 0.0) tmplam = (-b + sqrt(disc)) / (3.0 * a);
-              else tmplam = -(slope / (b + sqrt(disc)));
+          return false;
+        };
+        bool acc = false;
+        bool end = take(1.0, 2, acc);
+        jnext = acc;
+        if (!end) {
+          const double tq = -(slope / (2.0 * (scal - scal_old - slope)));
+          double alam = fmax(tq, 0.5 * 1.0);
+          int trials = 1;
+          if (alam == 0.5) {
+            end = take(0.5, 3, acc);
+            alam = 0.25;
+            trials++;
+          }
+          while (!end && trials < 256) {
+            // four halvings at once: lane r evaluates alam * 0.5^r
+            double al = alam;
+#pragma unroll
+            for (int r = 0; r < 3; r++)
+              if (role > r) al = al * 0.5;
+            {
+              cpx cq[NV];
+#pragma unroll
+              for (int m = 0; m < NV; m++) cq[m] = mk((old[m] + al * desc[m]) * org[m], 0.0);
+              cpx n1, n2;
+              implicit_c<NS>(P, cq, Bx, By, n1, n2);
+              n1a = n1.re;
+              n2a = n2.re;
             }
-            if (tmplam > 0.5 * alam) tmplam = 0.5 * alam;
+            for (int r = 0; r < 4 && !end && trials < 256; r++) {
+              end = take(alam, r, acc);
+              alam = alam * 0.5;
+              trials++;
+            }
           }
-          alam2 = alam;
-          f2 = scal;
-          alam = fmax(tmplam, 0.5 * alam);
         }
       }
     } else {
