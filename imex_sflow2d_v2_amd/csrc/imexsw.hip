@@ -269,12 +269,13 @@ int run_local(imexsw_ctx* c, int ops, int kid, bool fused = false) {
   return check_launch(c, "k_local");
 }
 
-int run_mask(imexsw_ctx* c, int solve_all) {
+// partial: rescan only the dirty tiles (the step loop)
+int run_mask(imexsw_ctx* c, int solve_all, int partial = 0) {
   {
     Timed tm(c, K_MASK);
     const dim3 grid(c->grid_mask.x, (c->grid_mask.y + kMaskGroup - 1) / kMaskGroup);
     hipLaunchKernelGGL(k_mask, grid, dim3(64 * kMaskGroup), 0, c->stream, c->P, c->D,
-                       solve_all);
+                       solve_all, partial);
   }
   return check_launch(c, "k_mask");
 }
@@ -308,7 +309,7 @@ int enqueue_step(imexsw_ctx* c, bool commit_prev) {
     hipLaunchKernelGGL(k_step_begin, dim3(1), dim3(1), 0, c->stream, c->P, c->D,
                        commit_prev ? 1 : 0, 1);
   }
-  if (run_mask(c, 0)) return -1;
+  if (run_mask(c, 0, 1)) return -1;
   if (run_stencil(c, c->P.cfl != -1.0, c->sparse)) return -1;
   if (run_local(c, OP_RK | OP_EROSION | OP_POST | OP_POLICY, K_LOCAL, c->sparse)) return -1;
   if (c->P.topo && run_topo(c, c->sparse ? 2 : 1)) return -1;
@@ -564,6 +565,8 @@ int imexsw_create(const imexsw_params* params, imexsw_ctx** out) {
   c->n_tiles = (int)(c->grid_mask.x * c->grid_mask.y);
   rc |= dalloc(c, &D.tile_sums, 4 * (size_t)c->n_tiles * kTileH);
   rc |= dalloc(c, &D.tile_flag, (size_t)c->n_tiles);
+  rc |= dalloc(c, &D.tile_dirty, (size_t)c->n_tiles);
+  rc |= dalloc(c, &D.tile_nact, (size_t)c->n_tiles);
   rc |= dalloc(c, &D.tile_list, (size_t)c->n_tiles);
   rc |= dalloc(c, &D.row_list, (size_t)c->n_tiles * kTileH);
   c->strip_h = (P.ny > 1) ? kTileH : 1;
@@ -580,7 +583,9 @@ int imexsw_create(const imexsw_params* params, imexsw_ctx** out) {
     imexsw_destroy(c);
     return -1;
   }
-  if (hipStreamSynchronize(c->stream) != hipSuccess) {
+  // no tile has been scanned yet: the first mask scans all
+  if (hipMemsetAsync(D.tile_dirty, 1, (size_t)c->n_tiles, c->stream) != hipSuccess ||
+      hipStreamSynchronize(c->stream) != hipSuccess) {
     fail(nullptr, "stream synchronise failed after allocation");
     imexsw_destroy(c);
     return -1;
@@ -693,6 +698,8 @@ int imexsw_set_source(imexsw_ctx* c, const int32_t* source_cell, const uint8_t* 
   }
   if (upload_plane(c, info.data(), const_cast<uint8_t*>(D.sinfo), P.nx, P.ny, P.pitch)) return -1;
   if (upload_aos(c, ring.data(), 2, const_cast<double*>(D.sring))) return -1;
+  HIPCHK(c, hipMemsetAsync(D.tile_dirty, 1, (size_t)c->n_tiles, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
   c->d_valid = false;
   return 0;
 }
@@ -702,6 +709,8 @@ int imexsw_upload_q(imexsw_ctx* c, const double* q) {
   if (upload_aos(c, q, c->nv, c->D.q)) return -1;
   hipLaunchKernelGGL(k_sflag, c->grid_tiles, dim3(256), 0, c->stream, c->P, c->D);
   if (check_launch(c, "k_sflag")) return -1;
+  // every tile's neighbourhood may have changed: the next mask scans all
+  HIPCHK(c, hipMemsetAsync(c->D.tile_dirty, 1, (size_t)c->n_tiles, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   c->d_valid = false;
   return 0;
@@ -725,13 +734,14 @@ int imexsw_check_solve(imexsw_ctx* c, int solve_all, double t, int32_t* solve_ce
   if (!c) return fail(c, "null argument");
   if (set_scalar(c, S_T, t)) return -1;
   if (set_scalar(c, S_DONE, 0.0)) return -1;
-  HIPCHK(c, hipMemsetAsync(c->D.counters + C_SOLVE_CELLS * kSlots, 0,
-                           kSlots * sizeof(unsigned long long), c->stream));
   HIPCHK(c, hipMemsetAsync(c->D.counters + C_COMPUTE_CELLS * kSlots, 0,
                            kSlots * sizeof(unsigned long long), c->stream));
   HIPCHK(c, hipMemsetAsync(c->D.counters + C_TILES * kSlots, 0,
                            2 * kSlots * sizeof(unsigned long long), c->stream));  // tiles, rows
   if (run_mask(c, solve_all)) return -1;
+  // the mask now reflects this call's solve_all: the next step's mask
+  // scans every tile again
+  HIPCHK(c, hipMemsetAsync(c->D.tile_dirty, 1, (size_t)c->n_tiles, c->stream));
   c->d_valid = false;
   if (solve_cells) {
     unsigned long long v[kSlots], n = 0;

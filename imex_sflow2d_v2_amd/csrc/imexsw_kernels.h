@@ -66,6 +66,8 @@ struct DevPtrs {
   unsigned long long* counters;  // see CounterSlot
   double* tile_sums;  // 4 per work tile
   uint8_t* tile_flag; // per work tile: any cell to compute
+  uint8_t* tile_dirty;  // per work tile: a state within reach may have changed
+  int* tile_nact;       // per work tile: active cells at its last mask scan
   int* tile_list;     // work tiles with a cell to compute (compacted)
   int* row_list;      // tile rows (tile * kTileH + row) with a cell to compute
   double* guess;      // 2 planes: learned Newton start for the momenta (NaN: none)
@@ -197,10 +199,17 @@ __device__ __forceinline__ unsigned lane_read(unsigned v, int src) {
 
 // Writes the tile's mask bytes, counters and tile flag; returns the
 // (wavefront-uniform) bit word of the tile rows with a cell to compute.
+// With `partial`, a tile that is not dirty keeps its mask bytes and counts
+// (nothing within reach changed state since its last scan, so neither can
+// have changed, and it has no cell to compute).  The active-cell counter is
+// a running total kept by per-tile differences.
 __device__ __forceinline__ unsigned mask_tile(const DevParams& P, const DevPtrs& D, int tx,
-                                              int tyy, int tiles_x, int solve_all) {
+                                              int tyy, int tiles_x, int solve_all,
+                                              int partial) {
   constexpr int LH = kTileH + 4;
   const int lane = threadIdx.x & 63;
+  const int tile = tyy * tiles_x + tx;
+  if (partial && !D.tile_dirty[tile]) return 0u;
   const int j = tx * kStripW - 2 + lane;
   const int k0 = tyy * kTileH;
   const bool injx = (j >= 0 && j < P.nx);
@@ -281,9 +290,11 @@ __device__ __forceinline__ unsigned mask_tile(const DevParams& P, const DevPtrs&
     ccount += __shfl_xor(ccount, off, 64);
     rows_m |= (unsigned)__shfl_xor((int)rows_m, off, 64);
   }
-  const int tile = tyy * tiles_x + tx;
   if (lane == 0) {
-    if (count) atomicAdd(ctr(D, C_SOLVE_CELLS, tile), (unsigned long long)count);
+    const int delta = count - D.tile_nact[tile];
+    if (delta) atomicAdd(ctr(D, C_SOLVE_CELLS, tile), (unsigned long long)(long long)delta);
+    D.tile_nact[tile] = count;
+    D.tile_dirty[tile] = 0;
     if (ccount) atomicAdd(ctr(D, C_COMPUTE_CELLS, tile), (unsigned long long)ccount);
     D.tile_flag[tile] = rows_m ? 1 : 0;
   }
@@ -325,12 +336,12 @@ __device__ __forceinline__ void append_group(const ListSink& L, int tile, unsign
 
 // grid (tiles_x, ceil(tiles_y / kMaskGroup)), kMaskGroup wavefronts
 __global__ void __launch_bounds__(64 * kMaskGroup) k_mask(DevParams P, DevPtrs D,
-                                                          int solve_all) {
+                                                          int solve_all, int partial) {
   const int tiles_x = (P.nx + kStripW - 1) / kStripW;
   const int tiles_y = (P.ny + kTileH - 1) / kTileH;
   const int tyy = blockIdx.y * kMaskGroup + (threadIdx.x >> 6);
   unsigned rows_m = 0;
-  if (tyy < tiles_y) rows_m = mask_tile(P, D, blockIdx.x, tyy, tiles_x, solve_all);
+  if (tyy < tiles_y) rows_m = mask_tile(P, D, blockIdx.x, tyy, tiles_x, solve_all, partial);
   append_group<kMaskGroup>(own_sink(D), tyy * tiles_x + blockIdx.x, rows_m);
 }
 
@@ -1341,10 +1352,7 @@ __global__ void k_advance_time(DevPtrs D) { advance_member(D); }
 // CFL accumulator and per-step counters / work lists
 __device__ __forceinline__ void reset_member(const DevParams& P, const DevPtrs& D) {
   *D.dt_bits = (unsigned long long)__double_as_longlong(P.max_dt);
-  for (int i = 0; i < kSlots; i++) {
-    *ctr(D, C_SOLVE_CELLS, i) = 0;
-    *ctr(D, C_COMPUTE_CELLS, i) = 0;
-  }
+  for (int i = 0; i < kSlots; i++) *ctr(D, C_COMPUTE_CELLS, i) = 0;
   *ctr(D, C_TILES, 0) = 0;
   *ctr(D, C_ROWS, 0) = 0;
 }
@@ -1630,6 +1638,20 @@ __device__ __forceinline__ void local_cell(const DevParams& P, const DevPtrs& D,
   }
 }
 
+// The stage kernels change states only in the listed rows: their tiles and
+// the tiles next to them (the mask reaches two cells) are rescanned by the
+// next step's mask.
+__device__ __forceinline__ void mark_dirty(const DevParams& P, const DevPtrs& D, int tile) {
+  const int tiles_x = (P.nx + kStripW - 1) / kStripW;
+  const int tiles_y = (P.ny + kTileH - 1) / kTileH;
+  const int tx = tile % tiles_x, ty = tile / tiles_x;
+  for (int dy = -1; dy <= 1; dy++)
+    for (int dx = -1; dx <= 1; dx++) {
+      const int x = tx + dx, y = ty + dy;
+      if (x >= 0 && x < tiles_x && y >= 0 && y < tiles_y) D.tile_dirty[y * tiles_x + x] = 1;
+    }
+}
+
 // One tile row (60 cells, one per lane 2..61) of k_local; per-row partial
 // sums go to tile_sums[4 * row ..] when OP_SUMS.
 template <int NS, int PH>
@@ -1786,6 +1808,7 @@ __global__ void __launch_bounds__(256) k_local(DevParams P, DevPtrs D, int ops) 
   // PH 3: kSparseLanes items (row parts) per listed row
   const int items = (PH == 3) ? kSparseLanes * nrows : nrows;
   for (int it = blockIdx.x * 4 + w; live && it < items; it += gridDim.x * 4) {
+    if (lane == 0) mark_dirty(P, D, D.row_list[(PH == 3) ? it / kSparseLanes : it] / kTileH);
     if (PH == 3)
       local_part_row<NS>(P, D, D.row_list[it / kSparseLanes], it % kSparseLanes, dt, ops,
                          cnt[w], imax);

@@ -204,6 +204,25 @@ def test_sparse_and_dense_steps_bitwise_identical(curv):
     assert sa["line_search_evals"] == sb["line_search_evals"]
 
 
+def test_partial_mask_scans_match_full_scan():
+    """The step's mask rescans only tiles next to the previous step's
+    computed rows (a multi-tile grid with a spreading flow and a source):
+    its mask bytes, the per-step active counts summed over the run and the
+    state agree with the oracle, which scans every cell each step."""
+    d = decks.small_2d(n=130, ny=70, seed=11, erosion=True, source=True)
+    orc, dev = run_pair(d, 65)
+    check_state(orc, dev)
+    assert dev.stats()["cell_updates_active"] == orc.counters()["active_cell_updates"]
+    assert dev.stats()["solve_cells"] == int(np.count_nonzero(orc.mask() & 1))
+    # an explicit check_solve of the current state, then more steps
+    n = dev.check_solve(False, t=dev.t)
+    assert n == orc.check_solve(False)
+    n1 = orc.run(20, dt0=d.dt0)
+    n2 = dev.advance(20, dt0=d.dt0)
+    assert n1 == n2
+    check_state(orc, dev)
+
+
 def test_graph_replays_match_direct_launches():
     """imexsw_advance replays a captured graph of 20 steps when per-kernel
     timing is off; every replay, with host downloads and synchronisations in
