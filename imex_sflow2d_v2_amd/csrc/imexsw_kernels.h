@@ -339,6 +339,17 @@ __global__ void __launch_bounds__(64 * kMaskGroup) k_mask(DevParams P, DevPtrs D
                                                           int solve_all, int partial) {
   const int tiles_x = (P.nx + kStripW - 1) / kStripW;
   const int tiles_y = (P.ny + kTileH - 1) / kTileH;
+  if (partial) {
+    // a workgroup whose tiles are all clean has nothing to do (the same
+    // test on every wavefront, so none waits at a barrier)
+    bool any = false;
+#pragma unroll
+    for (int g = 0; g < kMaskGroup; g++) {
+      const int ty = blockIdx.y * kMaskGroup + g;
+      any = any || (ty < tiles_y && D.tile_dirty[ty * tiles_x + blockIdx.x]);
+    }
+    if (!any) return;
+  }
   const int tyy = blockIdx.y * kMaskGroup + (threadIdx.x >> 6);
   unsigned rows_m = 0;
   if (tyy < tiles_y) rows_m = mask_tile(P, D, blockIdx.x, tyy, tiles_x, solve_all, partial);
