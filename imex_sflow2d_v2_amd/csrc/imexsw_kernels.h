@@ -6,15 +6,20 @@
 // conservative state is NV such planes, the explicit residual D another NV.
 //
 // Kernels per time step (N_RK = 2):
-//   k_mask     check_solve: active mask (wet cells dilated N_RK times)
-//   k_stencil  one pass: reconstruction, wave speeds, KT fluxes, divergence,
-//              explicit terms -> D = divFlux - expl_terms, and the CFL dt
-//              (min over active cells, as an ordered-bits atomicMin)
-//   k_dtpolicy host-loop dt clipping / growth limiter on the device
-//   k_local    stage-2 Newton + assembly checks, erosion/deposition,
-//              post-step diagnostics, per-tile sums
-//   k_topo     topography_reconstruction of the slope fields (when the
-//              topography changes)
+//   k_step_begin  commit the previous step (host-loop dt policy, t += dt,
+//                 counters) and reset the step's counters
+//   k_mask        check_solve: active mask (wet cells dilated N_RK times)
+//                 and the step's work lists
+//   k_stencil, k_stencil_rows
+//                 reconstruction, wave speeds, KT fluxes, divergence,
+//                 explicit terms -> D = divFlux - expl_terms, and the CFL dt
+//                 (an ordered-bits atomicMin); a strip march on dense steps,
+//                 six wavefronts per listed row on sparse steps
+//   k_local       stage-2 Newton + assembly checks, erosion/deposition,
+//                 post-step diagnostics (dense: two kernels; sparse: one
+//                 pass, four lanes per cell)
+//   k_topo[_list] topography_reconstruction of the slope fields (when the
+//                 topography changes)
 #pragma once
 
 #include "imexsw_mlp.h"
@@ -1342,7 +1347,6 @@ __device__ __forceinline__ void dtpolicy_member(const DevParams& P, const DevPtr
   s[S_DT] = r.dt;
   s[S_T_OUT] = r.t_out;
 }
-__global__ void k_dtpolicy(DevParams P, DevPtrs D) { dtpolicy_member(P, D); }
 
 // t = t + dt, step count and active cell-updates
 __device__ __forceinline__ void advance_member(const DevPtrs& D) {
@@ -1358,7 +1362,6 @@ __device__ __forceinline__ void advance_member(const DevPtrs& D) {
   *ctr(D, C_ACTIVE_UPD, 0) += n;
   *ctr(D, C_COMPUTE_UPD, 0) += nc;
 }
-__global__ void k_advance_time(DevPtrs D) { advance_member(D); }
 
 // CFL accumulator and per-step counters / work lists
 __device__ __forceinline__ void reset_member(const DevParams& P, const DevPtrs& D) {
@@ -1367,7 +1370,6 @@ __device__ __forceinline__ void reset_member(const DevParams& P, const DevPtrs& 
   *ctr(D, C_TILES, 0) = 0;
   *ctr(D, C_ROWS, 0) = 0;
 }
-__global__ void k_reset_step(DevParams P, DevPtrs D) { reset_member(P, D); }
 
 // Between two steps of imexsw_advance: commit the previous step (its time
 // step policy, t = t + dt, counters) when `commit`, then reset the per-step

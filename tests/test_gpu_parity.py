@@ -226,18 +226,23 @@ def test_partial_mask_scans_match_full_scan():
 def test_graph_replays_match_direct_launches():
     """imexsw_advance replays a captured graph of 20 steps when per-kernel
     timing is off; every replay, with host downloads and synchronisations in
-    between, must give the bits of the directly launched steps."""
+    between, must give the bits of directly launched steps (runs shorter
+    than a graph, and timed runs)."""
     d = decks.small_2d(n=70, ny=50, seed=9, erosion=True, topo_change=True, source=True)
     g = Solver(d.params).load_deck(d)
     e = Solver(d.params).load_deck(d)
-    e.set_timing(True)  # direct launches
     for rep in range(4):
+        if rep == 2:
+            g.set_timing(True)  # direct launches with per-kernel events
         n1 = g.advance(20, dt0=d.dt0)
-        n2 = e.advance(20, dt0=d.dt0)
+        n2 = e.advance(10, dt0=d.dt0) + e.advance(10, dt0=d.dt0)  # direct launches
         assert n1 == n2 == 20
         assert g.t == e.t, (rep, g.t, e.t)
         assert np.array_equal(g.q(), e.q()), rep
         g.sync()
+    # the timed steps reported every kernel
+    kt = g.kernel_times()
+    assert kt["mask"][1] == 40 and kt["local"][1] == 40 and kt["mask"][0] > 0.0
     # a non-multiple of the graph length mixes replays and direct steps
     g.advance(47, dt0=d.dt0)
     e.advance(47, dt0=d.dt0)
