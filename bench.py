@@ -421,7 +421,7 @@ def main():
             "line_search_evals_per_cell": st["line_search_evals"] / max(1, st["newton_cells"]),
             "simulated_time_s": g.t,
             "kernel_ms_per_step": {k: v[0] / steps for k, v in kt.items() if v[1]},
-            "kernel_times_pass_ms_per_step": el_timed / steps,
+            "kernel_times_pass_ms_per_step": el_timed / steps * 1e3,
             "step_bytes_per_active_cell": step_bytes(p),
         },
         "roofline": {
