@@ -244,7 +244,10 @@ int imexsw_post_step(imexsw_ctx *ctx, double sums[4]);
  * and dt_old_old are carried in the context, initialised from dt0 by the
  * first call), with no host round trip per step.  t_io = current time
  * (updated), t_end and t_output as in the host loop.  Stops early when t
- * reaches t_end. */
+ * reaches t_end.  Steps run as replays of captured 20-step graphs (direct
+ * launches for a remainder, and for every step while per-kernel timing is
+ * on); each replay runs the form for few or for many computed cells, chosen
+ * from the computed-cell count of an earlier step (stats.sparse_steps). */
 int imexsw_advance(imexsw_ctx *ctx, int32_t nsteps, double dt0, double *t_io,
                    double t_end, double t_output, int32_t *steps_done);
 
@@ -270,9 +273,11 @@ int imexsw_stats_get(imexsw_ctx *ctx, imexsw_stats *stats, int reset);
 
 /* Per-kernel device time, measured with HIP events recorded on the
  * context's stream around every launch.  names: mask, stencil, local,
- * topo, dtpolicy, post, erosion.  ms[i] accumulates milliseconds, calls[i] launches.
- * Returns the number of kernels (<= cap).  Timing is enabled by
- * imexsw_set_timing(ctx, 1). */
+ * topo, dtpolicy (the kernel between steps that commits the time-step
+ * policy), post, erosion.  ms[i] accumulates milliseconds, calls[i]
+ * launches.  Returns the number of kernels (<= cap).  Timing is enabled by
+ * imexsw_set_timing(ctx, 1); while it is on, imexsw_advance launches every
+ * kernel directly (the events slow the step). */
 int imexsw_set_timing(imexsw_ctx *ctx, int enable);
 int imexsw_kernel_times(imexsw_ctx *ctx, double *ms, int64_t *calls, int cap);
 const char *imexsw_kernel_name(int index);
