@@ -384,8 +384,11 @@ def ensemble_members(count=256, seed=4, n_min=128, n_max=1024, dx=10.0,
 def small_2d(n=48, ny=None, dx=10.0, seed=0, energy=True, alpha=False,
              settling=True, erosion=False, source=False, topo_change=False,
              slope_correction=False, curvature=False, n_solid=2, rheology=True,
-             bc=None, limiter=1, n_rk=2, wet_floor=0.0, entrainment=True):
-    """Small seeded 2D decks for the parity tests (seconds on the oracle)."""
+             bc=None, limiter=1, n_rk=2, wet_floor=0.0, entrainment=True,
+             source_time=(1e9, 1e9, 0.0, 1e9)):
+    """Small seeded 2D decks for the parity tests (seconds on the oracle).
+    ``source_time`` is the radial source's time_param (period, duration,
+    ramp time, end time; src/constitutive_2d.f90:5041-5088)."""
     ny = n if ny is None else ny
     rng = np.random.default_rng(seed)
     rho = (2500.0, 2200.0, 2700.0)[:n_solid]
@@ -396,7 +399,7 @@ def small_2d(n=48, ny=None, dx=10.0, seed=0, energy=True, alpha=False,
     if source:
         src = dict(x=0.5 * L + 0.3 * dx, y=0.5 * ny * dx - 0.2 * dx, r=2.6 * dx,
                    h=2.0, vel=5.0, T=700.0, alphas=[0.02, 0.01, 0.01][:n_solid],
-                   time_param=(1e9, 1e9, 0.0, 1e9))
+                   time_param=tuple(source_time))
     p = make_params(n, ny, dx, n_solid, energy_flag=energy, alpha_flag=alpha,
                     cfl=0.24, max_dt=1.0, rho_s=rho, diam_s=dia, sp_heat_s=shs,
                     settling=settling, erosion_coeff=(1e-3 if erosion else 0.0),

@@ -96,6 +96,15 @@ def test_radial_source_parity():
     check_state(orc, dev)
 
 
+def test_radial_source_time_law_parity():
+    """A pulsed source (period 0.2 s, 0.15 s on with 0.04 s cosine ramps,
+    off after 0.6 s): the run crosses every branch of the time law."""
+    d = decks.small_2d(n=48, ny=44, seed=3, source=True, source_time=(0.2, 0.15, 0.04, 0.6))
+    orc, dev = run_pair(d, 60)
+    assert orc.t > 0.6
+    check_state(orc, dev)
+
+
 def test_slope_curvature_parity():
     d = decks.small_2d(n=40, ny=40, seed=4, slope_correction=True, curvature=True)
     orc, dev = run_pair(d, 20)

@@ -464,3 +464,147 @@ def test_erosion_rate_law_and_solid_budget():
     rate = p.erosion_coeff * math.hypot(u, v) * h * (1.0 - al.sum()) * (1.0 - p.erodible_porosity)
     expect = [min(rate * p.erodible_fract[i], er0[i, j, k] / dt) for i in range(2)]
     np.testing.assert_allclose((ers1 - ers0)[j, k] / dt, expect, rtol=1e-12)
+
+
+# --------------------------------------------------------------------------
+# radial source time law: the raised-cosine ramps in their sin^2 form, the
+# period, the end time, and the Richardson-number invariant that fixes the
+# exponents of t_coeff (src/constitutive_2d.f90:5041-5130)
+# --------------------------------------------------------------------------
+def _source_bdry(o, t, vx, vy):
+    out = np.zeros(o.nv + 2)
+    o.lib.orc_source_bdry(o.h, t, vx, vy, out.ctypes.data_as(o.lib.orc_source_bdry.argtypes[4]))
+    return out
+
+
+def _t_coeff_expected(t, tp):
+    period, dur, ramp, end = tp
+    if t >= end:
+        return None
+    r = math.fmod(t, period)
+    if ramp == 0.0:
+        return 1.0 if r <= dur else 0.0
+    if r < ramp:
+        return math.sin(0.5 * math.pi * r / ramp) ** 2
+    if r <= dur - ramp:
+        return 1.0
+    if r <= dur:
+        return math.sin(0.5 * math.pi * (dur - r) / ramp) ** 2
+    return 0.0
+
+
+@pytest.mark.parametrize("alpha", [False, True])
+@pytest.mark.parametrize("tp", [(10.0, 6.0, 1.5, 25.0), (10.0, 6.0, 0.0, 25.0)])
+def test_source_time_law_closed_form(alpha, tp):
+    ns = 2
+    src = dict(x=0.0, y=0.0, r=1.0, h=2.0, vel=5.0, T=700.0, alphas=[0.02, 0.03],
+               time_param=tp)
+    p = _p(ns, alpha_flag=alpha, radial_source=True, source=src)
+    o = Oracle(p)
+    nv = o.nv
+    vx, vy = 0.6, -0.8
+    ts = np.concatenate([np.linspace(0.0, 30.0, 3001), [1.5, 4.5, 6.0, 11.5, 14.5, 16.0, 25.0]])
+    seen = set()
+    for t in ts:
+        sb = _source_bdry(o, float(t), vx, vy)
+        c = _t_coeff_expected(float(t), tp)
+        assert sb[3] == 700.0
+        if c is None:
+            # after the end time only the temperature is prescribed
+            assert np.all(sb[[0, 1, 2, 4, 5, nv, nv + 1]] == 0.0), (t, sb)
+            seen.add("off")
+            continue
+        seen.add("ramp" if 0.0 < c < 1.0 else c)
+        # 0.5 (1 - cos) carries an absolute (not relative) error of a few
+        # ulps of 1 near the ramp ends
+        dc = 1e-15
+        assert abs(sb[0] - c * 2.0) <= 2.0 * dc, (t, sb[0], c)
+        # velocities scale with sqrt(t_coeff), momenta with t_coeff**1.5
+        np.testing.assert_allclose(sb[[nv, nv + 1]], math.sqrt(c) * 5.0 * np.array([vx, vy]),
+                                   rtol=1e-14, atol=5.0 * dc / (2.0 * math.sqrt(c) + 1e-300) + 1e-15)
+        np.testing.assert_allclose(sb[[1, 2]], sb[0] * sb[[nv, nv + 1]], rtol=1e-13, atol=1e-14)
+        if c > 0.0:
+            # Ri ~ h / |u|^2 does not depend on t_coeff
+            ri = sb[0] / (sb[nv] ** 2 + sb[nv + 1] ** 2)
+            assert math.isclose(ri, 2.0 / 25.0, rel_tol=1e-12), (t, ri)
+        al = np.array([0.02, 0.03])
+        np.testing.assert_allclose(sb[4:6], al if alpha else c * 2.0 * al, rtol=1e-14,
+                                   atol=2.0 * dc * al.max())
+    # the samples cover every branch of the law
+    assert {"off", 0.0, 1.0} <= seen and ("ramp" in seen) == (tp[2] > 0.0)
+    # period: the law repeats until the end time
+    for t in (0.25, 1.125, 5.875, 6.75):  # t + 10 is exact
+        assert np.array_equal(_source_bdry(o, t, vx, vy), _source_bdry(o, t + 10.0, vx, vy))
+
+
+# --------------------------------------------------------------------------
+# post-step diagnostics (src/IMEX_SfloW2D.f90:253-311, 395-440): running
+# maxima of thickness, dynamic pressure and speed, and the OR-accumulated
+# vulnerability table, rebuilt from per-step snapshots with the closed-form
+# mixture density
+# --------------------------------------------------------------------------
+def _pdyn_closed(p, qp, nv):
+    ns = p.n_solid
+    h = qp[0]
+    with np.errstate(invalid="ignore", divide="ignore"):
+        a = qp[4:4 + ns] if p.alpha_flag else qp[4:4 + ns] / np.where(h > 0, h, 1.0)
+        rho_c = p.pres / (p.sp_gas_const_a * qp[3])
+    rho_s = np.array([p.rho_s[i] for i in range(ns)])[:, None, None]
+    rho_m = (1.0 - a.sum(axis=0)) * rho_c + (a * rho_s).sum(axis=0)
+    return 0.5 * rho_m * (qp[nv] ** 2 + qp[nv + 1] ** 2)
+
+
+def test_post_step_diagnostics_from_snapshots():
+    d = decks.small_2d(n=40, ny=36, seed=14, source=True, wet_floor=5e-4)
+    p = d.params
+    probe = Oracle(p).load_deck(d)
+    nv = probe.nv
+    wet0 = probe.q()[0] > 0
+    pd0 = _pdyn_closed(p, probe.qp(), nv)[wet0]
+    # levels inside the data, so that all four (thickness, pressure) pairs
+    # give different tables and a transposed table index fails
+    thk = (0.5, 2.0)
+    dyn = tuple(float(x) for x in np.quantile(pd0[pd0 > 0], [0.3, 0.8]))
+    for i, v in enumerate(thk):
+        p.thickness_levels[i] = v
+    for i, v in enumerate(dyn):
+        p.dyn_pres_levels[i] = v
+    o = Oracle(p).load_deck(d)
+    shape = (d.params.nx, d.params.ny)
+    hmax = np.zeros(shape)
+    pdmax = np.zeros(shape)
+    mvmax = np.zeros(shape)
+    vuln = np.zeros((4,) + shape, dtype=bool)
+    for step in range(31):
+        if step:
+            o.run(1, dt0=d.dt0)
+        q, qp = o.q(), o.qp()
+        # only the cells the step computed (interior source cells are not)
+        wet = (q[0] > 0.0) & ((o.mask() & 1) > 0)
+        h = np.where(wet, qp[0], 0.0)
+        pd = np.where(wet, _pdyn_closed(p, qp, nv), 0.0)
+        mv = np.where(wet, np.sqrt(qp[nv] ** 2 + qp[nv + 1] ** 2), 0.0)
+        deep = wet & (qp[0] > 0.001)
+        if step == 0:
+            h0, pd_first = h, np.where(deep, pd, 0.0)
+        hmax = np.maximum(hmax, h)
+        pdmax = np.maximum(pdmax, np.where(deep, pd, 0.0))
+        mvmax = np.maximum(mvmax, np.where(deep, mv, 0.0))
+        for a in range(2):
+            for b in range(2):
+                vuln[2 * a + b] |= wet & (h >= thk[a]) & (pd >= dyn[b])
+    hm_o, pd_o, mv_o, vu_o = o.post()
+    assert np.array_equal(hm_o, hmax)
+    np.testing.assert_allclose(pd_o, pdmax, rtol=1e-12, atol=0.0)
+    np.testing.assert_allclose(mv_o, mvmax, rtol=1e-15, atol=0.0)
+    assert vu_o.shape[0] == 4
+    assert np.array_equal(vu_o.astype(bool), vuln)
+    # the four tables differ pairwise (the index order is pinned), and a
+    # higher level implies the lower one
+    for i in range(4):
+        for j in range(i + 1, 4):
+            assert not np.array_equal(vuln[i], vuln[j]), (i, j)
+    assert not np.any(vuln[3] & ~vuln[0]) and not np.any(vuln[2] & ~vuln[0])
+    assert not np.any(vuln[1] & ~vuln[0])
+    # the run moved the maxima away from the initial state in many cells
+    assert (hmax > h0).sum() > 50 and (pdmax > pd_first).sum() > 0
