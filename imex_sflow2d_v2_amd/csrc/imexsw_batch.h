@@ -54,7 +54,7 @@ __global__ void __launch_bounds__(64 * kMaskGroup) kb_mask(const DevParams* __re
   const int t = g - L.tile_off[m];
   unsigned rows_m = 0;
   if (g < L.tile_off[L.members] && D.scal[S_DONE] == 0.0)
-    rows_m = mask_tile(P, D, t % tiles_x, t / tiles_x, tiles_x, 0, 0);
+    rows_m = mask_tile(P, D, t % tiles_x, t / tiles_x, tiles_x, 0, false);
   const ListSink sink{L.tiles, L.rows, &L.counts[0], &L.counts[1], m << kMemberShift};
   append_group<kMaskGroup>(sink, t, rows_m);
 }

@@ -204,17 +204,19 @@ __device__ __forceinline__ unsigned lane_read(unsigned v, int src) {
 
 // Writes the tile's mask bytes, counters and tile flag; returns the
 // (wavefront-uniform) bit word of the tile rows with a cell to compute.
-// With `partial`, a tile that is not dirty keeps its mask bytes and counts
-// (nothing within reach changed state since its last scan, so neither can
-// have changed, and it has no cell to compute).  The active-cell counter is
-// a running total kept by per-tile differences.
+// `clean`: a partial scan found the tile not dirty, so it keeps its mask
+// bytes and counts (nothing within reach changed state since its last scan,
+// so neither can have changed, and it has no cell to compute).  The
+// active-cell counter is a running total kept by per-tile differences.
 __device__ __forceinline__ unsigned mask_tile(const DevParams& P, const DevPtrs& D, int tx,
                                               int tyy, int tiles_x, int solve_all,
-                                              int partial) {
+                                              bool clean) {
   constexpr int LH = kTileH + 4;
   const int lane = threadIdx.x & 63;
   const int tile = tyy * tiles_x + tx;
-  if (partial && !D.tile_dirty[tile]) return 0u;
+  if (clean) return 0u;
+  // the tile's previous active count, loaded with the flags
+  const int nact_old = D.tile_nact[tile];
   const int j = tx * kStripW - 2 + lane;
   const int k0 = tyy * kTileH;
   const bool injx = (j >= 0 && j < P.nx);
@@ -296,7 +298,7 @@ __device__ __forceinline__ unsigned mask_tile(const DevParams& P, const DevPtrs&
     rows_m |= (unsigned)__shfl_xor((int)rows_m, off, 64);
   }
   if (lane == 0) {
-    const int delta = count - D.tile_nact[tile];
+    const int delta = count - nact_old;
     if (delta) atomicAdd(ctr(D, C_SOLVE_CELLS, tile), (unsigned long long)(long long)delta);
     D.tile_nact[tile] = count;
     D.tile_dirty[tile] = 0;
@@ -323,8 +325,14 @@ __device__ __forceinline__ void append_group(const ListSink& L, int tile, unsign
       nt += s_rows[g] != 0u;
       nr += __popc(s_rows[g]);
     }
-    s_tb = nt ? (int)atomicAdd(L.n_tiles, (unsigned long long)nt) : 0;
-    s_rb = nr ? (int)atomicAdd(L.n_rows, (unsigned long long)nr) : 0;
+    // nr > 0 exactly when nt > 0: one branch, both reservations in flight
+    int tb = 0, rb = 0;
+    if (nt) {
+      tb = (int)atomicAdd(L.n_tiles, (unsigned long long)nt);
+      rb = (int)atomicAdd(L.n_rows, (unsigned long long)nr);
+    }
+    s_tb = tb;
+    s_rb = rb;
   }
   __syncthreads();
   if (rows_m) {
@@ -344,6 +352,8 @@ __global__ void __launch_bounds__(64 * kMaskGroup) k_mask(DevParams P, DevPtrs D
                                                           int solve_all, int partial) {
   const int tiles_x = (P.nx + kStripW - 1) / kStripW;
   const int tiles_y = (P.ny + kTileH - 1) / kTileH;
+  const int w = threadIdx.x >> 6;
+  bool clean = false;
   if (partial) {
     // a workgroup whose tiles are all clean has nothing to do (the same
     // test on every wavefront, so none waits at a barrier)
@@ -351,13 +361,15 @@ __global__ void __launch_bounds__(64 * kMaskGroup) k_mask(DevParams P, DevPtrs D
 #pragma unroll
     for (int g = 0; g < kMaskGroup; g++) {
       const int ty = blockIdx.y * kMaskGroup + g;
-      any = any || (ty < tiles_y && D.tile_dirty[ty * tiles_x + blockIdx.x]);
+      const bool dirty = ty < tiles_y && D.tile_dirty[ty * tiles_x + blockIdx.x];
+      any = any || dirty;
+      if (g == w) clean = !dirty;
     }
     if (!any) return;
   }
-  const int tyy = blockIdx.y * kMaskGroup + (threadIdx.x >> 6);
+  const int tyy = blockIdx.y * kMaskGroup + w;
   unsigned rows_m = 0;
-  if (tyy < tiles_y) rows_m = mask_tile(P, D, blockIdx.x, tyy, tiles_x, solve_all, partial);
+  if (tyy < tiles_y) rows_m = mask_tile(P, D, blockIdx.x, tyy, tiles_x, solve_all, clean);
   append_group<kMaskGroup>(own_sink(D), tyy * tiles_x + blockIdx.x, rows_m);
 }
 
