@@ -256,3 +256,63 @@ def test_graph_replays_match_direct_launches():
     g.advance(47, dt0=d.dt0)
     e.advance(47, dt0=d.dt0)
     assert np.array_equal(g.q(), e.q())
+
+
+@pytest.fixture(scope="module")
+def volcano_deck():
+    return decks.volcano_2048()
+
+
+def test_volcano_2048_matches_oracle(volcano_deck):
+    """The benchmark's own workload at its full size (2048 x 2048, three
+    solid classes, radial source, erosion, topography change): 60 steps on
+    the device (three graph replays) against the oracle in both interface
+    semantics, the one the device implements (fresh faces) and the
+    reference's (stale faces)."""
+    d = volcano_deck
+    print("case volcano 2048x2048, 60 steps", flush=True)
+    dev = Solver(d.params).load_deck(d)
+    n_dev = dev.advance(60, dt0=d.dt0)
+    for fresh in (True, False):
+        orc = Oracle(d.params, fresh=fresh).load_deck(d)
+        assert orc.run(60, dt0=d.dt0) == n_dev
+        assert orc.t == dev.t, (fresh, orc.t, dev.t)
+        check_state(orc, dev)
+        assert dev.stats()["cell_updates_active"] == orc.counters()["active_cell_updates"]
+        del orc
+
+
+def test_volcano_2048_schedules_bitwise_identical(volcano_deck):
+    """Size-independent property at full size: the schedule for few
+    computed cells and the one for many give the same bits over 120 steps
+    (six graph replays each)."""
+    d = volcano_deck
+    runs = []
+    for tuning in ((16, 1), (1, 1 << 20)):
+        s = Solver(d.params)
+        s.set_tuning(*tuning)
+        s.load_deck(d)
+        s.advance(120, dt0=d.dt0)
+        runs.append((s, s.stats()))
+    (a, sa), (b, sb) = runs
+    print("sparse steps: %d and %d" % (sa["sparse_steps"], sb["sparse_steps"]), flush=True)
+    # the first run takes the dense form whenever more than 120 cells are
+    # computed (all but possibly the first replay)
+    assert sa["sparse_steps"] <= 20 and sb["sparse_steps"] == 120
+    assert a.t == b.t
+    assert np.array_equal(a.q(), b.q())
+    for x, y in zip(a.post(), b.post()):
+        assert np.array_equal(x, y)
+    for x, y in zip(a.erosion_state(), b.erosion_state()):
+        assert np.array_equal(x, y)
+    assert sa["newton_iters"] == sb["newton_iters"]
+
+
+def test_wet_2048_matches_oracle():
+    """Every cell of a 2048 x 2048 grid wet and computed (the dense
+    kernels at full size): 3 steps against the oracle."""
+    d = decks.wet_2048()
+    print("case wet 2048x2048, 3 steps", flush=True)
+    orc, dev = run_pair(d, 3)
+    check_state(orc, dev)
+    assert dev.stats()["newton_iters"] == orc.counters()["newton_iters"]
