@@ -297,6 +297,36 @@ def test_dambreak_converges_to_stoker(nx):
     assert err < {200: 2.0e-2, 800: 6.0e-3}[nx], err
 
 
+def _stoker_run(nx, n_rk):
+    L = 400.0
+    p = make_params(nx, 1, L / nx, 1, n_rk=n_rk, energy_flag=False, rheology=False, cfl=0.4,
+                    max_dt=10.0, rho_s=(2500.0,), diam_s=(1e-3,), sp_heat_s=(1617.0,))
+    xc = (np.arange(nx) + 0.5) * L / nx
+    q = decks._state_from_h(p, np.where(xc < L / 2, 2.0, 1.0)[:, None], 300.0, [0.01])
+    o = Oracle(p).load_deck(decks.Deck("stoker", p, np.zeros((nx, 1)), q, dt0=1e-3))
+    o.run(100000, dt0=1e-3, t_end=10.0)
+    rho_m, gp = _mix_rho_g(p, 1.0, [0.01], 300.0)
+    h = o.qp()[0, :, 0]
+    return h, np.abs(h - _stoker(2.0, 1.0, gp, xc - L / 2, 10.0)).mean()
+
+
+def test_tableaux_on_stoker():
+    """The other IMEX tableaux (src/solver_2d.f90:398-486) on the same
+    frictionless dam break.  IMEX-SSP(3,3,2) (N_RK = 3) evaluates every
+    stage's flux and converges to the exact solution.  The reference
+    evaluates a stage's flux only when its omega_tilde weight is positive
+    (src/solver_2d.f90:1327-1335), so with LRR(3,2,2) (N_RK = 4, weights
+    0, 1, 0, 0) the first stage's flux stays zero, the second stage starts
+    from q0 and the step reduces to N_RK = 2's forward Euler: the same
+    state to round-off."""
+    h2, e2 = _stoker_run(800, 2)
+    h4, e4 = _stoker_run(800, 4)
+    assert np.abs(h4 - h2).max() <= 1e-12, np.abs(h4 - h2).max()
+    _, e3a = _stoker_run(200, 3)
+    _, e3b = _stoker_run(800, 3)
+    assert e3b < 6.0e-3 and e3b < e3a / 3.0, (e3a, e3b)
+
+
 def test_mass_conservation_closed_box():
     nv = 6
     bc = {s: [(2, 0.0)] * nv for s in range(4)}
