@@ -60,3 +60,25 @@ def test_ensemble_member_matches_oracle():
     scale = np.abs(qo).max(axis=(1, 2))[:, None, None]
     assert np.all(np.abs(qo - qd) <= 1e-12 * scale)
     ens.close()
+
+
+def test_ensemble_full_config_members_bitwise():
+    """The benchmark's ensemble at full size (256 members of 130^2 to 1021^2
+    cells, 7.4e7 cells): one 25-step graph replay of the batch gives the
+    bits of the smallest, the largest, the first and the last member
+    advanced alone."""
+    ds = decks.ensemble_members(count=256)
+    sizes = [d.params.nx for d in ds]
+    picks = sorted({int(np.argmin(sizes)), int(np.argmax(sizes)), 0, len(ds) - 1})
+    print("ensemble 256 members, cells", sum(d.params.nx * d.params.ny for d in ds),
+          "checking", picks, flush=True)
+    ens = Ensemble([d.params for d in ds]).load(ds)
+    ens.set_tuning(16, 25)
+    assert ens.advance(25, dt0=ds[0].dt0) == 25
+    for i in picks:
+        alone = Solver(ds[i].params).load_deck(ds[i])
+        alone.advance(25, dt0=ds[i].dt0)
+        assert alone.t == ens.t[i], (i, alone.t, ens.t[i])
+        assert np.array_equal(alone.q(), ens.members[i].q()), i
+        del alone
+    ens.close()
