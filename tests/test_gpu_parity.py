@@ -316,3 +316,22 @@ def test_wet_2048_matches_oracle():
     orc, dev = run_pair(d, 3)
     check_state(orc, dev)
     assert dev.stats()["newton_iters"] == orc.counters()["newton_iters"]
+
+
+def test_dambreak_1d_full_run():
+    """The 1D dam break configuration to its end time (400 cells, 200 s,
+    2219 steps: 110 graph replays and a remainder)."""
+    d = decks.dambreak_1d()
+    orc, dev = run_pair(d, 100000, t_end=d.t_end)
+    assert orc.t == d.t_end
+    check_state(orc, dev)
+    assert dev.stats()["newton_iters"] == orc.counters()["newton_iters"]
+
+
+def test_incline_512_matches_oracle():
+    """The 512 x 512 incline configuration (fully wet, two solid classes,
+    settling): its first 50 steps against the oracle."""
+    d = decks.incline_2d()
+    orc, dev = run_pair(d, 50)
+    check_state(orc, dev)
+    assert dev.stats()["newton_iters"] == orc.counters()["newton_iters"]
