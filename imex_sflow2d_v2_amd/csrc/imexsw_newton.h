@@ -438,10 +438,10 @@ __device__ __forceinline__ NewtonOut solve_rk_step_quad(const DevParams& P, doub
       } else {
         // lnsrch's trials: 1, then a2 = max(quadratic model, 0.5), then
         // halvings (the cubic model is capped at half the last step, so
-        // max(tmplam, 0.5 * alam) is always 0.5 * alam).  Lane r of a round
+        // max(tmplam, 0.5 * alam) is always 0.5 * alam).  Lane r of a batch
         // evaluates one trial; the trials are then taken in order with the
         // serial rules, so the accepted point and the count are the serial
-        // ones.  Round 1: the Jacobian at trial 1 (lanes 0, 1), trial 1
+        // ones.  First batch: the Jacobian at trial 1 (lanes 0, 1), trial 1
         // (lane 2), and the trial 0.5 that follows whenever a2 = 0.5 (lane 3).
         const int role = threadIdx.x & 3;
         double n1a, n2a;  // this lane's implicit terms (real parts)

@@ -13,8 +13,8 @@
  * cells that leave the active set -- is reproduced.  `fresh` mode (see
  * orc_set_fresh) instead rebuilds every non-source cell's interface values
  * from the current state at every reconstruction; this is the semantics the
- * device implementation follows, and the difference between the two modes is
- * what the tests measure and DESIGN.md documents.
+ * device implementation follows, and the tests measure the difference
+ * between the two modes.
  */
 #ifndef IMEXSW_ORACLE_H
 #define IMEXSW_ORACLE_H
