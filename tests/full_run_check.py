@@ -2,12 +2,13 @@
 most of it the oracle on 16 host threads).
 
 Runs the 2048 x 2048 volcano workload that bench.py times for N steps
-(default 1000, the benchmark's length) on the device, and the oracle
-(fresh-face semantics) from the same deck, then compares every conserved
+(default 1000, the benchmark's length) on the device, and the oracle from
+the same deck in fresh-face semantics (the device's) or, with `stale`, in
+the reference's stale-face semantics, then compares every conserved
 variable, the running maxima and vulnerability table, the erosion fields and
 bed, the mask, the active cell-update count and the Newton iteration count.
 
-    python3 -u tests/full_run_check.py [steps]
+    python3 -u tests/full_run_check.py [steps] [stale]
 
 Exit status 0 when everything is bit-identical.
 """
@@ -24,14 +25,15 @@ from imex_sflow2d_v2_amd.solver import Solver  # noqa: E402
 from oracle import Oracle  # noqa: E402
 
 
-def main(steps):
+def main(steps, fresh=True):
     t0 = time.time()
     d = decks.volcano_2048()
     print("deck %.1f s" % (time.time() - t0), flush=True)
     dev = Solver(d.params).load_deck(d)
     n = dev.advance(steps, dt0=d.dt0)
     print("device steps", n, "t", dev.t, flush=True)
-    orc = Oracle(d.params, fresh=True, threads=16).load_deck(d)
+    orc = Oracle(d.params, fresh=fresh, threads=16).load_deck(d)
+    print("oracle faces:", "fresh" if fresh else "stale", flush=True)
     done = 0
     while done < steps:
         k = orc.run(min(100, steps - done), dt0=d.dt0)
@@ -65,4 +67,5 @@ def main(steps):
 
 
 if __name__ == "__main__":
-    sys.exit(main(int(sys.argv[1]) if len(sys.argv) > 1 else 1000))
+    sys.exit(main(int(sys.argv[1]) if len(sys.argv) > 1 else 1000,
+                  fresh=not (len(sys.argv) > 2 and sys.argv[2] == "stale")))
