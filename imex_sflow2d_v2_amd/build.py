@@ -13,7 +13,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libimexsw.so")
 SOURCES = ["imexsw.hip"]
 HEADERS = ["imexsw_math.h", "imexsw_physics.h", "imexsw_newton.h", "imexsw_kernels.h",
-           "imexsw_batch.h", "imexsw_mlp.h"]
+           "imexsw_batch.h", "imexsw_mlp.h", "imexsw_q2d.h"]
 ARCH = os.environ.get("IMEXSW_ARCH", "gfx950")
 
 HIPCC_FLAGS = [

@@ -16,6 +16,7 @@
 
 #include "imexsw_batch.h"
 #include "imexsw_kernels.h"
+#include "imexsw_q2d.h"
 
 using namespace imexsw;
 
@@ -1057,6 +1058,68 @@ int imexsw_sync(imexsw_ctx* c) {
   if (!c) return fail(c, "null argument");
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return 0;
+}
+
+int imexsw_q2d_write(const char* path, int32_t nv, int32_t nx, int32_t ny,
+                     const double* x_comp, const double* y_comp, const double* q) {
+  if (!path || !x_comp || !y_comp || !q || nv < 1 || nx < 1 || ny < 1) return -1;
+  FILE* fp = std::fopen(path, "w");
+  if (!fp) return -1;
+  std::string rec;
+  char fld[21];
+  for (int k = 0; k < ny; k++) {
+    for (int j = 0; j < nx; j++) {
+      rec.clear();
+      fortran_e20_12(x_comp[j], fld);
+      rec += fld;
+      fortran_e20_12(y_comp[k], fld);
+      rec += fld;
+      const double* qc = q + ((size_t)k * nx + j) * nv;
+      for (int i = 0; i < nv; i++) {
+        fortran_e20_12(std::fabs(qc[i]) < 1.0e-20 ? 0.0 : qc[i], fld);
+        rec += fld;
+      }
+      rec += '\n';
+      std::fputs(rec.c_str(), fp);
+    }
+    std::fputs("  \n", fp);
+  }
+  std::fputs("  \n  \n", fp);
+  return std::fclose(fp) == 0 ? 0 : -1;
+}
+
+int imexsw_q2d_read(const char* path, int32_t nv, int32_t n_solid, int32_t nx, int32_t ny,
+                    double* q) {
+  if (!path || !q || nv < 1 || nx < 1 || ny < 1 || n_solid < 0 || 4 + n_solid > nv) return -1;
+  FILE* fp = std::fopen(path, "r");
+  if (!fp) return -1;
+  std::string rec;
+  int rc = 0;
+  for (int k = 0; k < ny && !rc; k++) {
+    for (int j = 0; j < nx && !rc; j++) {
+      if (!read_record(fp, rec)) {
+        rc = -1;
+        break;
+      }
+      rec.resize((size_t)20 * (nv + 2), ' ');  // a short record pads with blanks
+      double* qc = q + ((size_t)k * nx + j) * nv;
+      double xy;
+      if (!fortran_e_field(rec.c_str(), 20, &xy) || !fortran_e_field(rec.c_str() + 20, 20, &xy)) {
+        rc = -1;
+        break;
+      }
+      for (int i = 0; i < nv; i++)
+        if (!fortran_e_field(rec.c_str() + 20 * (i + 2), 20, &qc[i])) rc = -1;
+      if (rc) break;
+      if (qc[0] <= 0.0)
+        for (int i = 0; i < nv; i++) qc[i] = 0.0;
+      for (int i = 4; i < 4 + n_solid; i++)
+        if (qc[i] <= 0.0) qc[i] = 0.0;
+    }
+    if (!rc && !read_record(fp, rec) && k < ny - 1) rc = -1;  // the row's closing record
+  }
+  std::fclose(fp);
+  return rc;
 }
 
 }  // extern "C"

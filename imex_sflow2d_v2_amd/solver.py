@@ -93,6 +93,8 @@ ABI = {
     "imexsw_harvest_get": (C.c_int, [C.c_void_p, _F, C.c_int64, C.POINTER(C.c_int64)]),
     "imexsw_set_warm_start": (C.c_int, [C.c_void_p, C.c_void_p]),
     "imexsw_sync": (C.c_int, [C.c_void_p]),
+    "imexsw_q2d_write": (C.c_int, [C.c_char_p, C.c_int32, C.c_int32, C.c_int32, _D, _D, _D]),
+    "imexsw_q2d_read": (C.c_int, [C.c_char_p, C.c_int32, C.c_int32, C.c_int32, C.c_int32, _D]),
 }
 
 _lib = None
@@ -125,6 +127,31 @@ def _dp(a):
 
 def _f(a, dtype=np.float64):
     return np.asfortranarray(a, dtype=dtype)
+
+
+def q2d_write(path, q, x_comp, y_comp):
+    """output_solution's .q_2d file (src/inpout_2d.f90:5766-5891) for the
+    state q[nv, nx, ny]; x_comp, y_comp: cell-centre coordinates."""
+    lib = load_library()
+    q = _f(q)
+    nv, nx, ny = q.shape
+    x = np.ascontiguousarray(x_comp, dtype=np.float64)
+    y = np.ascontiguousarray(y_comp, dtype=np.float64)
+    if x.shape != (nx,) or y.shape != (ny,):
+        raise ValueError("x_comp / y_comp do not match q's shape")
+    if lib.imexsw_q2d_write(os.fsencode(path), nv, nx, ny, _dp(x), _dp(y), _dp(q)) != 0:
+        raise ImexswError("cannot write %s" % path)
+
+
+def q2d_read(path, nv, n_solid, nx, ny):
+    """read_solution's .q_2d branch (src/inpout_2d.f90:5432-5453): the state
+    q[nv, nx, ny] (Fortran order)."""
+    lib = load_library()
+    q = np.zeros((nv, nx, ny), order="F")
+    if lib.imexsw_q2d_read(os.fsencode(path), nv, n_solid, nx, ny, _dp(q)) != 0:
+        raise ImexswError("cannot read %s as a %d x %d .q_2d state with %d variables"
+                          % (path, nx, ny, nv))
+    return q
 
 
 class Solver:

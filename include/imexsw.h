@@ -294,6 +294,30 @@ int imexsw_set_tuning(imexsw_ctx *ctx, int max_strip_rows, int target_strips);
 int imexsw_sync(imexsw_ctx *ctx);
 
 /* ---------------------------------------------------------------------------
+ * .q_2d state files (host only; no context, no device work)
+ * ------------------------------------------------------------------------- */
+
+/* Write a state in the .q_2d text format of output_solution
+ * (src/inpout_2d.f90:5766-5891, the conservative-variable file): one record
+ * per cell, j fastest within k, holding x_comp[j], y_comp[k] and
+ * q(1..nv, j, k), each as Fortran E20.12; values with |q| < 1e-20 are
+ * written as zero (the reference also zeroes them in its state array); a
+ * record with a blank after every row of cells and two at the end.  q is the
+ * host's column-major (var, j, k) array.  Returns 0, or -1 when the file
+ * cannot be written. */
+int imexsw_q2d_write(const char *path, int32_t nv, int32_t nx, int32_t ny,
+                     const double *x_comp, const double *y_comp, const double *q);
+
+/* Read a .q_2d restart file as read_solution does
+ * (src/inpout_2d.f90:5432-5453): for each row k, nx records of 20-column
+ * E fields x, y, q(1..nv) followed by one record that is skipped; a cell
+ * with q(1) <= 0 is set to zero, and negative solid masses q(5..4+n_solid)
+ * to zero.  Fields beyond the end of a record read as zero.  Returns 0, or
+ * -1 when the file is missing, short or has a malformed field. */
+int imexsw_q2d_read(const char *path, int32_t nv, int32_t n_solid, int32_t nx, int32_t ny,
+                    double *q);
+
+/* ---------------------------------------------------------------------------
  * Ensembles: many independent grids (members) of possibly different sizes
  * and parameters advanced together, one set of batched launches per time
  * step for all members, the steps captured in a HIP graph.  The reference

@@ -17,7 +17,7 @@ HEADER = os.path.join(ROOT, "include", "imexsw.h")
 def declared():
     src = open(HEADER).read()
     src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
-    return sorted(set(re.findall(r"\b(imexsw_[a-z_]+)\s*\(", src)))
+    return sorted(set(re.findall(r"\b(imexsw_[a-z0-9_]+)\s*\(", src)))
 
 
 def test_header_and_wrapper_agree():

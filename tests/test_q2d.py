@@ -1,0 +1,106 @@
+""".q_2d state files (imexsw_q2d_write / imexsw_q2d_read, host code of
+libimexsw.so; no GPU needed).
+
+The reader is checked on the reference's own restart fixture
+(TESTS/TEST_BOX/INIT_BOX_0000.q_2d, copied under tests/golden/) against
+numpy's parser; the writer against Fortran E20.12 output strings fixed by the
+language's E editing rules (0.d mantissa, 12 digits, E+dd or +ddd exponent,
+right-justified), and both by round trips.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from imex_sflow2d_v2_amd.solver import ImexswError, q2d_read, q2d_write
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+BOX = os.path.join(HERE, "golden", "INIT_BOX_0000.q_2d")
+
+
+def test_reader_on_reference_fixture():
+    # TEST_BOX: 400 x 1 cells, one solid class (n_vars = 5), cell size 125 m
+    q = q2d_read(BOX, 5, 1, 400, 1)
+    ref = np.loadtxt(BOX)
+    assert ref.shape == (400, 7)
+    expect = ref[:, 2:].T.copy()
+    # read_solution's clean-up: cells with q(1) <= 0 become zero, negative
+    # solid masses zero
+    expect[:, expect[0] <= 0.0] = 0.0
+    expect[4][expect[4] <= 0.0] = 0.0
+    np.testing.assert_array_equal(q[:, :, 0], expect)
+    assert (q[0] > 0).sum() > 0 and (q[0] == 0).sum() > 0
+
+
+def _record(tmp_path, values):
+    p = str(tmp_path / "one.q_2d")
+    q = np.array(values, dtype=np.float64).reshape(len(values), 1, 1)
+    q2d_write(p, q, [62.5], [-2.5e-300])
+    return open(p).read().split("\n")
+
+
+def test_writer_fortran_e20_12_fields(tmp_path):
+    lines = _record(tmp_path, [0.4, 0.0, 1.23e100, 9.9999999999995, 5e-21, -0.0015])
+    rec = lines[0]
+    fields = [rec[i:i + 20] for i in range(0, len(rec), 20)]
+    assert fields == [
+        "  0.625000000000E+02",   # x
+        " -0.250000000000-299",   # y (coordinates are not cut at 1e-20)
+        "  0.400000000000E+00",
+        "  0.000000000000E+00",
+        "  0.123000000000+101",   # three-digit exponent: no letter
+        "  0.100000000000E+02",   # rounds up to 12 digits
+        "  0.000000000000E+00",   # |q| < 1e-20 written as zero
+        " -0.150000000000E-02",
+    ]
+    # a blank record closes the row, two more end the file
+    assert lines[1].strip() == "" and lines[2].strip() == "" and lines[3].strip() == ""
+    assert lines[4] == "" and len(lines) == 5
+
+
+def test_round_trip(tmp_path):
+    rng = np.random.default_rng(3)
+    nv, nx, ny = 7, 13, 5
+    q = rng.standard_normal((nv, nx, ny)) * 10.0 ** rng.integers(-8, 9, (nv, nx, ny))
+    q[0] = np.abs(q[0]) + 1e-3          # positive mass
+    q[4:7] = np.abs(q[4:7])             # non-negative solids
+    q[0, 2, 3] = 0.0                    # an empty cell
+    q[1, 4, 1] = 3e-21                  # below the writer's cut
+    x = (np.arange(nx) + 0.5) * 2.5
+    y = (np.arange(ny) + 0.5) * 2.5 + 100.0
+    p = str(tmp_path / "s.q_2d")
+    q2d_write(p, q, x, y)
+    back = q2d_read(p, nv, 3, nx, ny)
+    expect = np.vectorize(lambda v: float("%.11e" % v))(np.where(np.abs(q) < 1e-20, 0.0, q))
+    expect[:, 2, 3] = 0.0
+    np.testing.assert_array_equal(back, expect)
+    text = open(p).read().split("\n")
+    assert len(text) == ny * (nx + 1) + 2 + 1
+    assert np.loadtxt(p, usecols=(0, 1))[:nx, 0].tolist() == [float("%.11e" % v) for v in x]
+
+
+def test_rewrite_reference_fixture(tmp_path):
+    """The fixture holds 13 significant digits, E20.12 output 12: rewriting
+    rounds each value once (relative change <= 5e-12), and writing what was
+    read back gives the same text again."""
+    q = q2d_read(BOX, 5, 1, 400, 1)
+    ref = np.loadtxt(BOX)
+    p1, p2 = str(tmp_path / "box1.q_2d"), str(tmp_path / "box2.q_2d")
+    q2d_write(p1, q, ref[:, 0], ref[:, 1][:1])
+    q1 = q2d_read(p1, 5, 1, 400, 1)
+    np.testing.assert_allclose(q1, q, rtol=5e-12, atol=0.0)
+    q2d_write(p2, q1, ref[:, 0], ref[:, 1][:1])
+    assert open(p1).read() == open(p2).read()
+
+
+def test_reader_errors(tmp_path):
+    with pytest.raises(ImexswError):
+        q2d_read(str(tmp_path / "missing.q_2d"), 5, 1, 4, 1)
+    short = tmp_path / "short.q_2d"
+    short.write_text(open(BOX).read().split("\n", 10)[0] + "\n")
+    with pytest.raises(ImexswError):
+        q2d_read(str(short), 5, 1, 400, 1)
+    bad = tmp_path / "bad.q_2d"
+    bad.write_text(" 1.0e+00" + " " * 12 + "  x.0000000000000e+00\n  \n")
+    with pytest.raises(ImexswError):
+        q2d_read(str(bad), 5, 1, 1, 1)
