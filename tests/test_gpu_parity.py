@@ -335,3 +335,24 @@ def test_incline_512_matches_oracle():
     orc, dev = run_pair(d, 50)
     check_state(orc, dev)
     assert dev.stats()["newton_iters"] == orc.counters()["newton_iters"]
+
+
+def test_q2d_restart_matches_oracle(tmp_path):
+    """A restart through a .q_2d file: the device state after 10 steps is
+    written with imexsw_q2d_write, read back with imexsw_q2d_read (12
+    significant digits, read_solution's clean-up), and the restarted device
+    run agrees with the oracle restarted from the same file."""
+    from imex_sflow2d_v2_amd.solver import q2d_read, q2d_write
+    d = decks.small_2d(n=40, ny=36, seed=12)
+    p = d.params
+    dev = Solver(p).load_deck(d)
+    dev.advance(10, dt0=d.dt0)
+    x, y, _, _ = decks.cell_centres(p)
+    path = str(tmp_path / "restart_0001.q_2d")
+    q2d_write(path, dev.q(), x, y)
+    qr = q2d_read(path, p.n_vars, p.n_solid, p.nx, p.ny)
+    np.testing.assert_allclose(qr, np.where(np.abs(dev.q()) < 1e-20, 0.0, dev.q()),
+                               rtol=5e-12, atol=0.0)
+    d2 = decks.Deck("restart", p, d.B_cent, qr, erodible=d.erodible, dt0=d.dt0)
+    orc, dev2 = run_pair(d2, 15)
+    check_state(orc, dev2)
