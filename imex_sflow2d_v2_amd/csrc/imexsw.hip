@@ -14,67 +14,18 @@
 #include <string>
 #include <vector>
 
-#include "imexsw_batch.h"
-#include "imexsw_kernels.h"
+#include "imexsw_ctx.h"
 #include "imexsw_q2d.h"
 
 using namespace imexsw;
 
 namespace {
 
-enum KernelId { K_MASK = 0, K_STENCIL, K_LOCAL, K_TOPO, K_DTPOLICY, K_POST, K_EROSION, K_NK };
 const char* kKernelNames[K_NK] = {"mask", "stencil", "local", "topo", "dtpolicy", "post", "erosion"};
 
 thread_local std::string g_create_error;
 
 }  // namespace
-
-struct imexsw_ctx {
-  imexsw_params p{};
-  DevParams P{};
-  DevPtrs D{};
-  hipStream_t stream = nullptr;
-  std::string err;
-  int ns = 0, nv = 0;
-  size_t plane = 0;
-  double* stage = nullptr;  // layout-conversion staging
-  size_t stage_bytes = 0;
-  dim3 grid_tiles, grid_mask, grid_stag;
-  int n_tiles = 0;  // work tiles (kStripW x kTileH)
-  int n_cu = 256;   // compute units of the device
-  int strip_h = 16;         // tallest stencil strip (rows)
-  int row_unit = 1;         // shortest stencil strip (rows)
-  int target_strips = 2048; // strips wanted in flight
-  bool d_valid = false;   // D holds the residual of the current q
-  bool dt_hist = false;   // dt_old / dt_old_old initialised
-  bool timing = false;
-  double ms[K_NK] = {0};
-  int64_t calls[K_NK] = {0};
-  std::vector<hipEvent_t> ev_pool;
-  std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> pending;
-  std::vector<void*> allocs;
-  // learned warm start (guess planes in D.guess) and harvest buffer
-  bool warm = false;
-  MlpDev mlp{};
-  // sparse steps (few computed cells): the row stencil and the fused local
-  // kernel; chosen on the host from the computed-cell count of an earlier
-  // step (both ways give the same bits)
-  bool sparse = false;
-  int64_t sparse_steps = 0;
-  unsigned long long* h_cnt = nullptr;  // pinned: 2 x kSlots computed-cell counters
-  hipEvent_t cnt_ev[2] = {nullptr, nullptr};
-  // captured kGraphSteps-step graphs of imexsw_advance (dense, sparse) and
-  // what they captured
-  hipGraph_t graph[2] = {nullptr, nullptr};
-  hipGraphExec_t exec[2] = {nullptr, nullptr};
-  DevParams graph_P{};
-  DevPtrs graph_D{};
-  long graph_shape = 0;
-  MlpDev graph_mlp{};
-  bool graph_warm = false;
-};
-
-constexpr int kGraphSteps = 20;
 
 namespace {
 
@@ -155,88 +106,7 @@ int check_launch(imexsw_ctx* c, const char* what) {
   return 0;
 }
 
-// ---- NS dispatch -----------------------------------------------------------
-// The stencil and local kernels walk device-side work lists (listed tiles
-// and tile rows) with a grid-stride loop, so the grid is sized for the
-// device, not for the work: as many workgroups as fit on the device at once
-// (one round, no queue of workgroups that find nothing to do), never more
-// than the largest possible list needs.
-template <typename K>
-int resident_grid(K kernel, int threads, int n_cu, int items) {
-  int per_cu = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, threads, 0) != hipSuccess ||
-      per_cu < 1)
-    per_cu = 1;
-  return std::max(1, std::min(items, per_cu * n_cu));
-}
 
-// rows: the three-wavefront row kernel (sparse steps); else strips
-template <int NS>
-void launch_stencil(imexsw_ctx* c, int compute_dt, bool rows) {
-  // work items: (listed tile, strip) pairs, at most tiles * kTileH / row_unit
-  const int items = c->n_tiles * (kTileH / c->row_unit);
-  if (c->P.ny > 1 && rows) {
-    static int res = 0;
-    if (!res) res = resident_grid(k_stencil_rows<NS>, 384, c->n_cu, 1 << 30);
-    hipLaunchKernelGGL((k_stencil_rows<NS>), dim3(std::min(c->n_tiles * kTileH, res)),
-                       dim3(384), 0, c->stream, c->P, c->D, compute_dt);
-  } else if (c->P.ny > 1) {
-    static int res = 0;
-    if (!res) res = resident_grid(k_stencil<NS, true>, 64, c->n_cu, 1 << 30);
-    hipLaunchKernelGGL((k_stencil<NS, true>), dim3(std::min(items, res)), dim3(64), 0,
-                       c->stream, c->P, c->D, c->row_unit, c->strip_h, c->target_strips,
-                       compute_dt);
-  } else {
-    static int res = 0;
-    if (!res) res = resident_grid(k_stencil<NS, false>, 64, c->n_cu, 1 << 30);
-    hipLaunchKernelGGL((k_stencil<NS, false>), dim3(std::min(items, res)), dim3(64), 0,
-                       c->stream, c->P, c->D, c->row_unit, c->strip_h, c->target_strips,
-                       compute_dt);
-  }
-}
-// fused: stage update, erosion and diagnostics of a row in one pass (sparse
-// steps without the learned warm start); else up to two kernels
-template <int NS>
-void launch_local(imexsw_ctx* c, int ops, bool fused) {
-  // work items: listed tile rows, four per workgroup
-  const int items = (c->n_tiles * kTileH + 3) / 4;
-  if (fused) {
-    // row parts, four per workgroup
-    static int res3 = 0;
-    if (!res3) res3 = resident_grid(k_local<NS, 3>, 256, c->n_cu, 1 << 30);
-    hipLaunchKernelGGL((k_local<NS, 3>), dim3(std::min(kSparseLanes * items, res3)), dim3(256), 0,
-                       c->stream, c->P, c->D, ops & ~(int)OP_SUMS);
-    return;
-  }
-  static int res0 = 0, res1 = 0, res2 = 0;
-  if (!res0) res0 = resident_grid(k_local<NS, 0>, 256, c->n_cu, 1 << 30);
-  if (!res1) res1 = resident_grid(k_local<NS, 1>, 256, c->n_cu, 1 << 30);
-  if (!res2) res2 = resident_grid(k_local<NS, 2>, 256, c->n_cu, 1 << 30);
-  if (ops & OP_RK) {
-    int rk = OP_RK | (ops & OP_POLICY);
-    if (c->warm) {
-      static int resg = 0;
-      if (!resg) resg = resident_grid(k_mlp_guess<NS>, 64, c->n_cu, 1 << 30);
-      hipLaunchKernelGGL((k_mlp_guess<NS>), dim3(std::min(c->n_tiles * kTileH, resg)), dim3(64),
-                         0, c->stream, c->P, c->D, c->mlp, (ops & OP_POLICY) ? 1 : 0);
-      rk |= OP_WARM;
-    }
-    if (c->D.harvest) rk |= OP_HARVEST;
-    if (!(rk & (OP_WARM | OP_HARVEST)))
-      hipLaunchKernelGGL((k_local<NS, 0>), dim3(std::min(items, res0)), dim3(256), 0, c->stream,
-                         c->P, c->D, rk);
-    else
-      hipLaunchKernelGGL((k_local<NS, 2>), dim3(std::min(items, res2)), dim3(256), 0, c->stream,
-                         c->P, c->D, rk);
-  }
-  if (ops & (OP_EROSION | OP_POST | OP_SUMS))
-    hipLaunchKernelGGL((k_local<NS, 1>), dim3(std::min(items, res1)), dim3(256), 0, c->stream,
-                       c->P, c->D, ops & ~(int)OP_RK);
-}
-template <int NS>
-void launch_qp(imexsw_ctx* c, double* dst) {
-  hipLaunchKernelGGL((k_qp_out<NS>), c->grid_tiles, dim3(256), 0, c->stream, c->P, c->D.q, dst);
-}
 
 #define NS_SWITCH(ns, CALL)        \
   switch (ns) {                    \
@@ -275,7 +145,7 @@ int run_mask(imexsw_ctx* c, int solve_all, int partial = 0) {
   {
     Timed tm(c, K_MASK);
     const dim3 grid(c->grid_mask.x, (c->grid_mask.y + kMaskGroup - 1) / kMaskGroup);
-    hipLaunchKernelGGL(k_mask, grid, dim3(64 * kMaskGroup), 0, c->stream, c->P, c->D,
+    hipLaunchKernelGGL(k_mask<>, grid, dim3(64 * kMaskGroup), 0, c->stream, c->P, c->D,
                        solve_all, partial);
   }
   return check_launch(c, "k_mask");
@@ -288,14 +158,14 @@ int run_topo(imexsw_ctx* c, int sparse) {
     Timed tm(c, K_TOPO);
     if (sparse == 2) {
       static int res = 0;
-      if (!res) res = resident_grid(k_topo_list, 256, c->n_cu, 1 << 30);
-      hipLaunchKernelGGL(k_topo_list, dim3(std::min(9 * c->n_tiles, res)), dim3(256), 0,
+      if (!res) res = resident_grid(k_topo_list<>, 256, c->n_cu, 1 << 30);
+      hipLaunchKernelGGL(k_topo_list<>, dim3(std::min(9 * c->n_tiles, res)), dim3(256), 0,
                          c->stream, c->P, c->D);
     } else {
-      hipLaunchKernelGGL(k_topo, c->grid_mask, dim3(256), 0, c->stream, c->P, c->D, sparse);
+      hipLaunchKernelGGL(k_topo<>, c->grid_mask, dim3(256), 0, c->stream, c->P, c->D, sparse);
     }
     if (c->D.gcoef)
-      hipLaunchKernelGGL(k_gstag, c->grid_stag, dim3(256), 0, c->stream, c->P, c->D);
+      hipLaunchKernelGGL(k_gstag<>, c->grid_stag, dim3(256), 0, c->stream, c->P, c->D);
   }
   return check_launch(c, "k_topo");
 }
@@ -307,7 +177,7 @@ int run_topo(imexsw_ctx* c, int sparse) {
 int enqueue_step(imexsw_ctx* c, bool commit_prev) {
   {
     Timed tm(c, K_DTPOLICY);
-    hipLaunchKernelGGL(k_step_begin, dim3(1), dim3(1), 0, c->stream, c->P, c->D,
+    hipLaunchKernelGGL(k_step_begin<>, dim3(1), dim3(1), 0, c->stream, c->P, c->D,
                        commit_prev ? 1 : 0, 1);
   }
   if (run_mask(c, 0, 1)) return -1;
@@ -317,7 +187,7 @@ int enqueue_step(imexsw_ctx* c, bool commit_prev) {
   return check_launch(c, "k_step_begin");
 }
 int end_steps(imexsw_ctx* c) {
-  hipLaunchKernelGGL(k_step_begin, dim3(1), dim3(1), 0, c->stream, c->P, c->D, 1, 0);
+  hipLaunchKernelGGL(k_step_begin<>, dim3(1), dim3(1), 0, c->stream, c->P, c->D, 1, 0);
   return check_launch(c, "k_step_begin");
 }
 
@@ -372,7 +242,7 @@ int upload_aos(imexsw_ctx* c, const double* host, int nvar, double* dst) {
   const size_t n = (size_t)nvar * c->P.nx * c->P.ny;
   if (ensure_stage(c, n * sizeof(double))) return -1;
   HIPCHK(c, hipMemcpyAsync(c->stage, host, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
-  hipLaunchKernelGGL(k_aos_to_planes, c->grid_tiles, dim3(256), 0, c->stream, c->P, c->stage, nvar, dst);
+  hipLaunchKernelGGL(k_aos_to_planes<>, c->grid_tiles, dim3(256), 0, c->stream, c->P, c->stage, nvar, dst);
   if (check_launch(c, "k_aos_to_planes")) return -1;
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return 0;
@@ -381,7 +251,7 @@ int upload_aos(imexsw_ctx* c, const double* host, int nvar, double* dst) {
 int download_aos(imexsw_ctx* c, const double* src, int nvar, double* host) {
   const size_t n = (size_t)nvar * c->P.nx * c->P.ny;
   if (ensure_stage(c, n * sizeof(double))) return -1;
-  hipLaunchKernelGGL(k_planes_to_aos, c->grid_tiles, dim3(256), 0, c->stream, c->P, src, nvar, c->stage);
+  hipLaunchKernelGGL(k_planes_to_aos<>, c->grid_tiles, dim3(256), 0, c->stream, c->P, src, nvar, c->stage);
   if (check_launch(c, "k_planes_to_aos")) return -1;
   HIPCHK(c, hipMemcpyAsync(host, c->stage, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -708,7 +578,7 @@ int imexsw_set_source(imexsw_ctx* c, const int32_t* source_cell, const uint8_t* 
 int imexsw_upload_q(imexsw_ctx* c, const double* q) {
   if (!c || !q) return fail(c, "null argument");
   if (upload_aos(c, q, c->nv, c->D.q)) return -1;
-  hipLaunchKernelGGL(k_sflag, c->grid_tiles, dim3(256), 0, c->stream, c->P, c->D);
+  hipLaunchKernelGGL(k_sflag<>, c->grid_tiles, dim3(256), 0, c->stream, c->P, c->D);
   if (check_launch(c, "k_sflag")) return -1;
   // every tile's neighbourhood may have changed: the next mask scans all
   HIPCHK(c, hipMemsetAsync(c->D.tile_dirty, 1, (size_t)c->n_tiles, c->stream));
@@ -1128,26 +998,6 @@ int imexsw_q2d_read(const char* path, int32_t nv, int32_t n_solid, int32_t nx, i
 // ensembles: many member grids advanced by one set of launches per step,
 // the steps captured in a HIP graph
 // ===========================================================================
-struct imexsw_ensemble {
-  std::vector<imexsw_ctx*> members;
-  int ns = 0;
-  hipStream_t stream = nullptr;
-  std::string err;
-  DevParams* d_params = nullptr;
-  DevPtrs* d_ptrs = nullptr;
-  int* d_tile_off = nullptr;
-  int* d_tiles = nullptr;
-  int* d_rows = nullptr;
-  unsigned long long* d_counts = nullptr;
-  int total_tiles = 0;
-  int n_cu = 256;
-  int strip_h = 8;
-  int graph_steps = 25;
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t exec = nullptr;
-  int graph_strip_h = 0;
-  double last_ms = 0.0;
-};
 
 namespace {
 
@@ -1168,32 +1018,20 @@ int efail(imexsw_ensemble* e, const char* fmt, ...) {
     if (e_ != hipSuccess) return efail((e), "%s: %s", #call, hipGetErrorString(e_)); \
   } while (0)
 
-BatchLists batch_lists(imexsw_ensemble* e) {
-  return BatchLists{e->d_tiles, e->d_rows, e->d_counts, e->d_tile_off, (int)e->members.size()};
-}
+
 
 template <int NS>
 void launch_batch_step(imexsw_ensemble* e) {
   const BatchLists L = batch_lists(e);
   const int M = L.members;
-  const int per = kTileH / e->strip_h;
   const dim3 mgrid((M + 255) / 256);
-  hipLaunchKernelGGL(kb_reset, mgrid, dim3(256), 0, e->stream, e->d_params, e->d_ptrs, L);
-  hipLaunchKernelGGL(kb_mask, dim3((e->total_tiles + kMaskGroup - 1) / kMaskGroup),
+  hipLaunchKernelGGL(kb_reset<>, mgrid, dim3(256), 0, e->stream, e->d_params, e->d_ptrs, L);
+  hipLaunchKernelGGL(kb_mask<>, dim3((e->total_tiles + kMaskGroup - 1) / kMaskGroup),
                      dim3(64 * kMaskGroup), 0, e->stream, e->d_params, e->d_ptrs, L);
-  static int rs = 0, rl0 = 0, rl1 = 0;
-  if (!rs) rs = resident_grid(kb_stencil<NS>, 64, e->n_cu, 1 << 30);
-  if (!rl0) rl0 = resident_grid(kb_local<NS, 0>, 256, e->n_cu, 1 << 30);
-  if (!rl1) rl1 = resident_grid(kb_local<NS, 1>, 256, e->n_cu, 1 << 30);
-  hipLaunchKernelGGL((kb_stencil<NS>), dim3(std::min(e->total_tiles * per, rs)), dim3(64), 0,
-                     e->stream, e->d_params, e->d_ptrs, L, e->strip_h);
-  hipLaunchKernelGGL(kb_dtpolicy, mgrid, dim3(256), 0, e->stream, e->d_params, e->d_ptrs, M);
-  const int litems = (e->total_tiles * kTileH + 3) / 4;
-  hipLaunchKernelGGL((kb_local<NS, 0>), dim3(std::min(litems, rl0)), dim3(256), 0, e->stream,
-                     e->d_params, e->d_ptrs, L, (int)OP_RK);
-  hipLaunchKernelGGL((kb_local<NS, 1>), dim3(std::min(litems, rl1)), dim3(256), 0, e->stream,
-                     e->d_params, e->d_ptrs, L, (int)(OP_EROSION | OP_POST));
-  hipLaunchKernelGGL(kb_advance, mgrid, dim3(256), 0, e->stream, e->d_ptrs, M);
+  launch_batch_stencil<NS>(e);
+  hipLaunchKernelGGL(kb_dtpolicy<>, mgrid, dim3(256), 0, e->stream, e->d_params, e->d_ptrs, M);
+  launch_batch_local<NS>(e);
+  hipLaunchKernelGGL(kb_advance<>, mgrid, dim3(256), 0, e->stream, e->d_ptrs, M);
 }
 
 int batch_step(imexsw_ensemble* e) {
@@ -1621,7 +1459,7 @@ int imexsw_mlp_set_params(imexsw_mlp* m, const float* params, const float* norm)
   if (!m) return mfail(m, "null argument");
   if (params) {
     MHIPCHK(m, hipMemcpyAsync(m->w, params, sizeof(float) * kMlpParams, hipMemcpyHostToDevice, m->stream));
-    hipLaunchKernelGGL(k_mlp_cast, dim3((kMlpParams + 255) / 256), dim3(256), 0, m->stream, m->w, m->wb);
+    hipLaunchKernelGGL(k_mlp_cast<>, dim3((kMlpParams + 255) / 256), dim3(256), 0, m->stream, m->w, m->wb);
     MHIPCHK(m, hipMemsetAsync(m->m1, 0, sizeof(float) * kMlpParams, m->stream));
     MHIPCHK(m, hipMemsetAsync(m->m2, 0, sizeof(float) * kMlpParams, m->stream));
     m->steps = 0;
@@ -1650,7 +1488,7 @@ int imexsw_mlp_forward(imexsw_mlp* m, const float* x, int64_t n, float* y) {
   standardise(m->host_norm, x, nullptr, (size_t)n, xs, ts, kMlpIn);
   if (mlp_data(m, (size_t)n)) return -1;
   MHIPCHK(m, hipMemcpyAsync(m->x, xs.data(), xs.size() * sizeof(float), hipMemcpyHostToDevice, m->stream));
-  hipLaunchKernelGGL(k_mlp_forward, dim3((unsigned)((n + 63) / 64)), dim3(64), 0, m->stream,
+  hipLaunchKernelGGL(k_mlp_forward<>, dim3((unsigned)((n + 63) / 64)), dim3(64), 0, m->stream,
                      mlp_dev(m), m->x, (int)n, m->t);
   std::vector<float> ys((size_t)n * kMlpOut);
   MHIPCHK(m, hipMemcpyAsync(ys.data(), m->t, ys.size() * sizeof(float), hipMemcpyDeviceToHost, m->stream));
@@ -1675,9 +1513,9 @@ int imexsw_mlp_grad(imexsw_mlp* m, const float* x, const float* t, int32_t n, fl
   MHIPCHK(m, hipMemcpyAsync(m->t, t, sizeof(float) * kMlpOut * n, hipMemcpyHostToDevice, m->stream));
   MHIPCHK(m, hipMemsetAsync(m->loss, 0, sizeof(float), m->stream));
   const MlpTrainBuf T = mlp_buf(m);
-  hipLaunchKernelGGL(k_mlp_fwd_bwd, dim3((n + 63) / 64), dim3(64), 0, m->stream, mlp_dev(m), T,
+  hipLaunchKernelGGL(k_mlp_fwd_bwd<>, dim3((n + 63) / 64), dim3(64), 0, m->stream, mlp_dev(m), T,
                      m->x, m->t, n);
-  hipLaunchKernelGGL(k_mlp_wgrad, dim3(40), dim3(64), 0, m->stream, T, n);
+  hipLaunchKernelGGL(k_mlp_wgrad<>, dim3(40), dim3(64), 0, m->stream, T, n);
   if (grad) MHIPCHK(m, hipMemcpyAsync(grad, m->grad, sizeof(float) * kMlpParams, hipMemcpyDeviceToHost, m->stream));
   float l = 0.f;
   MHIPCHK(m, hipMemcpyAsync(&l, m->loss, sizeof(float), hipMemcpyDeviceToHost, m->stream));
@@ -1755,13 +1593,13 @@ int imexsw_mlp_train(imexsw_mlp* m, const float* pairs, int64_t n, int32_t epoch
     for (int b = 0; b < nb; b++) {
       const int64_t off = (int64_t)b * batch;
       const int cnt = (int)std::min<int64_t>(batch, n - off);
-      hipLaunchKernelGGL(k_mlp_fwd_bwd, dim3((cnt + 63) / 64), dim3(64), 0, m->stream, mlp_dev(m),
+      hipLaunchKernelGGL(k_mlp_fwd_bwd<>, dim3((cnt + 63) / 64), dim3(64), 0, m->stream, mlp_dev(m),
                          T, m->x + off * kMlpInPad, m->t + off * kMlpOut, cnt);
-      hipLaunchKernelGGL(k_mlp_wgrad, dim3(40), dim3(64), 0, m->stream, T, cnt);
+      hipLaunchKernelGGL(k_mlp_wgrad<>, dim3(40), dim3(64), 0, m->stream, T, cnt);
       m->steps++;
       const float b1 = 0.9f, b2 = 0.999f;
       const float c1 = 1.f - std::pow(b1, (float)m->steps), c2 = 1.f - std::pow(b2, (float)m->steps);
-      hipLaunchKernelGGL(k_mlp_adam, dim3((kMlpParams + 255) / 256), dim3(256), 0, m->stream, m->w,
+      hipLaunchKernelGGL(k_mlp_adam<>, dim3((kMlpParams + 255) / 256), dim3(256), 0, m->stream, m->w,
                          m->wb, m->grad, m->m1, m->m2, lr, b1, b2, 1e-8f, c1, c2);
     }
     hipError_t e = hipGetLastError();
@@ -1784,7 +1622,7 @@ int imexsw_mfma_probe(const uint16_t* A, const uint16_t* B, float* D) {
     return mfail(nullptr, "probe allocation failed");
   (void)hipMemcpy(a, A, 512, hipMemcpyHostToDevice);
   (void)hipMemcpy(b, B, 512, hipMemcpyHostToDevice);
-  hipLaunchKernelGGL(k_mfma_probe, dim3(1), dim3(64), 0, 0, a, b, d);
+  hipLaunchKernelGGL(k_mfma_probe<>, dim3(1), dim3(64), 0, 0, a, b, d);
   hipError_t e = hipMemcpy(D, d, 1024, hipMemcpyDeviceToHost);
   (void)hipFree(a);
   (void)hipFree(b);

@@ -25,6 +25,7 @@ struct BatchLists {
   int members;
 };
 
+template <int = 0>
 __global__ void kb_reset(const DevParams* __restrict__ Ps, const DevPtrs* __restrict__ Ds,
                          BatchLists L) {
   const int m = blockIdx.x * blockDim.x + threadIdx.x;
@@ -37,6 +38,7 @@ __global__ void kb_reset(const DevParams* __restrict__ Ps, const DevPtrs* __rest
 
 // one wavefront per tile of every member (global tile id g), kMaskGroup
 // consecutive tiles per workgroup
+template <int = 0>
 __global__ void __launch_bounds__(64 * kMaskGroup) kb_mask(const DevParams* __restrict__ Ps,
                                                            const DevPtrs* __restrict__ Ds,
                                                            BatchLists L) {
@@ -80,6 +82,7 @@ __global__ void __launch_bounds__(64) kb_stencil(const DevParams* __restrict__ P
   }
 }
 
+template <int = 0>
 __global__ void kb_dtpolicy(const DevParams* __restrict__ Ps, const DevPtrs* __restrict__ Ds,
                             int members) {
   const int m = blockIdx.x * blockDim.x + threadIdx.x;
@@ -115,6 +118,7 @@ __global__ void __launch_bounds__(256) kb_local(const DevParams* __restrict__ Ps
   }
 }
 
+template <int = 0>
 __global__ void kb_advance(const DevPtrs* __restrict__ Ds, int members) {
   const int m = blockIdx.x * blockDim.x + threadIdx.x;
   if (m < members) advance_member(Ds[m]);

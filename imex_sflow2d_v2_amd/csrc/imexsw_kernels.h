@@ -348,6 +348,7 @@ __device__ __forceinline__ void append_group(const ListSink& L, int tile, unsign
 }
 
 // grid (tiles_x, ceil(tiles_y / kMaskGroup)), kMaskGroup wavefronts
+template <int = 0>
 __global__ void __launch_bounds__(64 * kMaskGroup) k_mask(DevParams P, DevPtrs D,
                                                           int solve_all, int partial) {
   const int tiles_x = (P.nx + kStripW - 1) / kStripW;
@@ -1387,6 +1388,7 @@ __device__ __forceinline__ void reset_member(const DevParams& P, const DevPtrs& 
 // step policy, t = t + dt, counters) when `commit`, then reset the per-step
 // counters for the next one.  With commit and without reset: the end of the
 // last step.
+template <int = 0>
 __global__ void k_step_begin(DevParams P, DevPtrs D, int commit, int reset) {
   if (commit) {
     dtpolicy_member(P, D);
@@ -1887,6 +1889,7 @@ __device__ __forceinline__ void topo_tile(const DevParams& P, const DevPtrs& D, 
   }
 }
 
+template <int = 0>
 __global__ void __launch_bounds__(256) k_topo(DevParams P, DevPtrs D, int sparse) {
   topo_tile(P, D, blockIdx.x, blockIdx.y, gridDim.x, gridDim.y, sparse);
 }
@@ -1942,6 +1945,7 @@ __device__ __forceinline__ void topo_cell(const DevParams& P, const DevPtrs& D, 
 // Every tile of the 3x3 neighbourhoods is done once: a listed tile by
 // itself, any other by the smallest listed tile next to it.  Persistent
 // grid, four wavefronts per workgroup as in topo_tile.
+template <int = 0>
 __global__ void __launch_bounds__(256) k_topo_list(DevParams P, DevPtrs D) {
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int tiles_x = (P.nx + kStripW - 1) / kStripW;
@@ -1973,6 +1977,7 @@ __global__ void __launch_bounds__(256) k_topo_list(DevParams P, DevPtrs D) {
 }
 
 // staggered grav_coeff (slope correction only), src/geometry_2d.f90:838-848
+template <int = 0>
 __global__ void k_gstag(DevParams P, DevPtrs D) {
   const int j = blockIdx.x * kTileX + (threadIdx.x & 63);
   const int k = blockIdx.y * kTileY + (threadIdx.x >> 6);
@@ -1995,6 +2000,7 @@ __global__ void k_gstag(DevParams P, DevPtrs D) {
 // ---------------------------------------------------------------------------
 // layout conversion: host Fortran (var, j, k) <-> device planes
 // ---------------------------------------------------------------------------
+template <int = 0>
 __global__ void k_aos_to_planes(DevParams P, const double* __restrict__ src, int nvar,
                                 double* __restrict__ dst) {
   const int j = blockIdx.x * kTileX + (threadIdx.x & 63);
@@ -2005,6 +2011,7 @@ __global__ void k_aos_to_planes(DevParams P, const double* __restrict__ src, int
 }
 
 // state flags of the current state (after an upload)
+template <int = 0>
 __global__ void k_sflag(DevParams P, DevPtrs D) {
   const int j = blockIdx.x * kTileX + (threadIdx.x & 63);
   const int k = blockIdx.y * kTileY + (threadIdx.x >> 6);
@@ -2015,6 +2022,7 @@ __global__ void k_sflag(DevParams P, DevPtrs D) {
   D.sflag[c] = state_flags(D.q[c], zero);
 }
 
+template <int = 0>
 __global__ void k_planes_to_aos(DevParams P, const double* __restrict__ src, int nvar,
                                 double* __restrict__ dst) {
   const int j = blockIdx.x * kTileX + (threadIdx.x & 63);

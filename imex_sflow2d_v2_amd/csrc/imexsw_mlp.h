@@ -178,6 +178,7 @@ __device__ __forceinline__ void input_operands(const short (*xs)[kMlpInPad], bf1
 // ---------------------------------------------------------------------------
 // layout check of the matrix-core operands: D = A(16x16) . B(16x16), both
 // row-major bf16, one wavefront
+template <int = 0>
 __global__ void __launch_bounds__(64) k_mfma_probe(const short* A, const short* B, float* D) {
   const int l = threadIdx.x, i = l & 15, g = l >> 4;
   const bf16x4 a{A[i * 16 + 4 * g], A[i * 16 + 4 * g + 1], A[i * 16 + 4 * g + 2],
@@ -191,6 +192,7 @@ __global__ void __launch_bounds__(64) k_mfma_probe(const short* A, const short* 
 }
 
 // inference on standardised inputs x[n][16] -> y[n][2] (standardised)
+template <int = 0>
 __global__ void __launch_bounds__(64) k_mlp_forward(MlpDev M, const float* x, int n, float* y) {
   __shared__ short xs[64][kMlpInPad];
   const int l = threadIdx.x;
@@ -224,6 +226,7 @@ struct MlpTrainBuf {
 // forward + backward of the activations for a minibatch x[n][16], t[n][2]
 // (standardised); loss = mean over samples of the squared error summed over
 // the two outputs.  One wavefront per 64 samples.
+template <int = 0>
 __global__ void __launch_bounds__(64) k_mlp_fwd_bwd(MlpDev M, MlpTrainBuf T, const float* x,
                                                     const float* t, int n) {
   __shared__ short xs[64][kMlpInPad];
@@ -320,6 +323,7 @@ __global__ void __launch_bounds__(64) k_mlp_fwd_bwd(MlpDev M, MlpTrainBuf T, con
 // Weight and bias gradients over the minibatch: one wavefront per 16x16
 // tile of a weight matrix, K = samples.  dW_l = dZ_l^T(out x samples) .
 // A_{l-1}(samples x in).  Tiles: W1 4, W2 16, W3 16, W4 4.
+template <int = 0>
 __global__ void __launch_bounds__(64) k_mlp_wgrad(MlpTrainBuf T, int n) {
   const int l = threadIdx.x, i = l & 15, g = l >> 4;
   int tile = blockIdx.x, layer, mb, nb;
@@ -361,6 +365,7 @@ __global__ void __launch_bounds__(64) k_mlp_wgrad(MlpTrainBuf T, int n) {
 
 // Adam on the fp32 master weights, bf16 copies refreshed.  Padded rows and
 // columns (inputs 9..15, outputs 2..15) have zero gradients and stay zero.
+template <int = 0>
 __global__ void k_mlp_adam(float* w, short* wb, const float* grad, float* m1, float* m2,
                            float lr, float b1, float b2, float eps, float c1, float c2) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
@@ -375,6 +380,7 @@ __global__ void k_mlp_adam(float* w, short* wb, const float* grad, float* m1, fl
   wb[p] = f2bf(v);
 }
 
+template <int = 0>
 __global__ void k_mlp_cast(const float* w, short* wb) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p < kMlpParams) wb[p] = f2bf(w[p]);

@@ -1,0 +1,107 @@
+// imexsw_ns.hip -- launches of the kernels templated on the number of solid
+// classes, compiled once per value with -DIMEXSW_NS=<0..6> so that the
+// instantiations build in parallel (the C ABI is in imexsw.hip).
+#include "imexsw_ctx.h"
+
+#ifndef IMEXSW_NS
+#error "compile with -DIMEXSW_NS=<number of solid classes>"
+#endif
+
+namespace imexsw {
+
+template <int NS>
+void launch_stencil(imexsw_ctx* c, int compute_dt, bool rows) {
+  // work items: (listed tile, strip) pairs, at most tiles * kTileH / row_unit
+  const int items = c->n_tiles * (kTileH / c->row_unit);
+  if (c->P.ny > 1 && rows) {
+    static int res = 0;
+    if (!res) res = resident_grid(k_stencil_rows<NS>, 384, c->n_cu, 1 << 30);
+    hipLaunchKernelGGL((k_stencil_rows<NS>), dim3(std::min(c->n_tiles * kTileH, res)),
+                       dim3(384), 0, c->stream, c->P, c->D, compute_dt);
+  } else if (c->P.ny > 1) {
+    static int res = 0;
+    if (!res) res = resident_grid(k_stencil<NS, true>, 64, c->n_cu, 1 << 30);
+    hipLaunchKernelGGL((k_stencil<NS, true>), dim3(std::min(items, res)), dim3(64), 0,
+                       c->stream, c->P, c->D, c->row_unit, c->strip_h, c->target_strips,
+                       compute_dt);
+  } else {
+    static int res = 0;
+    if (!res) res = resident_grid(k_stencil<NS, false>, 64, c->n_cu, 1 << 30);
+    hipLaunchKernelGGL((k_stencil<NS, false>), dim3(std::min(items, res)), dim3(64), 0,
+                       c->stream, c->P, c->D, c->row_unit, c->strip_h, c->target_strips,
+                       compute_dt);
+  }
+}
+template <int NS>
+void launch_local(imexsw_ctx* c, int ops, bool fused) {
+  // work items: listed tile rows, four per workgroup
+  const int items = (c->n_tiles * kTileH + 3) / 4;
+  if (fused) {
+    // row parts, four per workgroup
+    static int res3 = 0;
+    if (!res3) res3 = resident_grid(k_local<NS, 3>, 256, c->n_cu, 1 << 30);
+    hipLaunchKernelGGL((k_local<NS, 3>), dim3(std::min(kSparseLanes * items, res3)), dim3(256), 0,
+                       c->stream, c->P, c->D, ops & ~(int)OP_SUMS);
+    return;
+  }
+  static int res0 = 0, res1 = 0, res2 = 0;
+  if (!res0) res0 = resident_grid(k_local<NS, 0>, 256, c->n_cu, 1 << 30);
+  if (!res1) res1 = resident_grid(k_local<NS, 1>, 256, c->n_cu, 1 << 30);
+  if (!res2) res2 = resident_grid(k_local<NS, 2>, 256, c->n_cu, 1 << 30);
+  if (ops & OP_RK) {
+    int rk = OP_RK | (ops & OP_POLICY);
+    if (c->warm) {
+      static int resg = 0;
+      if (!resg) resg = resident_grid(k_mlp_guess<NS>, 64, c->n_cu, 1 << 30);
+      hipLaunchKernelGGL((k_mlp_guess<NS>), dim3(std::min(c->n_tiles * kTileH, resg)), dim3(64),
+                         0, c->stream, c->P, c->D, c->mlp, (ops & OP_POLICY) ? 1 : 0);
+      rk |= OP_WARM;
+    }
+    if (c->D.harvest) rk |= OP_HARVEST;
+    if (!(rk & (OP_WARM | OP_HARVEST)))
+      hipLaunchKernelGGL((k_local<NS, 0>), dim3(std::min(items, res0)), dim3(256), 0, c->stream,
+                         c->P, c->D, rk);
+    else
+      hipLaunchKernelGGL((k_local<NS, 2>), dim3(std::min(items, res2)), dim3(256), 0, c->stream,
+                         c->P, c->D, rk);
+  }
+  if (ops & (OP_EROSION | OP_POST | OP_SUMS))
+    hipLaunchKernelGGL((k_local<NS, 1>), dim3(std::min(items, res1)), dim3(256), 0, c->stream,
+                       c->P, c->D, ops & ~(int)OP_RK);
+}
+template <int NS>
+void launch_qp(imexsw_ctx* c, double* dst) {
+  hipLaunchKernelGGL((k_qp_out<NS>), c->grid_tiles, dim3(256), 0, c->stream, c->P, c->D.q, dst);
+}
+
+template <int NS>
+void launch_batch_stencil(imexsw_ensemble* e) {
+  const BatchLists L = batch_lists(e);
+  const int per = kTileH / e->strip_h;
+  static int rs = 0;
+  if (!rs) rs = resident_grid(kb_stencil<NS>, 64, e->n_cu, 1 << 30);
+  hipLaunchKernelGGL((kb_stencil<NS>), dim3(std::min(e->total_tiles * per, rs)), dim3(64), 0,
+                     e->stream, e->d_params, e->d_ptrs, L, e->strip_h);
+}
+template <int NS>
+void launch_batch_local(imexsw_ensemble* e) {
+  const BatchLists L = batch_lists(e);
+  static int rl0 = 0, rl1 = 0;
+  if (!rl0) rl0 = resident_grid(kb_local<NS, 0>, 256, e->n_cu, 1 << 30);
+  if (!rl1) rl1 = resident_grid(kb_local<NS, 1>, 256, e->n_cu, 1 << 30);
+  const int litems = (e->total_tiles * kTileH + 3) / 4;
+  hipLaunchKernelGGL((kb_local<NS, 0>), dim3(std::min(litems, rl0)), dim3(256), 0, e->stream,
+                     e->d_params, e->d_ptrs, L, (int)OP_RK);
+  hipLaunchKernelGGL((kb_local<NS, 1>), dim3(std::min(litems, rl1)), dim3(256), 0, e->stream,
+                     e->d_params, e->d_ptrs, L, (int)(OP_EROSION | OP_POST));
+}
+
+template void launch_stencil<IMEXSW_NS>(imexsw_ctx*, int, bool);
+template void launch_local<IMEXSW_NS>(imexsw_ctx*, int, bool);
+template void launch_qp<IMEXSW_NS>(imexsw_ctx*, double*);
+#if IMEXSW_NS >= 1 && IMEXSW_NS <= 3
+template void launch_batch_stencil<IMEXSW_NS>(imexsw_ensemble*);
+template void launch_batch_local<IMEXSW_NS>(imexsw_ensemble*);
+#endif
+
+}  // namespace imexsw
