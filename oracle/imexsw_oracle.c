@@ -1419,8 +1419,11 @@ void orc_get_source(const orc_t *o, int32_t *sc, uint8_t *bits, double *v) {
 /* ------------------------------------------------------------------------ */
 /* state I/O                                                                */
 /* ------------------------------------------------------------------------ */
+/* a new state starts a new run: the next orc_run takes its step history
+   from dt0 (src/IMEX_SfloW2D.f90:248-249) */
 void orc_set_q(orc_t *o, const double *q) {
   memcpy(o->q, q, (size_t)o->nv * o->nx * o->ny * 8);
+  o->dt_old = -1.0;
 }
 void orc_get_q(const orc_t *o, double *q) {
   memcpy(q, o->q, (size_t)o->nv * o->nx * o->ny * 8);
@@ -2863,12 +2866,15 @@ int orc_run(orc_t *o, int nsteps, double dt0, double *t_io, double t_end,
     o->dt_old_old = o->dt_old;
   }
   int s = 0;
-  while (s < nsteps && t < t_end) {
+  if (t_end - t_output < 1.0e-7)
+    t_output = t_end;
+  /* the loop pauses where the reference writes output (t >= t_output,
+     src/IMEX_SfloW2D.f90:510-527); the caller raises t_output and calls
+     again, as output_solution does (src/inpout_2d.f90:6166) */
+  while (s < nsteps && t < t_end && t < t_output) {
     orc_check_solve(o, 0);
     o->t = t;
     double dt = orc_timestep(o);
-    if (t_end - t_output < 1.0e-7)
-      t_output = t_end;
     if (t + dt > t_end)
       dt = t_end - t;
     if (t + dt > t_output)
