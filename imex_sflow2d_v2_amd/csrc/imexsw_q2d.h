@@ -19,8 +19,14 @@ namespace imexsw {
 // in 20 columns.  The digits are the correctly rounded 12 significant
 // digits of the binary value (printf %.11e gives d.ddddddddddd, which is the
 // same digit string with the exponent one smaller).
+// Not-a-number and infinities are written as gfortran writes them in a
+// 20-column E field: NaN, Infinity, -Infinity, right-justified.
 inline void fortran_e20_12(double v, char out[21]) {
-  char m[40];
+  if (!std::isfinite(v)) {
+    std::snprintf(out, 21, "%20s", std::isnan(v) ? "NaN" : (v < 0.0 ? "-Infinity" : "Infinity"));
+    return;
+  }
+  char m[40] = {0};
   std::snprintf(m, sizeof m, "%.11e", std::fabs(v));
   // m = "d.ddddddddddde[+-]XX[X]"
   char digits[13];

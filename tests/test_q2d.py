@@ -104,3 +104,15 @@ def test_reader_errors(tmp_path):
     bad.write_text(" 1.0e+00" + " " * 12 + "  x.0000000000000e+00\n  \n")
     with pytest.raises(ImexswError):
         q2d_read(str(bad), 5, 1, 1, 1)
+
+
+def test_writer_non_finite_values(tmp_path):
+    """NaN and infinities (a state the library reports through its event
+    counters) are written as gfortran writes them in E20.12 and read back."""
+    lines = _record(tmp_path, [float("nan"), float("inf"), -float("inf"), 1.5])
+    fields = [lines[0][i:i + 20] for i in range(0, len(lines[0]), 20)]
+    assert fields[2:] == ["                 NaN", "            Infinity",
+                          "           -Infinity", "  0.150000000000E+01"]
+    q = q2d_read(str(tmp_path / "one.q_2d"), 4, 0, 1, 1)
+    assert np.isnan(q[0, 0, 0]) and q[1, 0, 0] == np.inf and q[2, 0, 0] == -np.inf
+    assert q[3, 0, 0] == 1.5
