@@ -27,6 +27,7 @@ thread_local std::string g_create_error;
 
 }  // namespace
 
+
 namespace {
 
 int fail(imexsw_ctx* c, const char* fmt, ...) {
@@ -119,14 +120,23 @@ int check_launch(imexsw_ctx* c, const char* what) {
     default: CALL(6); break;       \
   }
 
-int run_stencil(imexsw_ctx* c, int compute_dt, bool rows = false) {
+// The stencil pass: CFL minimum (compute_dt) and divFlux - expl_terms of the
+// state.  With N_RK != 2 the CFL pass writes its residual into the first
+// stage's planes; stage >= 0 runs on that stage's state instead of q.
+int run_stencil(imexsw_ctx* c, int compute_dt, bool rows = false, int stage = -1) {
+  DevPtrs D = c->D;
+  if (c->general) {
+    const size_t sp = (size_t)c->nv * c->plane;
+    D.D = c->G.Dx + (size_t)std::max(stage, 0) * sp;
+    if (stage >= 0) D.q = c->G.qrk + (size_t)stage * sp;
+  }
   {
     Timed tm(c, K_STENCIL);
-#define CALL(N) launch_stencil<N>(c, compute_dt, rows)
+#define CALL(N) launch_stencil<N>(c, D, compute_dt, rows)
     NS_SWITCH(c->ns, CALL)
 #undef CALL
   }
-  c->d_valid = true;
+  if (stage < 0) c->d_valid = true;
   return check_launch(c, "k_stencil");
 }
 
@@ -140,8 +150,29 @@ int run_local(imexsw_ctx* c, int ops, int kid, bool fused = false) {
   return check_launch(c, "k_local");
 }
 
+// check_solve for N_RK != 2: seeds, N_RK dilations, the final pass
+int run_mask_general(imexsw_ctx* c, int solve_all) {
+  {
+    Timed tm(c, K_MASK);
+    const int grid = (int)std::min<size_t>((c->plane + 255) / 256, (size_t)c->n_cu * 8);
+    hipLaunchKernelGGL(kg_mask_seed<>, dim3(grid), dim3(256), 0, c->stream, c->P, c->D,
+                       c->G.ma, solve_all);
+    uint8_t *a = c->G.ma, *b = c->G.mb;
+    for (int i = 0; i < c->P.nrk; i++) {
+      hipLaunchKernelGGL(kg_mask_dilate<>, dim3(grid), dim3(256), 0, c->stream, c->P, c->D.scal,
+                         a, b);
+      std::swap(a, b);
+    }
+    const dim3 fgrid(c->grid_mask.x, (c->grid_mask.y + kMaskGroup - 1) / kMaskGroup);
+    hipLaunchKernelGGL(kg_mask_final<>, fgrid, dim3(64 * kMaskGroup), 0, c->stream, c->P, c->D,
+                       a);
+  }
+  return check_launch(c, "kg_mask");
+}
+
 // partial: rescan only the dirty tiles (the step loop)
 int run_mask(imexsw_ctx* c, int solve_all, int partial = 0) {
+  if (c->general) return run_mask_general(c, solve_all);
   {
     Timed tm(c, K_MASK);
     const dim3 grid(c->grid_mask.x, (c->grid_mask.y + kMaskGroup - 1) / kMaskGroup);
@@ -170,6 +201,41 @@ int run_topo(imexsw_ctx* c, int sparse) {
   return check_launch(c, "k_topo");
 }
 
+// imex_RK_solver for N_RK != 2 (imexsw_stages.h), after the CFL pass has
+// written the residual of q^n into the first stage's planes: the stages,
+// each with its flux pass when omega_tilde > 0 (the first stage's is the CFL
+// pass's when that stage is explicit, its state then being q^n), and the
+// assembly.  policy: dt from the time-step policy, else the scalar dt.
+int run_stages(imexsw_ctx* c, int policy) {
+  const RkTableau& T = c->tab;
+  for (int st = 0; st < T.nrk; st++) {
+    {
+      Timed tm(c, K_LOCAL);
+#define CALL(N) launch_stage<N>(c, st, policy)
+      NS_SWITCH(c->ns, CALL)
+#undef CALL
+    }
+    if (check_launch(c, "kg_stage")) return -1;
+    const bool same_as_cfl_pass = st == 0 && T.a_dirk[0][0] == 0.0;
+    if (T.omega_tilde[st] > 0.0 && !same_as_cfl_pass && run_stencil(c, 0, false, st)) return -1;
+  }
+  {
+    Timed tm(c, K_LOCAL);
+#define CALL(N) launch_assemble<N>(c, policy)
+    NS_SWITCH(c->ns, CALL)
+#undef CALL
+  }
+  return check_launch(c, "kg_assemble");
+}
+
+// every stage state starts as q (what inactive cells keep)
+int init_stages(imexsw_ctx* c) {
+  const int grid = (int)std::min<size_t>((c->plane * c->nv + 255) / 256, (size_t)c->n_cu * 8);
+  hipLaunchKernelGGL(kg_stage_init<>, dim3(grid), dim3(256), 0, c->stream, c->P, c->D.scal,
+                     c->D.q, c->G.qrk, c->tab.nrk);
+  return check_launch(c, "kg_stage_init");
+}
+
 // one time step of the main loop (src/IMEX_SfloW2D.f90:337-531), all on the
 // context's stream.  The stage kernel applies the time-step policy itself;
 // k_step_begin of the next step (commit_prev) or end_steps commits it and
@@ -179,6 +245,14 @@ int enqueue_step(imexsw_ctx* c, bool commit_prev) {
     Timed tm(c, K_DTPOLICY);
     hipLaunchKernelGGL(k_step_begin<>, dim3(1), dim3(1), 0, c->stream, c->P, c->D,
                        commit_prev ? 1 : 0, 1);
+  }
+  if (c->general) {
+    if (run_mask(c, 0) || init_stages(c)) return -1;
+    if (run_stencil(c, c->P.cfl != -1.0)) return -1;
+    if (run_stages(c, 1)) return -1;
+    if (run_local(c, OP_EROSION | OP_POST | OP_POLICY, K_LOCAL)) return -1;
+    if (c->P.topo && run_topo(c, 1)) return -1;
+    return check_launch(c, "k_step_begin");
   }
   if (run_mask(c, 0, 1)) return -1;
   if (run_stencil(c, c->P.cfl != -1.0, c->sparse)) return -1;
@@ -204,15 +278,86 @@ void drop_ctx_graph(imexsw_ctx* c) {
 // strip-height rule of k_stencil applied to w computed cells) and the
 // learned warm start is off.
 bool want_sparse(const imexsw_ctx* c, unsigned long long w) {
-  if (c->warm || c->D.harvest || c->P.ny < 2) return false;
+  if (c->general || c->warm || c->D.harvest || c->P.ny < 2) return false;
   return c->strip_h < 2 ||
          w / (unsigned long long)(kStripW * 2) < (unsigned long long)c->target_strips;
+}
+
+// Capture kGraphSteps steps of the sparse (m = 1) or dense (m = 0) form
+// into a graph and instantiate it.
+int capture_steps(imexsw_ctx* c, int m) {
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+  const bool sparse = c->sparse;
+  c->sparse = (m == 1);
+  int rc = 0;
+  for (int i = 0; i < kGraphSteps && !rc; i++) rc = enqueue_step(c, i > 0);
+  if (!rc) rc = end_steps(c);
+  c->sparse = sparse;
+  hipGraph_t g = nullptr;
+  hipError_t err = hipStreamEndCapture(c->stream, &g);
+  if (rc) {
+    if (g) (void)hipGraphDestroy(g);
+    return -1;
+  }
+  if (err != hipSuccess) return fail(c, "hipStreamEndCapture: %s", hipGetErrorString(err));
+  c->graph[m] = g;
+  HIPCHK(c, hipGraphInstantiate(&c->exec[m], g, nullptr, nullptr, 0));
+  return 0;
 }
 
 unsigned long long sum_slots(const unsigned long long* v) {
   unsigned long long s = 0;
   for (int i = 0; i < kSlots; i++) s += v[i];
   return s;
+}
+
+// The IMEX Runge-Kutta tableaux of allocate_solver_variables
+// (src/solver_2d.f90:398-486): a_tilde_ij and omega_tilde of the explicit
+// part, a_dirk_ij and omega of the implicit part.  N_RK = 3 is
+// IMEX-SSP(3,3,2) (Pareschi & Russo 2005, Table IV), N_RK = 4 LRR(3,2,2)
+// (Pareschi & Russo 2000, Table 3).  The assembly takes the last stage when
+// the weights equal the last rows (src/solver_2d.f90:1341-1377).
+void fill_tableau(int nrk, RkTableau& T) {
+  std::memset(&T, 0, sizeof T);
+  T.nrk = nrk;
+  if (nrk == 1) {
+    T.a_tilde[0][0] = 1.0;
+    T.omega_tilde[0] = 1.0;
+  } else if (nrk == 2) {
+    T.a_tilde[1][0] = 1.0;
+    T.omega_tilde[0] = 1.0;
+    T.a_dirk[1][1] = 1.0;
+    T.omega[1] = 1.0;
+  } else if (nrk == 3) {
+    T.a_tilde[1][0] = 0.5;
+    T.a_tilde[2][0] = 0.5;
+    T.a_tilde[2][1] = 0.5;
+    for (int r = 0; r < 3; r++) T.omega_tilde[r] = 1.0 / 3.0;
+    T.a_dirk[0][0] = 0.25;
+    T.a_dirk[1][1] = 0.25;
+    T.a_dirk[2][0] = 1.0 / 3.0;
+    T.a_dirk[2][1] = 1.0 / 3.0;
+    T.a_dirk[2][2] = 1.0 / 3.0;
+    for (int r = 0; r < 3; r++) T.omega[r] = 1.0 / 3.0;
+  } else if (nrk == 4) {
+    T.a_tilde[1][0] = 0.5;
+    T.a_tilde[2][0] = 1.0 / 3.0;
+    T.a_tilde[3][1] = 1.0;
+    T.omega_tilde[1] = 1.0;
+    T.a_dirk[1][1] = 0.5;
+    T.a_dirk[2][2] = 1.0 / 3.0;
+    T.a_dirk[3][2] = 0.75;
+    T.a_dirk[3][3] = 0.25;
+    T.omega[2] = 0.75;
+    T.omega[3] = 0.25;
+  }
+  double d1 = 0.0, d2 = 0.0;
+  for (int r = 0; r < nrk; r++) {
+    d1 += std::fabs(T.omega_tilde[r] - T.a_tilde[nrk - 1][r]);
+    d2 += std::fabs(T.omega[r] - T.a_dirk[nrk - 1][r]);
+  }
+  T.stiff = (d1 == 0.0 && d2 == 0.0) ? 1 : 0;
 }
 
 int set_scalar(imexsw_ctx* c, int slot, double v) {
@@ -251,7 +396,8 @@ int upload_aos(imexsw_ctx* c, const double* host, int nvar, double* dst) {
 int download_aos(imexsw_ctx* c, const double* src, int nvar, double* host) {
   const size_t n = (size_t)nvar * c->P.nx * c->P.ny;
   if (ensure_stage(c, n * sizeof(double))) return -1;
-  hipLaunchKernelGGL(k_planes_to_aos<>, c->grid_tiles, dim3(256), 0, c->stream, c->P, src, nvar, c->stage);
+  hipLaunchKernelGGL(k_planes_to_aos<>, c->grid_tiles, dim3(256), 0, c->stream, c->P, src, nvar,
+                     c->stage, 0, c->P.ny);
   if (check_launch(c, "k_planes_to_aos")) return -1;
   HIPCHK(c, hipMemcpyAsync(host, c->stage, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -290,8 +436,7 @@ int imexsw_create(const imexsw_params* params, imexsw_ctx** out) {
   const imexsw_params& p = *params;
   if (p.nx < 2 || p.ny < 1) return fail(nullptr, "grid must have nx >= 2 and ny >= 1");
   if (p.n_solid < 0 || p.n_solid > IMEXSW_MAX_SOLID) return fail(nullptr, "n_solid out of range");
-  if (p.n_rk != 2)
-    return fail(nullptr, "the device path implements N_RK = 2 (IMEX-Euler, the tableau of every shipped deck)");
+  if (p.n_rk < 1 || p.n_rk > kMaxRK) return fail(nullptr, "N_RK must be 1, 2, 3 or 4 (got %d)", p.n_rk);
   if (p.rheology_flag && p.rheology_model != 1)
     return fail(nullptr, "rheology model %d is not supported (only 1, Voellmy-Salm)", p.rheology_model);
   if (p.n_thickness_levels < 0 || p.n_dyn_pres_levels < 0 ||
@@ -434,6 +579,18 @@ int imexsw_create(const imexsw_params* params, imexsw_ctx** out) {
   c->grid_mask = dim3((P.nx + kStripW - 1) / kStripW, (P.ny + kTileH - 1) / kTileH);
   c->grid_stag = dim3((P.nx + 1 + kTileX - 1) / kTileX, (P.ny + 1 + kTileY - 1) / kTileY);
   c->n_tiles = (int)(c->grid_mask.x * c->grid_mask.y);
+  if (p.n_rk != 2) {
+    // the tableau-driven stage pipeline: stage states, residuals and
+    // implicit terms of every stage, two byte planes for the mask dilation
+    c->general = true;
+    fill_tableau(p.n_rk, c->tab);
+    const size_t sp = (size_t)c->nv * pl * p.n_rk;
+    rc |= dalloc(c, &c->G.qrk, sp);
+    rc |= dalloc(c, &c->G.Dx, sp);
+    rc |= dalloc(c, &c->G.Im, sp);
+    rc |= dalloc(c, &c->G.ma, pl);
+    rc |= dalloc(c, &c->G.mb, pl);
+  }
   rc |= dalloc(c, &D.tile_sums, 4 * (size_t)c->n_tiles * kTileH);
   rc |= dalloc(c, &D.tile_flag, (size_t)c->n_tiles);
   rc |= dalloc(c, &D.tile_dirty, (size_t)c->n_tiles);
@@ -444,10 +601,12 @@ int imexsw_create(const imexsw_params* params, imexsw_ctx** out) {
   c->row_unit = 1;
   {
     int dev = 0, ncu = 0;
-    if (hipGetDevice(&dev) == hipSuccess &&
-        hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
-        ncu > 0)
-      c->n_cu = ncu;
+    if (hipGetDevice(&dev) == hipSuccess) {
+      c->device = dev;
+      if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
+          ncu > 0)
+        c->n_cu = ncu;
+    }
   }
   if (rc) {
     g_create_error = c->err;
@@ -473,6 +632,7 @@ void imexsw_destroy(imexsw_ctx* c) {
   for (hipEvent_t e : c->cnt_ev)
     if (e) (void)hipEventDestroy(e);
   if (c->h_cnt) (void)hipHostFree(c->h_cnt);
+  if (c->h_done) (void)hipHostFree(c->h_done);
   if (c->D.harvest) (void)hipFree(c->D.harvest);
   for (void* p : c->allocs) (void)hipFree(p);
   if (c->stage) (void)hipFree(c->stage);
@@ -584,6 +744,9 @@ int imexsw_upload_q(imexsw_ctx* c, const double* q) {
   HIPCHK(c, hipMemsetAsync(c->D.tile_dirty, 1, (size_t)c->n_tiles, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   c->d_valid = false;
+  // a new state starts a new run: the next imexsw_advance takes its step
+  // history from dt0 again
+  c->dt_hist = false;
   return 0;
 }
 
@@ -627,6 +790,9 @@ int imexsw_check_solve(imexsw_ctx* c, int solve_all, double t, int32_t* solve_ce
 
 int imexsw_timestep(imexsw_ctx* c, double* dt) {
   if (!c || !dt) return fail(c, "null argument");
+  // the procedures called one at a time run whatever state imexsw_advance
+  // stopped in
+  if (set_scalar(c, S_DONE, 0.0)) return -1;
   const unsigned long long init = (unsigned long long)(*(const long long*)&c->P.max_dt);
   HIPCHK(c, hipMemcpyAsync(c->D.dt_bits, &init, sizeof init, hipMemcpyHostToDevice, c->stream));
   if (run_stencil(c, c->P.cfl != -1.0)) return -1;
@@ -643,8 +809,14 @@ int imexsw_rk_step(imexsw_ctx* c, double t, double dt) {
   if (!c) return fail(c, "null argument");
   if (!(dt > 0.0)) return fail(c, "dt must be positive");
   if (set_scalar(c, S_T, t) || set_scalar(c, S_DT, dt) || set_scalar(c, S_DONE, 0.0)) return -1;
-  if (!c->d_valid && run_stencil(c, 0)) return -1;
-  if (run_local(c, OP_RK, K_LOCAL)) return -1;
+  if (c->general) {
+    if (init_stages(c)) return -1;
+    if (!c->d_valid && run_stencil(c, 0)) return -1;
+    if (run_stages(c, 0)) return -1;
+  } else {
+    if (!c->d_valid && run_stencil(c, 0)) return -1;
+    if (run_local(c, OP_RK, K_LOCAL)) return -1;
+  }
   c->d_valid = false;
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return 0;
@@ -652,7 +824,7 @@ int imexsw_rk_step(imexsw_ctx* c, double t, double dt) {
 
 int imexsw_erosion_deposition(imexsw_ctx* c, double dt) {
   if (!c) return fail(c, "null argument");
-  if (set_scalar(c, S_DT, dt)) return -1;
+  if (set_scalar(c, S_DT, dt) || set_scalar(c, S_DONE, 0.0)) return -1;
   if (run_local(c, OP_EROSION, K_EROSION)) return -1;
   c->d_valid = false;
   HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -669,6 +841,7 @@ int imexsw_topography_reconstruction(imexsw_ctx* c) {
 
 int imexsw_post_step(imexsw_ctx* c, double sums[4]) {
   if (!c) return fail(c, "null argument");
+  if (set_scalar(c, S_DONE, 0.0)) return -1;
   if (sums)
     HIPCHK(c, hipMemsetAsync(c->D.tile_sums, 0, 4 * (size_t)c->n_tiles * kTileH * sizeof(double),
                              c->stream));
@@ -704,8 +877,19 @@ int imexsw_advance(imexsw_ctx* c, int32_t nsteps, double dt0, double* t_io, doub
     c->dt_hist = true;
   }
   HIPCHK(c, hipMemcpyAsync(c->D.scal, s, sizeof s, hipMemcpyHostToDevice, c->stream));
+  {
+    // already at the output time (or the end): no step, nothing launched
+    double t_out = t_output;
+    if (t_end - t_out < 1.0e-7) t_out = t_end;
+    if (!(*t_io < t_end) || !(*t_io < t_out) || nsteps <= 0) {
+      HIPCHK(c, hipStreamSynchronize(c->stream));
+      if (steps_done) *steps_done = 0;
+      return 0;
+    }
+  }
   if (!c->h_cnt) {
     HIPCHK(c, hipHostMalloc((void**)&c->h_cnt, 2 * kSlots * sizeof(unsigned long long)));
+    HIPCHK(c, hipHostMalloc((void**)&c->h_done, 2 * sizeof(double)));
     for (hipEvent_t& e : c->cnt_ev) HIPCHK(c, hipEventCreateWithFlags(&e, hipEventDisableTiming));
   }
   const unsigned long long* d_cnt = c->D.counters + C_COMPUTE_CELLS * kSlots;
@@ -720,9 +904,13 @@ int imexsw_advance(imexsw_ctx* c, int32_t nsteps, double dt0, double* t_io, doub
   // launches per step); the rest, and every step when timing, is launched
   // directly.  The graphs are recaptured when the context's constants or
   // pointers changed since the capture.  Each replay is followed by a copy of
-  // its last step's computed-cell count, which decides between sparse and
-  // dense steps two replays later (so the device always has a replay queued).
+  // its last step's computed-cell count and of the done flag: the count
+  // decides between sparse and dense steps two replays later (so the device
+  // always has a replay queued), and the flag ends the call once the run has
+  // reached t_output or t_end (the steps after that are no-ops on the
+  // device).
   int n = 0;
+  bool stop = false;
   if (!c->timing && nsteps >= kGraphSteps) {
     if (std::memcmp(&c->graph_P, &c->P, sizeof(DevParams)) ||
         std::memcmp(&c->graph_D, &c->D, sizeof(DevPtrs)) ||
@@ -738,33 +926,31 @@ int imexsw_advance(imexsw_ctx* c, int32_t nsteps, double dt0, double* t_io, doub
     for (int r = 0; n + kGraphSteps <= nsteps; n += kGraphSteps, r++) {
       if (r >= 2) {
         HIPCHK(c, hipEventSynchronize(c->cnt_ev[r & 1]));
+        if (c->h_done[r & 1] != 0.0) {
+          stop = true;
+          break;
+        }
         c->sparse = want_sparse(c, sum_slots(c->h_cnt + (r & 1) * kSlots));
       }
       const int m = c->sparse ? 1 : 0;
-      if (!c->exec[m]) {
-        HIPCHK(c, hipStreamSynchronize(c->stream));
-        HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-        int rc = 0;
-        for (int i = 0; i < kGraphSteps && !rc; i++) rc = enqueue_step(c, i > 0);
-        if (!rc) rc = end_steps(c);
-        hipGraph_t g = nullptr;
-        hipError_t err = hipStreamEndCapture(c->stream, &g);
-        if (rc) {
-          if (g) (void)hipGraphDestroy(g);
-          return -1;
-        }
-        if (err != hipSuccess) return fail(c, "hipStreamEndCapture: %s", hipGetErrorString(err));
-        c->graph[m] = g;
-        HIPCHK(c, hipGraphInstantiate(&c->exec[m], g, nullptr, nullptr, 0));
-      }
+      if (!c->exec[m] && capture_steps(c, m)) return -1;
       HIPCHK(c, hipGraphLaunch(c->exec[m], c->stream));
       if (c->sparse) c->sparse_steps += kGraphSteps;
       HIPCHK(c, hipMemcpyAsync(c->h_cnt + (r & 1) * kSlots, d_cnt, kSlots * sizeof(unsigned long long),
                                hipMemcpyDeviceToHost, c->stream));
+      HIPCHK(c, hipMemcpyAsync(c->h_done + (r & 1), c->D.scal + S_DONE, sizeof(double),
+                               hipMemcpyDeviceToHost, c->stream));
       HIPCHK(c, hipEventRecord(c->cnt_ev[r & 1], c->stream));
     }
   }
-  for (; n < nsteps; n++) {
+  for (int i = 0; !stop && n < nsteps; n++, i++) {
+    if (i > 0 && i % kGraphSteps == 0) {
+      double done = 0.0;
+      HIPCHK(c, hipMemcpyAsync(&done, c->D.scal + S_DONE, sizeof done, hipMemcpyDeviceToHost,
+                               c->stream));
+      HIPCHK(c, hipStreamSynchronize(c->stream));
+      if (done != 0.0) break;
+    }
     if (enqueue_step(c, false) || end_steps(c)) return -1;
     if (c->sparse) c->sparse_steps++;
   }
@@ -786,7 +972,7 @@ int imexsw_download(imexsw_ctx* c, double* q, double* qp, double* hmax, double* 
   if (qp) {
     const size_t n = (size_t)(c->nv + 2) * P.nx * P.ny;
     if (ensure_stage(c, n * sizeof(double))) return -1;
-#define CALL(N) launch_qp<N>(c, c->stage)
+#define CALL(N) launch_qp<N>(c, c->stage, 0, P.ny)
     NS_SWITCH(c->ns, CALL)
 #undef CALL
     if (check_launch(c, "k_qp_out")) return -1;
@@ -1124,6 +1310,7 @@ int imexsw_ensemble_create(int32_t n_members, const imexsw_params* params,
     const imexsw_params& p = params[m];
     if (p.n_solid != ns) return efail(nullptr, "member %d: all members need the same n_solid", m);
     if (p.ny < 2) return efail(nullptr, "member %d: ensembles need 2-D grids", m);
+    if (p.n_rk != 2) return efail(nullptr, "member %d: ensembles run N_RK = 2", m);
     if (p.topo_change_flag || p.radial_source_flag || p.slope_correction_flag || p.curvature_term_flag)
       return efail(nullptr, "member %d: ensembles run the flat-source-free model (no topography "
                             "change, radial source, slope correction or curvature)", m);
@@ -1634,6 +1821,7 @@ int imexsw_mfma_probe(const uint16_t* A, const uint16_t* B, float* D) {
 // appended to a device buffer of `capacity` pairs (0 stops and frees)
 int imexsw_harvest(imexsw_ctx* c, int64_t capacity) {
   if (!c || capacity < 0) return fail(c, "bad argument");
+  if (c->general && capacity > 0) return fail(c, "the harvest runs with N_RK = 2 only");
   HIPCHK(c, hipStreamSynchronize(c->stream));
   if (c->D.harvest) {
     (void)hipFree(c->D.harvest);
@@ -1670,6 +1858,7 @@ int imexsw_set_warm_start(imexsw_ctx* c, imexsw_mlp* m) {
     c->warm = false;
     return 0;
   }
+  if (c->general) return fail(c, "the learned warm start runs with N_RK = 2 only");
   if (!c->D.guess && dalloc(c, &c->D.guess, 2 * c->plane)) return -1;
   c->mlp = mlp_dev(m);
   c->warm = true;

@@ -14,10 +14,13 @@
 
 #include "imexsw_batch.h"
 #include "imexsw_kernels.h"
+#include "imexsw_stages.h"
 
 using imexsw::DevParams;
 using imexsw::DevPtrs;
 using imexsw::MlpDev;
+using imexsw::RkPtrs;
+using imexsw::RkTableau;
 
 enum KernelId { K_MASK = 0, K_STENCIL, K_LOCAL, K_TOPO, K_DTPOLICY, K_POST, K_EROSION, K_NK };
 
@@ -34,9 +37,14 @@ struct imexsw_ctx {
   dim3 grid_tiles, grid_mask, grid_stag;
   int n_tiles = 0;  // work tiles (kStripW x kTileH)
   int n_cu = 256;   // compute units of the device
+  int device = 0;   // the HIP device current at imexsw_create
   int strip_h = 16;         // tallest stencil strip (rows)
   int row_unit = 1;         // shortest stencil strip (rows)
   int target_strips = 2048; // strips wanted in flight
+  // N_RK != 2: the tableau-driven stage pipeline and its stage planes
+  bool general = false;
+  RkTableau tab{};
+  RkPtrs G{};
   bool d_valid = false;   // D holds the residual of the current q
   bool dt_hist = false;   // dt_old / dt_old_old initialised
   bool timing = false;
@@ -54,6 +62,7 @@ struct imexsw_ctx {
   bool sparse = false;
   int64_t sparse_steps = 0;
   unsigned long long* h_cnt = nullptr;  // pinned: 2 x kSlots computed-cell counters
+  double* h_done = nullptr;             // pinned: 2 copies of the run's done flag
   hipEvent_t cnt_ev[2] = {nullptr, nullptr};
   // captured kGraphSteps-step graphs of imexsw_advance (dense, sparse) and
   // what they captured
@@ -112,15 +121,23 @@ inline BatchLists batch_lists(imexsw_ensemble* e) {
 
 // Launches of the kernels templated on the number of solid classes NS,
 // defined in imexsw_ns.hip (one object per NS).
-// rows: the six-wavefront row stencil (sparse steps); else strips
+// rows: the six-wavefront row stencil (sparse steps); else strips.  D: the
+// context's pointers, or a copy whose state and residual planes are a
+// stage's (N_RK != 2)
 template <int NS>
-void launch_stencil(imexsw_ctx* c, int compute_dt, bool rows);
+void launch_stencil(imexsw_ctx* c, const DevPtrs& D, int compute_dt, bool rows);
 // fused: stage update, erosion and diagnostics of a row in one pass (sparse
 // steps without the learned warm start); else up to two kernels
 template <int NS>
 void launch_local(imexsw_ctx* c, int ops, bool fused);
+// qp of rows k0 .. k1-1 into dst in the host layout
 template <int NS>
-void launch_qp(imexsw_ctx* c, double* dst);
+void launch_qp(imexsw_ctx* c, double* dst, int k0, int k1);
+// N_RK != 2: stage `st` of the tableau and the assembly (imexsw_stages.h)
+template <int NS>
+void launch_stage(imexsw_ctx* c, int st, int policy);
+template <int NS>
+void launch_assemble(imexsw_ctx* c, int policy);
 // the ensemble's batched stencil and stage/erosion passes (NS = 1..3)
 template <int NS>
 void launch_batch_stencil(imexsw_ensemble* e);

@@ -351,6 +351,9 @@ __device__ __forceinline__ void append_group(const ListSink& L, int tile, unsign
 template <int = 0>
 __global__ void __launch_bounds__(64 * kMaskGroup) k_mask(DevParams P, DevPtrs D,
                                                           int solve_all, int partial) {
+  // a run that has stopped (t_output or t_end reached) keeps the mask of
+  // its last step, the solve_mask the reference leaves behind
+  if (D.scal[S_DONE] != 0.0) return;
   const int tiles_x = (P.nx + kStripW - 1) / kStripW;
   const int tiles_y = (P.ny + kTileH - 1) / kTileH;
   const int w = threadIdx.x >> 6;
@@ -1338,6 +1341,9 @@ __device__ __forceinline__ StepDt step_dt(const DevParams& P, const DevPtrs& D) 
   double dt = (P.cfl != -1.0) ? __longlong_as_double((long long)*D.dt_bits) : P.max_dt;
   r.dt_cfl = dt;
   if (t_end - t_out < 1.0e-7) t_out = t_end;
+  // paused at an output time: the host writes its output and raises t_output
+  // before the next call (src/IMEX_SfloW2D.f90:510-527, :6166)
+  if (!(t < t_out)) return r;
   if (t + dt > t_end) dt = t_end - t;
   if (t + dt > t_out) dt = t_out - t;
   dt = fmin(dt, 1.1 * 0.5 * (s[S_DT_OLD] + s[S_DT_OLD_OLD]));
@@ -1374,10 +1380,15 @@ __device__ __forceinline__ void advance_member(const DevPtrs& D) {
   *ctr(D, C_STEPS, 0) += 1;
   *ctr(D, C_ACTIVE_UPD, 0) += n;
   *ctr(D, C_COMPUTE_UPD, 0) += nc;
+  // the run stops once t reaches the output time (t_output, snapped to t_end
+  // by the policy), where the reference's loop writes its output
+  if (s[S_T] >= s[S_T_OUT]) s[S_DONE] = 1.0;
 }
 
-// CFL accumulator and per-step counters / work lists
+// CFL accumulator and per-step counters / work lists (a stopped run keeps
+// the last step's, for the procedures the host calls at the pause)
 __device__ __forceinline__ void reset_member(const DevParams& P, const DevPtrs& D) {
+  if (D.scal[S_DONE] != 0.0) return;
   *D.dt_bits = (unsigned long long)__double_as_longlong(P.max_dt);
   for (int i = 0; i < kSlots; i++) *ctr(D, C_COMPUTE_CELLS, i) = 0;
   *ctr(D, C_TILES, 0) = 0;
@@ -1406,6 +1417,64 @@ __global__ void k_step_begin(DevParams P, DevPtrs D, int commit, int reset) {
 // step), else the scalar S_DT
 enum LocalOps { OP_RK = 1, OP_EROSION = 2, OP_POST = 4, OP_SUMS = 8, OP_WARM = 16,
                 OP_HARVEST = 32, OP_POLICY = 64 };
+
+// The assembly checks of imex_RK_solver (src/solver_2d.f90:1379-1624) on
+// the assembled state q of cell c: NaN trap, small or negative thickness,
+// negative solids, low temperature, solids above the total; qp receives the
+// physical variables.  Events count into e (C_NEWTON_CELLS-relative) when
+// `counts`.
+template <int NS>
+__device__ __forceinline__ void assembly_checks(const DevParams& P, const DevPtrs& D, size_t c,
+                                                double* q, double* qp, unsigned long long* e,
+                                                bool counts) {
+  constexpr int NV = 4 + NS;
+  bool isn = false;
+#pragma unroll
+  for (int i = 0; i < NV; i++) isn |= isnan(q[i]);
+  if (counts && isn) atomicAdd(&e[C_NAN - C_NEWTON_CELLS], 1ull);
+  if (q[0] < kEps) {
+    if (q[0] > -1.0e-7) {
+#pragma unroll
+      for (int i = 0; i < NV; i++) q[i] = 0.0;
+    } else {
+      if (counts) atomicAdd(&e[C_NEGH - C_NEWTON_CELLS], 1ull);
+    }
+  }
+  {
+    bool anyneg = false, anybig = false;
+#pragma unroll
+    for (int i = 0; i < NS; i++) {
+      anyneg |= q[4 + i] < 0.0;
+      anybig |= q[4 + i] <= -1.0e-7;
+    }
+    if (anyneg) {
+      if (anybig) {
+        if (counts) atomicAdd(&e[C_NEGS - C_NEWTON_CELLS], 1ull);
+      } else {
+#pragma unroll
+        for (int i = 0; i < NS; i++)
+          if (q[4 + i] > -1.0e-7) q[4 + i] = 0.0;
+      }
+    }
+  }
+  phys<NS>(P, q, qp);
+  if (counts && qp[3] < 273.0) atomicAdd(&e[C_LOWT - C_NEWTON_CELLS], 1ull);
+  double ssol = 0.0;
+#pragma unroll
+  for (int i = 0; i < NS; i++) ssol += q[4 + i];
+  if (ssol > q[0]) {
+    if (((ssol - q[0]) / q[0] < 1.0e-10) || (q[0] < kEps)) {
+#pragma unroll
+      for (int i = 0; i < NS; i++) q[4 + i] = q[4 + i] / ssol * q[0];
+    } else {
+      if (counts) atomicAdd(&e[C_SUMS - C_NEWTON_CELLS], 1ull);
+    }
+  }
+  if (D.nodata && D.nodata[c]) {
+#pragma unroll
+    for (int i = 0; i < NV; i++) q[i] = 0.0;
+  }
+}
 
 // one cell of k_local; e (the wavefront's counters in LDS) accumulates
 // counter C_NEWTON_CELLS + m in e[m]
@@ -1538,52 +1607,7 @@ __device__ __forceinline__ void local_cell(const DevParams& P, const DevPtrs& D,
       atomicAdd(&e[C_HIST0 - C_NEWTON_CELLS + b], 1ull);
       imax = max(imax, it);
     }
-    // assembly checks (:1379-1624)
-    bool isn = false;
-#pragma unroll
-    for (int i = 0; i < NV; i++) isn |= isnan(q[i]);
-    if (counts && isn) atomicAdd(&e[C_NAN - C_NEWTON_CELLS], 1ull);
-    if (q[0] < kEps) {
-      if (q[0] > -1.0e-7) {
-#pragma unroll
-        for (int i = 0; i < NV; i++) q[i] = 0.0;
-      } else {
-        if (counts) atomicAdd(&e[C_NEGH - C_NEWTON_CELLS], 1ull);
-      }
-    }
-    {
-      bool anyneg = false, anybig = false;
-#pragma unroll
-      for (int i = 0; i < NS; i++) {
-        anyneg |= q[4 + i] < 0.0;
-        anybig |= q[4 + i] <= -1.0e-7;
-      }
-      if (anyneg) {
-        if (anybig && counts) atomicAdd(&e[C_NEGS - C_NEWTON_CELLS], 1ull);
-        else {
-#pragma unroll
-          for (int i = 0; i < NS; i++)
-            if (q[4 + i] > -1.0e-7) q[4 + i] = 0.0;
-        }
-      }
-    }
-    phys<NS>(P, q, qp);
-    if (counts && qp[3] < 273.0) atomicAdd(&e[C_LOWT - C_NEWTON_CELLS], 1ull);
-    double ssol = 0.0;
-#pragma unroll
-    for (int i = 0; i < NS; i++) ssol += q[4 + i];
-    if (ssol > q[0]) {
-      if (((ssol - q[0]) / q[0] < 1.0e-10) || (q[0] < kEps)) {
-#pragma unroll
-        for (int i = 0; i < NS; i++) q[4 + i] = q[4 + i] / ssol * q[0];
-      } else {
-        if (counts) atomicAdd(&e[C_SUMS - C_NEWTON_CELLS], 1ull);
-      }
-    }
-    if (D.nodata && D.nodata[c]) {
-#pragma unroll
-      for (int i = 0; i < NV; i++) q[i] = 0.0;
-    }
+    assembly_checks<NS>(P, D, c, q, qp, e, counts);
   }
   if (ero) {
     phys_or_dry<NS>(P, q, qp);
@@ -2022,25 +2046,27 @@ __global__ void k_sflag(DevParams P, DevPtrs D) {
   D.sflag[c] = state_flags(D.q[c], zero);
 }
 
+// rows k0 .. k1-1 of nvar planes into the host layout (var, j, k - k0)
 template <int = 0>
 __global__ void k_planes_to_aos(DevParams P, const double* __restrict__ src, int nvar,
-                                double* __restrict__ dst) {
+                                double* __restrict__ dst, int k0, int k1) {
   const int j = blockIdx.x * kTileX + (threadIdx.x & 63);
-  const int k = blockIdx.y * kTileY + (threadIdx.x >> 6);
-  if (j >= P.nx || k >= P.ny) return;
-  const size_t pl = plane(P), s = ((size_t)k * P.nx + j) * nvar;
+  const int k = k0 + blockIdx.y * kTileY + (threadIdx.x >> 6);
+  if (j >= P.nx || k >= k1) return;
+  const size_t pl = plane(P), s = ((size_t)(k - k0) * P.nx + j) * nvar;
   for (int i = 0; i < nvar; i++) dst[s + i] = src[i * pl + cidx(P, j, k)];
 }
 
 // qp(n_vars+2, j, k) of the current state (the module array qp after the
-// post-step loop: src/IMEX_SfloW2D.f90:395-445)
+// post-step loop: src/IMEX_SfloW2D.f90:395-445), rows k0 .. k1-1
 template <int NS>
-__global__ void k_qp_out(DevParams P, const double* __restrict__ q, double* __restrict__ dst) {
+__global__ void k_qp_out(DevParams P, const double* __restrict__ q, double* __restrict__ dst,
+                         int k0, int k1) {
   constexpr int NV = 4 + NS;
   const int j = blockIdx.x * kTileX + (threadIdx.x & 63);
-  const int k = blockIdx.y * kTileY + (threadIdx.x >> 6);
-  if (j >= P.nx || k >= P.ny) return;
-  const size_t pl = plane(P), c = cidx(P, j, k), s = ((size_t)k * P.nx + j) * (NV + 2);
+  const int k = k0 + blockIdx.y * kTileY + (threadIdx.x >> 6);
+  if (j >= P.nx || k >= k1) return;
+  const size_t pl = plane(P), c = cidx(P, j, k), s = ((size_t)(k - k0) * P.nx + j) * (NV + 2);
   double qq[NV], qp[NV + 2];
 #pragma unroll
   for (int i = 0; i < NV; i++) qq[i] = q[i * pl + c];

@@ -10,25 +10,25 @@
 namespace imexsw {
 
 template <int NS>
-void launch_stencil(imexsw_ctx* c, int compute_dt, bool rows) {
+void launch_stencil(imexsw_ctx* c, const DevPtrs& D, int compute_dt, bool rows) {
   // work items: (listed tile, strip) pairs, at most tiles * kTileH / row_unit
   const int items = c->n_tiles * (kTileH / c->row_unit);
   if (c->P.ny > 1 && rows) {
     static int res = 0;
     if (!res) res = resident_grid(k_stencil_rows<NS>, 384, c->n_cu, 1 << 30);
     hipLaunchKernelGGL((k_stencil_rows<NS>), dim3(std::min(c->n_tiles * kTileH, res)),
-                       dim3(384), 0, c->stream, c->P, c->D, compute_dt);
+                       dim3(384), 0, c->stream, c->P, D, compute_dt);
   } else if (c->P.ny > 1) {
     static int res = 0;
     if (!res) res = resident_grid(k_stencil<NS, true>, 64, c->n_cu, 1 << 30);
     hipLaunchKernelGGL((k_stencil<NS, true>), dim3(std::min(items, res)), dim3(64), 0,
-                       c->stream, c->P, c->D, c->row_unit, c->strip_h, c->target_strips,
+                       c->stream, c->P, D, c->row_unit, c->strip_h, c->target_strips,
                        compute_dt);
   } else {
     static int res = 0;
     if (!res) res = resident_grid(k_stencil<NS, false>, 64, c->n_cu, 1 << 30);
     hipLaunchKernelGGL((k_stencil<NS, false>), dim3(std::min(items, res)), dim3(64), 0,
-                       c->stream, c->P, c->D, c->row_unit, c->strip_h, c->target_strips,
+                       c->stream, c->P, D, c->row_unit, c->strip_h, c->target_strips,
                        compute_dt);
   }
 }
@@ -70,8 +70,26 @@ void launch_local(imexsw_ctx* c, int ops, bool fused) {
                        c->P, c->D, ops & ~(int)OP_RK);
 }
 template <int NS>
-void launch_qp(imexsw_ctx* c, double* dst) {
-  hipLaunchKernelGGL((k_qp_out<NS>), c->grid_tiles, dim3(256), 0, c->stream, c->P, c->D.q, dst);
+void launch_qp(imexsw_ctx* c, double* dst, int k0, int k1) {
+  const dim3 grid(c->grid_tiles.x, (k1 - k0 + kTileY - 1) / kTileY);
+  hipLaunchKernelGGL((k_qp_out<NS>), grid, dim3(256), 0, c->stream, c->P, c->D.q, dst, k0, k1);
+}
+
+template <int NS>
+void launch_stage(imexsw_ctx* c, int st, int policy) {
+  const int items = (c->n_tiles * kTileH + 3) / 4;
+  static int res = 0;
+  if (!res) res = resident_grid(kg_stage<NS>, 256, c->n_cu, 1 << 30);
+  hipLaunchKernelGGL((kg_stage<NS>), dim3(std::min(items, res)), dim3(256), 0, c->stream, c->P,
+                     c->D, c->G, c->tab, st, policy);
+}
+template <int NS>
+void launch_assemble(imexsw_ctx* c, int policy) {
+  const int items = (c->n_tiles * kTileH + 3) / 4;
+  static int res = 0;
+  if (!res) res = resident_grid(kg_assemble<NS>, 256, c->n_cu, 1 << 30);
+  hipLaunchKernelGGL((kg_assemble<NS>), dim3(std::min(items, res)), dim3(256), 0, c->stream,
+                     c->P, c->D, c->G, c->tab, policy);
 }
 
 template <int NS>
@@ -96,9 +114,11 @@ void launch_batch_local(imexsw_ensemble* e) {
                      e->d_params, e->d_ptrs, L, (int)(OP_EROSION | OP_POST));
 }
 
-template void launch_stencil<IMEXSW_NS>(imexsw_ctx*, int, bool);
+template void launch_stencil<IMEXSW_NS>(imexsw_ctx*, const DevPtrs&, int, bool);
+template void launch_stage<IMEXSW_NS>(imexsw_ctx*, int, int);
+template void launch_assemble<IMEXSW_NS>(imexsw_ctx*, int);
 template void launch_local<IMEXSW_NS>(imexsw_ctx*, int, bool);
-template void launch_qp<IMEXSW_NS>(imexsw_ctx*, double*);
+template void launch_qp<IMEXSW_NS>(imexsw_ctx*, double*, int, int);
 #if IMEXSW_NS >= 1 && IMEXSW_NS <= 3
 template void launch_batch_stencil<IMEXSW_NS>(imexsw_ensemble*);
 template void launch_batch_local<IMEXSW_NS>(imexsw_ensemble*);

@@ -244,7 +244,8 @@ class Solver:
             self.set_source(*init_source_np(self.params))
         self.upload_q(deck.q)
         if self.ns:
-            self.upload_erosion(deck.erodible)
+            z = np.zeros((self.nx, self.ny, self.ns), order="F")
+            self.upload_erosion(deck.erodible, z, z)
         self.t = 0.0
         if init:
             self.check_solve(True)
@@ -285,6 +286,12 @@ class Solver:
                                           C.byref(n)))
         self.t = float(t[0])
         return n.value
+
+    def get_dt(self):
+        """The last step's dt (after the host loop's clipping)."""
+        v = np.zeros(1)
+        self._chk(self.lib.imexsw_get_dt(self.h, _dp(v)))
+        return float(v[0])
 
     def set_tuning(self, max_strip_rows=16, target_strips=2048):
         self._chk(self.lib.imexsw_set_tuning(self.h, int(max_strip_rows), int(target_strips)))

@@ -200,7 +200,10 @@ int imexsw_set_source(imexsw_ctx *ctx, const int32_t *source_cell,
 
 /* Upload the conservative state q(n_vars,nx,ny) (read_solution /
  * init_2d write the module array q: src/inpout_2d.f90:5013,
- * src/init_2d.f90:36, 89). */
+ * src/init_2d.f90:36, 89).  The upload starts a new run: the next
+ * imexsw_advance initialises dt_old and dt_old_old from its dt0, as the
+ * main program does after the initial state is set
+ * (src/IMEX_SfloW2D.f90:248-249). */
 int imexsw_upload_q(imexsw_ctx *ctx, const double *q);
 
 /* Upload erodible(n_solid,nx,ny) and optionally deposit/erosion
@@ -244,7 +247,11 @@ int imexsw_post_step(imexsw_ctx *ctx, double sums[4]);
  * and dt_old_old are carried in the context, initialised from dt0 by the
  * first call), with no host round trip per step.  t_io = current time
  * (updated), t_end and t_output as in the host loop.  Stops early when t
- * reaches t_end.  Steps run as replays of captured 20-step graphs (direct
+ * reaches t_output (t_output within 1e-7 of t_end counts as t_end): there
+ * the reference's loop writes its output and output_solution raises
+ * t_output (src/IMEX_SfloW2D.f90:510-527, src/inpout_2d.f90:6166), so the
+ * host does the same and calls again; a call with t >= t_output takes no
+ * step.  Steps run as replays of captured 20-step graphs (direct
  * launches for a remainder, and for every step while per-kernel timing is
  * on); each replay runs the form for few or for many computed cells, chosen
  * from the computed-cell count of an earlier step (stats.sparse_steps). */

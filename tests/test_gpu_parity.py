@@ -136,6 +136,85 @@ def test_entry_points_match_oracle():
     check_state(orc, dev)
 
 
+@pytest.mark.parametrize("n_rk", [3, 4, 1])
+def test_dambreak_1d_tableaux_parity(n_rk):
+    """The 1-D dam-break configuration with the other IMEX tableaux of
+    src/solver_2d.f90:398-486 (N_RK = 3 is the second-order IMEX-SSP(3,3,2)):
+    every stage's Newton solve and flux evaluation on the device against the
+    oracle, whose N_RK = 3 and 4 results are pinned on the Stoker solution."""
+    d = decks.dambreak_1d(n_rk=n_rk)
+    orc, dev = run_pair(d, 300, t_end=30.0)
+    check_state(orc, dev)
+    s = dev.stats()
+    c = orc.counters()
+    assert s["newton_iters"] == c["newton_iters"]
+    assert s["newton_cells"] == c["newton_cells"]
+
+
+@pytest.mark.parametrize("n_rk", [3, 4])
+@pytest.mark.parametrize("kw", [dict(), dict(erosion=True, topo_change=True),
+                                dict(source=True), dict(slope_correction=True, curvature=True)])
+def test_small_2d_tableaux_parity(n_rk, kw):
+    """2-D decks with dry fronts (the mask dilates N_RK times), erosion with
+    topography change, the radial source ring and slope correction, with
+    N_RK = 3 and 4; graph replays (40 steps) and a direct-launch remainder."""
+    d = decks.small_2d(n=40, ny=36, seed=5, n_rk=n_rk, **kw)
+    orc, dev = run_pair(d, 45)
+    check_state(orc, dev)
+    assert dev.stats()["newton_iters"] == orc.counters()["newton_iters"]
+
+
+def test_tableau_3_entry_points_match_oracle():
+    """The procedures one at a time with N_RK = 3: check_solve, timestep,
+    imex_RK_solver, update_erosion_deposition_cell, post-step sums."""
+    d = decks.small_2d(n=40, ny=36, seed=6, n_rk=3)
+    orc = Oracle(d.params, fresh=True).load_deck(d)
+    dev = Solver(d.params).load_deck(d)
+    t = 0.0
+    for _ in range(6):
+        n_o = orc.check_solve(False)
+        n_d = dev.check_solve(False, t=t)
+        assert n_o == n_d
+        dt_o = orc.timestep()
+        dt_d = dev.timestep()
+        assert dt_o == dt_d
+        dt = min(dt_o, 0.05)
+        orc.imex_rk(t, dt)
+        dev.imex_RK_solver(dt, t=t)
+        orc.erosion_deposition(dt)
+        dev.update_erosion_deposition_cell(dt)
+        s_o = orc.post_step()
+        s_d = dev.post_step()
+        assert np.allclose(s_o, s_d, rtol=1e-12, atol=0.0), (s_o, s_d)
+        t += dt
+    check_state(orc, dev)
+
+
+def test_advance_pauses_at_output_times():
+    """imexsw_advance with t_output < t_end and far more steps than needed
+    returns at t = t_output with a positive dt history, as the reference's
+    loop stops to write output there; calls with later output times go on,
+    and every stretch matches the oracle run the same way (graph replays
+    and the direct-launch remainder both pause)."""
+    d = decks.small_2d(n=40, ny=36, seed=4)
+    orc = Oracle(d.params, fresh=True).load_deck(d)
+    dev = Solver(d.params).load_deck(d)
+    t_end = 1e30
+    for t_out in (0.05, 0.05, 0.13, 0.4):
+        n1 = orc.run(1 << 30, dt0=d.dt0, t_end=t_end, t_output=t_out)
+        n2 = dev.advance(1 << 30, dt0=d.dt0, t_end=t_end, t_output=t_out)
+        print("t_output", t_out, "steps", n1, n2, "t", orc.t, dev.t, flush=True)
+        assert n1 == n2
+        assert dev.t == t_out and orc.t == t_out
+        assert dev.get_dt() > 0.0 and dev.get_dt() == orc.dt_last[0]
+        check_state(orc, dev)
+    # the last stretch, to t_end, still advances
+    n2 = dev.advance(25, dt0=d.dt0, t_end=0.45, t_output=1.0)
+    n1 = orc.run(25, dt0=d.dt0, t_end=0.45, t_output=1.0)
+    assert n1 == n2 > 0 and dev.t == orc.t
+    check_state(orc, dev)
+
+
 def test_mass_conservation_closed_box():
     """With reflecting walls (Dirichlet hu = hv = 0 faces) and no
     sources/sinks, total mass is conserved to round-off."""
