@@ -7,22 +7,31 @@ stencil pass with the CFL reduction, dt policy, stage-2 Newton and assembly,
 erosion/deposition, post-step diagnostics, topography update) with the state
 resident in HBM.  The default workload is the synthetic volcano: 2048^2
 cells, dx = 5 m, cone + 32 bumps + fractal noise, radial source 1e6 kg/s,
-three solid classes, erosion/deposition with topography change, 1000
-timesteps from t = 0.  ``--workload wet`` runs the fully wet 2048^2 incline
-variant (every cell active) that exposes the stencil and Newton kernels.
+three solid classes, erosion/deposition with topography change, a run of
+1000 timesteps from t = 0.  ``--workload wet`` runs the fully wet 2048^2
+incline variant (every cell active) that exposes the stencil and Newton
+kernels.
 
-Printed: one JSON line.  ``value`` is grid cell-updates/s (nx*ny*steps /
-wall time of the timed steps, run as the library runs them: captured graphs
-replayed on its stream); the active variant (sum of solve_cells / time) is
-in ``config``.  ``roofline`` prices the dominant kernel's algorithmic HBM
-bytes per launch against 8 TB/s using the kernel's average duration,
-measured with HIP events on the library's stream in a second pass of the
-same steps (per-kernel events need direct launches, which slow the step, so
-that pass's wall time is reported separately); ``traffic`` comes from the
-committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary of that kernel when one
-exists.  ``cpu_baseline`` times the C oracle (the reference
-algorithm restated, OpenMP over cells, reference interface semantics) on the
-host cores over the first steps of the same workload.
+Timing.  W warm-up steps run on the context that is timed (code objects,
+caches, the captured step graphs); the deck is then loaded again and the run
+starts from t = 0.  It is timed in segments between synchronisations: the
+first K steps, the `develop` steps after them, the timed window of K steps
+at a developed state (steps 500..500+K by default), and the rest of the
+1000-step run.  ``value`` is grid cell-updates/s of the window (nx*ny*K /
+its wall time, the steps run as the library runs them: captured graphs
+replayed on its stream); the t = 0 window and the whole run are reported in
+``config``, as are active and computed cell-updates/s.
+
+``roofline`` prices the dominant kernel's algorithmic HBM bytes per launch
+against 8 TB/s using the kernel's average duration in the same window,
+measured with HIP events on the library's stream in a second pass (per-kernel
+events need direct launches, which slow the step, so that pass's wall time
+is reported separately).  The fully wet 2048^2 grid runs in the same
+invocation and its roofline (with rocprofv3 FETCH_SIZE/WRITE_SIZE traffic
+from the committed summaries) is in ``config.wet_2048``.  ``cpu_baseline``
+times the C oracle (the reference algorithm restated, OpenMP over cells,
+reference interface semantics) on the host cores, started from the device's
+state at the start of the window.
 """
 import argparse
 import json
@@ -106,27 +115,70 @@ def make_deck(workload):
     raise SystemExit("unknown workload %r" % workload)
 
 
-def cpu_baseline(workload, steps, threads):
+def host_cpu():
+    """Model name and physical core count of the host CPU (/proc/cpuinfo)."""
+    model, cores = None, set()
+    phys = core = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                k, _, v = line.partition(":")
+                k, v = k.strip(), v.strip()
+                if k == "model name" and model is None:
+                    model = v
+                elif k == "physical id":
+                    phys = v
+                elif k == "core id":
+                    core = v
+                elif not k and phys is not None:
+                    cores.add((phys, core))
+                    phys = core = None
+        if phys is not None:
+            cores.add((phys, core))
+    except OSError:
+        pass
+    return model, (len(cores) or None)
+
+
+def cpu_baseline(workload, steps, threads, start=None):
     """Time the oracle (reference interface semantics) on the host cores
-    over the first `steps` timesteps of the workload."""
+    over `steps` timesteps of the workload: from t = 0, or from the state
+    `start` (q, erodible, B_cent, t, dt) that the device reached, so that
+    the sample covers the same window as the timed device steps."""
     from oracle.pyoracle import Oracle
     d = make_deck(workload)
     o = Oracle(d.params, fresh=False, threads=threads).load_deck(d)
+    dt0, t_start = d.dt0, 0.0
+    if start is not None:
+        q, erodible, B, t_start, dt0 = start
+        o.lib.orc_set_topography(o.h, np.asfortranarray(B).ctypes.data_as(
+            o.lib.orc_set_topography.argtypes[1]), None)
+        o.set_q(q)
+        if d.params.n_solid:
+            e = np.asfortranarray(erodible)
+            o.lib.orc_set_erodible(o.h, e.ctypes.data_as(o.lib.orc_set_erodible.argtypes[1]))
+        o.t = t_start
     t0 = time.perf_counter()
-    n = o.run(steps, dt0=d.dt0)
+    n = o.run(steps, dt0=dt0)
     el = time.perf_counter() - t0
     c = o.counters()
     o.close()
     cells = d.params.nx * d.params.ny
+    model, phys = host_cpu()
+    where = ("from t = %.4g s (the device's state at the start of the timed window)" % t_start
+             if start is not None else "from t = 0")
     return {
         "value": cells * n / el,
         "unit": "cell-updates/s",
         "cores": threads,
         "kind": "port",
-        "sample": "first %d timesteps of the %s workload (%dx%d) from t=0 on %d "
-                  "host threads, %.2f s wall; active cell-updates/s %.4g"
-                  % (n, workload, d.params.nx, d.params.ny, threads, el,
+        "sample": "%d timesteps of the %s workload (%dx%d) %s on %d host threads, %.2f s wall; "
+                  "active cell-updates/s %.4g"
+                  % (n, workload, d.params.nx, d.params.ny, where, threads, el,
                      c["active_cell_updates"] / el),
+        "host_cpu": model,
+        "host_physical_cores": phys,
+        "active_cell_updates_per_s": c["active_cell_updates"] / el,
     }
 
 
@@ -138,7 +190,7 @@ def bench_ensemble(args):
     advances the members one after the other, each in its own context."""
     from imex_sflow2d_v2_amd import decks
     from imex_sflow2d_v2_amd.solver import Ensemble, Solver
-    steps = args.steps if args.steps != 1000 else 500
+    steps = args.ens_steps
     ds = decks.ensemble_members(count=args.members)
     params = [d.params for d in ds]
     cells = sum(p.nx * p.ny for p in params)
@@ -300,16 +352,97 @@ def bench_warmstart(args):
     }
 
 
+def roofline(p, kt, st, steps, traffic_workload=None):
+    """Roofline object of the dominant kernel of a timing pass: algorithmic
+    bytes per launch (bytes per processed cell x cells processed per launch)
+    over the kernel's mean launch duration from the HIP events."""
+    cells = p.nx * p.ny
+    dname, (dms, dcalls) = max(((k, v) for k, v in kt.items() if v[1] > 0),
+                               key=lambda kv: kv[1][0])
+    avg_s = dms / dcalls * 1e-3
+    units = st["cell_updates_computed"] / steps if dname in ("stencil", "local") else cells
+    achieved = kernel_bytes(p, dname) * units / avg_s / 1e9
+    return {
+        "bound": "hbm",
+        "kernel": dname,
+        "achieved": achieved,
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": achieved / HBM_PEAK_GBS,
+        "traffic": parse_traffic(dname, traffic_workload) if traffic_workload else None,
+        "bytes_per_unit": kernel_bytes(p, dname),
+        "units_per_launch": units,
+        "avg_launch_ms": avg_s * 1e3,
+    }
+
+
+def bench_wet_roofline(nsteps):
+    """The fully wet 2048^2 grid (every cell active and computed): the
+    HBM-roofline case of the plain path.  Steps per second through the
+    captured graphs, and each big kernel's achieved bytes over its mean
+    duration from a timing pass."""
+    from imex_sflow2d_v2_amd.solver import Solver
+    d = make_deck("wet")
+    p = d.params
+    g = Solver(p).load_deck(d)
+    g.advance(2 * 20, dt0=d.dt0)  # capture the graphs
+    g.load_deck(d)
+    g.sync()
+    g.stats(reset=True)
+    a = time.perf_counter()
+    n = g.advance(nsteps, dt0=d.dt0)
+    g.sync()
+    el = time.perf_counter() - a
+    st = g.stats()
+    g.close()
+    h = Solver(p).load_deck(d)
+    h.set_timing(True)
+    h.advance(nsteps, dt0=d.dt0)
+    h.sync()
+    kt = h.kernel_times()
+    h.close()
+    kernels = {}
+    for name in ("local", "stencil", "mask"):
+        if name in kt and kt[name][1]:
+            ms = kt[name][0] / kt[name][1]
+            units = st["cell_updates_computed"] / max(n, 1) if name != "mask" else p.nx * p.ny
+            ach = kernel_bytes(p, name) * units / (ms * 1e-3) / 1e9
+            kernels[name] = {"avg_launch_ms": ms, "achieved_GBs": ach,
+                             "frac": ach / HBM_PEAK_GBS, "bytes_per_unit": kernel_bytes(p, name),
+                             "traffic": parse_traffic(name, "wet")}
+    return {
+        "steps": n,
+        "ms_per_step": el / max(n, 1) * 1e3,
+        "grid_cell_updates_per_s": p.nx * p.ny * n / el,
+        "newton_iters_per_cell": st["newton_iters"] / max(1, st["newton_cells"]),
+        "roofline": roofline(p, kt, st, max(n, 1), traffic_workload="wet"),
+        "kernels": kernels,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--steps", type=int, default=100,
+                    help="timed steps (the window at a developed state)")
     ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--develop", type=int, default=None,
+                    help="untimed steps from t = 0 before the timed window "
+                         "(default 500 for the volcano, 0 for the wet grid)")
+    ap.add_argument("--total", type=int, default=1000,
+                    help="length of the whole run the window is part of (the "
+                         "volcano configuration's 1000 timesteps); 0: stop after "
+                         "the window")
+    ap.add_argument("--wet-steps", type=int, default=10,
+                    help="steps of the fully wet 2048^2 grid run for its roofline")
+    ap.add_argument("--no-wet", action="store_true")
     ap.add_argument("--workload", default="volcano",
                     choices=["volcano", "wet", "ensemble", "warmstart"])
+    ap.add_argument("--ens-steps", type=int, default=500,
+                    help="ensemble workload: steps per member")
     ap.add_argument("--members", type=int, default=256,
                     help="ensemble workload: number of members")
-    ap.add_argument("--cpu-steps", type=int, default=30)
+    ap.add_argument("--cpu-steps", type=int, default=100)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-probe", action="store_true",
@@ -351,50 +484,102 @@ def main():
     d = make_deck(args.workload)
     p = d.params
     cells = p.nx * p.ny
+    develop = args.develop if args.develop is not None else (500 if args.workload == "volcano" else 0)
+    K = args.steps
 
-    # warm-up on its own context (code objects, caches), then a fresh
-    # context timed from t = 0
-    if args.warmup > 0:
-        w = Solver(p).load_deck(d)
-        w.advance(args.warmup, dt0=d.dt0)
-        w.sync()
-        w.close()
-    # the timed steps: the library's production path (captured 20-step
-    # graphs replayed on its stream)
+    # Warm-up on the context that is then timed: code objects, caches and the
+    # captured step graphs (the develop steps below capture them too).
+    # Loading the deck again starts a new run from t = 0 with the same
+    # device pointers, so the graphs stay valid.
     g = Solver(p).load_deck(d)
+    if args.warmup > 0:
+        g.advance(max(args.warmup, 2 * 20), dt0=d.dt0)
+        g.sync()
+        g.load_deck(d)
     g.sync()
-    g.stats(reset=True)
-    g.sync()
-    t0 = time.perf_counter()
-    n = g.advance(args.steps, dt0=d.dt0)
-    g.sync()
-    el = time.perf_counter() - t0
-    st = g.stats()
+    # One run from t = 0 in segments, each timed between synchronisations:
+    # the first K steps, the develop steps, the timed window of K steps at a
+    # developed state (the headline), and the rest up to `total` steps.
+    seg = {}
+    stats = {}
+
+    def segment(name, n):
+        if n <= 0:
+            return 0
+        g.stats(reset=True)
+        g.sync()
+        a = time.perf_counter()
+        done = g.advance(n, dt0=d.dt0)
+        g.sync()
+        seg[name] = (done, time.perf_counter() - a)
+        stats[name] = g.stats()
+        return done
+
+    n0 = segment("t0_window", min(K, develop) if develop else 0)
+    segment("develop", develop - n0)
+    t_win = g.t
+    dt_win = g.get_dt()
+    start_state = None
+    if not args.no_cpu:
+        er = g.erosion_state()
+        start_state = (g.q(), er[0], er[3], t_win, dt_win)
+    n = segment("window", K)
+    rest = args.total - develop - K if args.total else 0
+    segment("rest", rest)
+    t_final = g.t
     g.close()
-    # the same steps again with a HIP event pair around every kernel on the
+    el = seg["window"][1]
+    st = stats["window"]
+
+    # The timed window again with a HIP event pair around every kernel on the
     # library's stream (direct launches: the events slow the step down, so
-    # this pass gives the kernel durations, not the throughput)
+    # this pass gives the kernel durations, not the throughput).
     h = Solver(p).load_deck(d)
     h.set_timing(True)
+    if develop:
+        h.advance(develop, dt0=d.dt0)
     h.sync()
+    h.set_timing(True)  # clears the develop steps' kernel times
     t1 = time.perf_counter()
-    h.advance(args.steps, dt0=d.dt0)
+    h.advance(K, dt0=d.dt0)
     h.sync()
     el_timed = time.perf_counter() - t1
     kt = h.kernel_times()
     h.close()
 
     steps = max(n, 1)
+    rl = roofline(p, kt, st, steps, traffic_workload=None if develop else args.workload)
     active = st["cell_updates_active"]
-    dom = max(((k, v) for k, v in kt.items() if v[1] > 0), key=lambda kv: kv[1][0])
-    dname, (dms, dcalls) = dom
-    avg_s = dms / dcalls * 1e-3
     computed = st["cell_updates_computed"]
-    units = computed / steps if dname in ("stencil", "local") else cells
-    alg_bytes = kernel_bytes(p, dname) * units
-    achieved = alg_bytes / avg_s / 1e9
-    traffic = parse_traffic(dname, args.workload)
-
+    cfg = {
+        "workload": "%s_%dx%d" % (args.workload, p.nx, p.ny),
+        "nx": p.nx, "ny": p.ny, "n_solid": p.n_solid, "n_rk": p.n_rk,
+        "dx": p.cell_size,
+        "active_cell_updates_per_s": active / el,
+        "active_fraction": active / float(cells * steps),
+        "computed_cell_updates_per_s": computed / el,
+        "newton_iters_per_cell": st["newton_iters"] / max(1, st["newton_cells"]),
+        "newton_iter_hist_1_2_3_4_8_16_32_100": st["newton_iter_hist"],
+        "newton_iter_max": st["newton_iter_max"],
+        "line_search_evals_per_cell": st["line_search_evals"] / max(1, st["newton_cells"]),
+        "kernel_ms_per_step": {k: v[0] / steps for k, v in kt.items() if v[1]},
+        "kernel_times_pass_ms_per_step": el_timed / steps * 1e3,
+        "step_bytes_per_active_cell": step_bytes(p),
+        "simulated_time_s": t_final,
+    }
+    for name in ("t0_window", "rest"):
+        if name in seg:
+            nn, tt = seg[name]
+            cfg[name + "_ms_per_step"] = tt / max(nn, 1) * 1e3
+    tot_n = sum(v[0] for v in seg.values())
+    tot_t = sum(v[1] for v in seg.values())
+    cfg["whole_run"] = {"steps": tot_n, "ms_per_step": tot_t / max(tot_n, 1) * 1e3,
+                        "grid_cell_updates_per_s": cells * tot_n / tot_t}
+    cfg["window"] = ("steps %d-%d of one run from t = 0 (t = %.4g s at the window's start); "
+                     "step graphs captured before the window"
+                     % (develop, develop + steps, t_win))
+    if args.workload == "volcano" and not args.no_wet:
+        cfg["wet_2048"] = bench_wet_roofline(args.wet_steps)
     out = {
         "metric": "FP64 cell-updates/s at 2048^2",
         "value": cells * steps / el,
@@ -408,39 +593,14 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {
-            "workload": "%s_%dx%d" % (args.workload, p.nx, p.ny),
-            "nx": p.nx, "ny": p.ny, "n_solid": p.n_solid, "n_rk": p.n_rk,
-            "dx": p.cell_size,
-            "active_cell_updates_per_s": active / el,
-            "active_fraction": active / float(cells * steps),
-            "computed_cell_updates_per_s": computed / el,
-            "newton_iters_per_cell": st["newton_iters"] / max(1, st["newton_cells"]),
-            "newton_iter_hist_1_2_3_4_8_16_32_100": st["newton_iter_hist"],
-            "newton_iter_max": st["newton_iter_max"],
-            "line_search_evals_per_cell": st["line_search_evals"] / max(1, st["newton_cells"]),
-            "simulated_time_s": g.t,
-            "kernel_ms_per_step": {k: v[0] / steps for k, v in kt.items() if v[1]},
-            "kernel_times_pass_ms_per_step": el_timed / steps * 1e3,
-            "step_bytes_per_active_cell": step_bytes(p),
-        },
-        "roofline": {
-            "bound": "hbm",
-            "kernel": dname,
-            "achieved": achieved,
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS,
-            "traffic": traffic,
-            "bytes_per_unit": kernel_bytes(p, dname),
-            "units_per_launch": units,
-            "avg_launch_ms": avg_s * 1e3,
-        },
+        "config": cfg,
+        "roofline": rl,
         "cpu_baseline": None,
     }
     if not args.no_cpu:
         threads = args.cpu_threads or min(16, len(os.sched_getaffinity(0)))
-        out["cpu_baseline"] = cpu_baseline(args.workload, args.cpu_steps, threads)
+        out["cpu_baseline"] = cpu_baseline(args.workload, max(args.cpu_steps, K), threads,
+                                           start=start_state if develop else None)
     print(json.dumps(out), flush=True)
     return 0
 
