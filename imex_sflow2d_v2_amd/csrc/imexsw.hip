@@ -11,6 +11,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -27,6 +28,19 @@ thread_local std::string g_create_error;
 
 }  // namespace
 
+// Staging memory of imexsw_download_noalloc: one array per GPU, held until
+// the process ends.  Host code uses its device address (resolved once per
+// device by imexsw_create with hipGetSymbolAddress), never the name.
+constexpr size_t kScratchDoubles = (size_t)2 << 20;  // 16 MiB
+static __device__ __align__(256) double scratch[kScratchDoubles];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kMaxDevices] = {nullptr};
+
+}  // namespace
 
 namespace {
 
@@ -39,6 +53,20 @@ int fail(imexsw_ctx* c, const char* fmt, ...) {
   if (c) c->err = buf;
   else g_create_error = buf;
   return -1;
+}
+
+// the device address of `scratch` on device `dev` (the current device),
+// resolved once per device; on failure the message goes to c
+int resolve_scratch(imexsw_ctx* c, int dev) {
+  if (dev < 0 || dev >= kMaxDevices) return fail(c, "device index %d out of range", dev);
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch_ptr[dev]) return 0;
+  void* ptr = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !ptr)
+    return fail(c, "hipGetSymbolAddress(scratch) on device %d: %s", dev, hipGetErrorString(e));
+  g_scratch_ptr[dev] = static_cast<double*>(ptr);
+  return 0;
 }
 
 #define HIPCHK(c, call)                                                            \
@@ -606,6 +634,7 @@ int imexsw_create(const imexsw_params* params, imexsw_ctx** out) {
       if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
           ncu > 0)
         c->n_cu = ncu;
+      if (!rc && resolve_scratch(c, dev)) rc = 1;
     }
   }
   if (rc) {
@@ -988,6 +1017,42 @@ int imexsw_download(imexsw_ctx* c, double* q, double* qp, double* hmax, double* 
     if (download_plane(c, c->D.vuln, bits.data(), P.nx, P.ny, P.pitch)) return -1;
     for (size_t cc = 0; cc < bits.size(); cc++)
       for (int i = 0; i < nl; i++) vuln[cc * nl + i] = (bits[cc] >> i) & 1u;
+  }
+  return 0;
+}
+
+int imexsw_download_noalloc(imexsw_ctx* c, double* q, double* qp) {
+  if (!c) return fail(c, "null argument");
+  const DevParams& P = c->P;
+  double* ptr = (c->device >= 0 && c->device < kMaxDevices) ? g_scratch_ptr[c->device] : nullptr;
+  if (!ptr) return fail(c, "the allocation-free download has no staging memory on device %d",
+                        c->device);
+  for (int which = 0; which < 2; which++) {
+    double* host = which == 0 ? q : qp;
+    if (!host) continue;
+    const int nvar = which == 0 ? c->nv : c->nv + 2;
+    const size_t row = (size_t)nvar * P.nx;
+    if (row > kScratchDoubles)
+      return fail(c, "imexsw_download_noalloc: a grid row holds %zu values, more than the %zu "
+                     "of its staging memory (nx <= %zu for %d variables)",
+                  row, kScratchDoubles, kScratchDoubles / nvar, nvar);
+    const int rows = (int)std::min<size_t>((size_t)P.ny, kScratchDoubles / row);
+    for (int k0 = 0; k0 < P.ny; k0 += rows) {
+      const int k1 = std::min(P.ny, k0 + rows);
+      if (which == 0) {
+        const dim3 grid(c->grid_tiles.x, (k1 - k0 + kTileY - 1) / kTileY);
+        hipLaunchKernelGGL(k_planes_to_aos<>, grid, dim3(256), 0, c->stream, P, c->D.q, nvar, ptr,
+                           k0, k1);
+      } else {
+#define CALL(N) launch_qp<N>(c, ptr, k0, k1)
+        NS_SWITCH(c->ns, CALL)
+#undef CALL
+      }
+      if (check_launch(c, "imexsw_download_noalloc")) return -1;
+      HIPCHK(c, hipMemcpyAsync(host + (size_t)k0 * row, ptr, (size_t)(k1 - k0) * row * sizeof(double),
+                               hipMemcpyDeviceToHost, c->stream));
+    }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
   }
   return 0;
 }

@@ -59,6 +59,7 @@ ABI = {
     "imexsw_advance": (C.c_int, [C.c_void_p, C.c_int32, C.c_double, _D, C.c_double,
                                  C.c_double, _I32]),
     "imexsw_download": (C.c_int, [C.c_void_p, _D, _D, _D, _D, _D, _U8]),
+    "imexsw_download_noalloc": (C.c_int, [C.c_void_p, _D, _D]),
     "imexsw_download_erosion": (C.c_int, [C.c_void_p, _D, _D, _D, _D]),
     "imexsw_download_mask": (C.c_int, [C.c_void_p, _U8]),
     "imexsw_get_dt": (C.c_int, [C.c_void_p, _D]),

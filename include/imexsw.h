@@ -264,6 +264,18 @@ int imexsw_advance(imexsw_ctx *ctx, int32_t nsteps, double dt0, double *t_io,
 int imexsw_download(imexsw_ctx *ctx, double *q, double *qp, double *hmax,
                     double *pdynmax, double *mod_vel_max, uint8_t *vuln);
 
+/* Allocation-free download of q(n_vars,nx,ny) and qp(n_vars+2,nx,ny) (either
+ * may be NULL), the same values as imexsw_download gives.  No counterpart
+ * in the reference: it is for callers that cannot allocate device memory
+ * (code under stream capture, serving loops, hosts that forbid hipMalloc).
+ * The rows are converted through a static staging array of 2^21 doubles
+ * (16 MiB) per GPU, held until the process ends, in as many chunks as
+ * needed; a grid whose single row holds more values (nvar * nx > 2^21) is
+ * refused with an error.  Calls on one GPU share that array: calls that
+ * overlap in time on one GPU (contexts used from several host threads)
+ * race, so serialise them.  Synchronises the context's stream. */
+int imexsw_download_noalloc(imexsw_ctx *ctx, double *q, double *qp);
+
 /* Download erosion state: erodible(n_solid,nx,ny), deposit(nx,ny,n_solid),
  * erosion(nx,ny,n_solid), B_cent(nx,ny).  Any pointer may be NULL. */
 int imexsw_download_erosion(imexsw_ctx *ctx, double *erodible, double *deposit,
