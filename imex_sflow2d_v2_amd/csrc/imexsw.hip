@@ -303,10 +303,10 @@ void drop_ctx_graph(imexsw_ctx* c) {
 }
 
 // Sparse steps when the stencil strips would be one row high (the
-// strip-height rule of k_stencil applied to w computed cells) and the
-// learned warm start is off.
+// strip-height rule of k_stencil applied to w computed cells), with N_RK = 2
+// and no harvest (the harvest runs in the dense stage kernel).
 bool want_sparse(const imexsw_ctx* c, unsigned long long w) {
-  if (c->general || c->warm || c->D.harvest || c->P.ny < 2) return false;
+  if (c->general || c->D.harvest || c->P.ny < 2) return false;
   return c->strip_h < 2 ||
          w / (unsigned long long)(kStripW * 2) < (unsigned long long)c->target_strips;
 }
@@ -944,12 +944,14 @@ int imexsw_advance(imexsw_ctx* c, int32_t nsteps, double dt0, double* t_io, doub
     if (std::memcmp(&c->graph_P, &c->P, sizeof(DevParams)) ||
         std::memcmp(&c->graph_D, &c->D, sizeof(DevPtrs)) ||
         c->graph_shape != c->strip_h * 1000003 + c->target_strips || c->graph_warm != c->warm ||
+        c->graph_closed != c->closed_start ||
         std::memcmp(&c->graph_mlp, &c->mlp, sizeof(MlpDev))) {
       drop_ctx_graph(c);
       c->graph_P = c->P;
       c->graph_D = c->D;
       c->graph_shape = c->strip_h * 1000003 + c->target_strips;
       c->graph_warm = c->warm;
+      c->graph_closed = c->closed_start;
       c->graph_mlp = c->mlp;
     }
     for (int r = 0; n + kGraphSteps <= nsteps; n += kGraphSteps, r++) {
@@ -1916,6 +1918,15 @@ int imexsw_harvest_get(imexsw_ctx* c, float* pairs, int64_t cap, int64_t* n) {
   return 0;
 }
 
+// closed-form Newton start for the following stage-2 solves
+int imexsw_set_closed_form_start(imexsw_ctx* c, int enable) {
+  if (!c) return fail(c, "null argument");
+  if (c->general && enable) return fail(c, "the closed-form start runs with N_RK = 2 only");
+  if (enable && c->warm) return fail(c, "switch the learned warm start off first");
+  c->closed_start = enable != 0;
+  return 0;
+}
+
 // learned warm start for the following stage-2 Newton solves (NULL: off)
 int imexsw_set_warm_start(imexsw_ctx* c, imexsw_mlp* m) {
   if (!c) return fail(c, "null argument");
@@ -1924,6 +1935,7 @@ int imexsw_set_warm_start(imexsw_ctx* c, imexsw_mlp* m) {
     return 0;
   }
   if (c->general) return fail(c, "the learned warm start runs with N_RK = 2 only");
+  if (c->closed_start) return fail(c, "switch the closed-form start off first");
   if (!c->D.guess && dalloc(c, &c->D.guess, 2 * c->plane)) return -1;
   c->mlp = mlp_dev(m);
   c->warm = true;

@@ -53,8 +53,10 @@ struct imexsw_ctx {
   std::vector<hipEvent_t> ev_pool;
   std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> pending;
   std::vector<void*> allocs;
-  // learned warm start (guess planes in D.guess) and harvest buffer
+  // learned warm start (guess planes in D.guess) and harvest buffer; the
+  // closed-form Newton start
   bool warm = false;
+  bool closed_start = false;
   MlpDev mlp{};
   // sparse steps (few computed cells): the row stencil and the fused local
   // kernel; chosen on the host from the computed-cell count of an earlier
@@ -73,6 +75,7 @@ struct imexsw_ctx {
   long graph_shape = 0;
   MlpDev graph_mlp{};
   bool graph_warm = false;
+  bool graph_closed = false;
 };
 
 // time steps per captured graph of imexsw_advance

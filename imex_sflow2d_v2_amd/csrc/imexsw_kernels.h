@@ -1416,7 +1416,7 @@ __global__ void k_step_begin(DevParams P, DevPtrs D, int commit, int reset) {
 // OP_POLICY: the step's dt is step_dt (the policy is committed after the
 // step), else the scalar S_DT
 enum LocalOps { OP_RK = 1, OP_EROSION = 2, OP_POST = 4, OP_SUMS = 8, OP_WARM = 16,
-                OP_HARVEST = 32, OP_POLICY = 64 };
+                OP_HARVEST = 32, OP_POLICY = 64, OP_CLOSED = 128 };
 
 // The assembly checks of imex_RK_solver (src/solver_2d.f90:1379-1624) on
 // the assembled state q of cell c: NaN trap, small or negative thickness,
@@ -1480,14 +1480,15 @@ __device__ __forceinline__ void assembly_checks(const DevParams& P, const DevPtr
 // counter C_NEWTON_CELLS + m in e[m]
 // PH selects the part compiled in: 0 = stage update and assembly checks
 // (OP_RK), 1 = erosion/deposition and diagnostics (OP_EROSION, OP_POST,
-// OP_SUMS), 2 = the stage update with the learned warm start and the pair
-// harvest (OP_WARM, OP_HARVEST), 3 = parts 0 and 1 in one pass (few
-// computed cells: a row goes through both without waiting for every other
-// row's stage update).  Separate kernels keep the plain stage update's
-// register footprint free of the learned part, and on a wet grid the stage
-// update's free of the erosion part.
-// (PH 3 runs each cell on kSparseLanes lanes: every lane computes, `counts`
-// is set on one of them)
+// OP_SUMS), 2 = the stage update with a Newton starting point (the learned
+// one, OP_WARM, or the closed form, OP_CLOSED) and the pair harvest
+// (OP_HARVEST), 3 = parts 0 and 1 in one pass (few computed cells: a row
+// goes through both without waiting for every other row's stage update),
+// 4 = part 3 with a Newton starting point.  Separate kernels keep the plain
+// stage update's register footprint free of the starting-point code, and on
+// a wet grid the stage update's free of the erosion part.
+// (PH 3 and 4 run each cell on kSparseLanes lanes: every lane computes,
+// `counts` is set on one of them)
 template <int NS, int PH>
 __device__ __forceinline__ void local_cell(const DevParams& P, const DevPtrs& D, size_t c,
                                            double dt, int ops, unsigned long long* e,
@@ -1496,10 +1497,10 @@ __device__ __forceinline__ void local_cell(const DevParams& P, const DevPtrs& D,
   constexpr int NV = 4 + NS;
   constexpr int NSX = NS > 0 ? NS : 1;
   const size_t pl = plane(P);
-  const bool rk = (PH == 0 || PH == 2 || PH == 3) && (ops & OP_RK);
-  const bool ero = (PH == 1 || PH == 3) && (ops & OP_EROSION) &&
+  const bool rk = (PH == 0 || PH == 2 || PH >= 3) && (ops & OP_RK);
+  const bool ero = (PH == 1 || PH >= 3) && (ops & OP_EROSION) &&
                    !((P.erosion_coeff == 0.0) && !P.settling && !P.entrain);
-  const bool post = (PH == 1 || PH == 3) && (ops & OP_POST);
+  const bool post = (PH == 1 || PH >= 3) && (ops & OP_POST);
   // every load of the cell first, the mask byte among them, so that the
   // cell waits for memory once
   double q[NV], qp[NV + 2];
@@ -1516,7 +1517,7 @@ __device__ __forceinline__ void local_cell(const DevParams& P, const DevPtrs& D,
     bxy = D.Bxy ? D.Bxy[c] : 0.0;
     byy = D.Byy ? D.Byy[c] : 0.0;
     gc = D.gcoef ? D.gcoef[c] : 1.0;
-    if (PH == 2 && (ops & OP_WARM)) {
+    if ((PH == 2 || PH == 4) && (ops & OP_WARM)) {
       wg[0] = D.guess[c];
       wg[1] = D.guess[pl + c];
     }
@@ -1578,10 +1579,11 @@ __device__ __forceinline__ void local_cell(const DevParams& P, const DevPtrs& D,
     int nonconv = 0, ls = 0, wu = -1;
     float hv[kHarvestWidth];
     bool hok = false;
-    int it = stage2<NS, (PH == 3) ? kSparseLanes : 1>(P, q0, Dr, dt, bx, by, bxx, bxy, byy, gc,
+    int it = stage2<NS, (PH >= 3) ? kSparseLanes : 1>(P, q0, Dr, dt, bx, by, bxx, bxy, byy, gc,
                                                        q, nonconv, ls,
-                        (PH == 2 && (ops & OP_WARM)) ? wg : nullptr, &wu,
-                        (PH == 2 && (ops & OP_HARVEST)) ? hv : nullptr, &hok);
+                        ((PH == 2 || PH == 4) && (ops & OP_WARM)) ? wg : nullptr, &wu,
+                        (PH == 2 && (ops & OP_HARVEST)) ? hv : nullptr, &hok,
+                        (PH == 2 || PH == 4) && (ops & OP_CLOSED));
     if (counts && wu == 1) atomicAdd(&e[C_WARM_USED - C_NEWTON_CELLS], 1ull);
     if (counts && wu == 0) atomicAdd(&e[C_WARM_FALLBACK - C_NEWTON_CELLS], 1ull);
     if (PH == 2 && (ops & OP_HARVEST)) {
@@ -1612,7 +1614,7 @@ __device__ __forceinline__ void local_cell(const DevParams& P, const DevPtrs& D,
   if (ero) {
     phys_or_dry<NS>(P, q, qp);
     double er[NSX], dep[NSX], eq[NV], topo;
-    mass_exchange<NS, (PH == 3) ? kSparseLanes : 1>(P, qp, bx, by, erod, dt, er, dep, eq, topo);
+    mass_exchange<NS, (PH >= 3) ? kSparseLanes : 1>(P, qp, bx, by, erod, dt, er, dep, eq, topo);
 #pragma unroll
     for (int i = 0; i < NV; i++) q[i] = q[i] + dt * eq[i];
 #pragma unroll
@@ -1718,7 +1720,7 @@ __device__ __forceinline__ void local_row(const DevParams& P, const DevPtrs& D, 
   double s_mass = 0.0, s_vol = 0.0, s_area = 0.0, s_sol = 0.0;
   if (own)
     local_cell<NS, PH>(P, D, cidx(P, j, k), dt, ops, e, imax, s_mass, s_vol, s_area, s_sol);
-  if ((PH == 1 || PH == 3) && (ops & OP_SUMS)) {
+  if (PH == 1 && (ops & OP_SUMS)) {
     double v4[4] = {s_mass, s_vol, s_area, s_sol};
 #pragma unroll
     for (int m = 0; m < 4; m++) {
@@ -1733,7 +1735,7 @@ __device__ __forceinline__ void local_row(const DevParams& P, const DevPtrs& D, 
 // kSparseLanes parts of 60 / kSparseLanes, cell i of the part on lanes
 // kSparseLanes*i .. kSparseLanes*i + kSparseLanes-1 (lanes 60..63 idle).  No
 // per-row sums.
-template <int NS>
+template <int NS, int PH>
 __device__ __forceinline__ void local_part_row(const DevParams& P, const DevPtrs& D, int row,
                                                int part, double dt, int ops,
                                                unsigned long long* e, int& imax) {
@@ -1745,7 +1747,7 @@ __device__ __forceinline__ void local_part_row(const DevParams& P, const DevPtrs
   const int k = (tile / tiles_x) * kTileH + row % kTileH;
   double s_mass = 0.0, s_vol = 0.0, s_area = 0.0, s_sol = 0.0;
   if (i < kPart && j < P.nx && k < P.ny)
-    local_cell<NS, 3>(P, D, cidx(P, j, k), dt, ops & ~(int)OP_SUMS, e, imax, s_mass, s_vol,
+    local_cell<NS, PH>(P, D, cidx(P, j, k), dt, ops & ~(int)OP_SUMS, e, imax, s_mass, s_vol,
                       s_area, s_sol, (lane % kSparseLanes) == 0);
 }
 
@@ -1857,12 +1859,12 @@ __global__ void __launch_bounds__(256) k_local(DevParams P, DevPtrs D, int ops) 
   }
   const int nrows = (int)*ctr(D, C_ROWS, 0);
   // PH 3: kSparseLanes items (row parts) per listed row
-  const int items = (PH == 3) ? kSparseLanes * nrows : nrows;
+  const int items = (PH >= 3) ? kSparseLanes * nrows : nrows;
   for (int it = blockIdx.x * 4 + w; live && it < items; it += gridDim.x * 4) {
-    if (lane == 0) mark_dirty(P, D, D.row_list[(PH == 3) ? it / kSparseLanes : it] / kTileH);
-    if (PH == 3)
-      local_part_row<NS>(P, D, D.row_list[it / kSparseLanes], it % kSparseLanes, dt, ops,
-                         cnt[w], imax);
+    if (lane == 0) mark_dirty(P, D, D.row_list[(PH >= 3) ? it / kSparseLanes : it] / kTileH);
+    if (PH >= 3)
+      local_part_row<NS, (PH >= 3 ? PH : 3)>(P, D, D.row_list[it / kSparseLanes],
+                                             it % kSparseLanes, dt, ops, cnt[w], imax);
     else
       local_row<NS, PH>(P, D, D.row_list[it], dt, ops, cnt[w], imax);
   }

@@ -331,7 +331,8 @@ template <int NS>
 __device__ __forceinline__ NewtonOut solve_rk_step_quad(const DevParams& P, double* qj,
                                                        const double* qold, const double* Rj,
                                                        double a_diag, double dt, double Bx,
-                                                       double By) {
+                                                       double By, const double* rt0 = nullptr,
+                                                       double scal0 = 0.0) {
   constexpr int NV = 4 + NS;
   const double TOLF = 1.0e-10;
   const double TOLX = kEps;
@@ -344,6 +345,13 @@ __device__ __forceinline__ NewtonOut solve_rk_step_quad(const DevParams& P, doub
   NewtonOut out{0, 0, 0};
   int it;
   bool fresh = false;
+  if (rt0) {
+    // residual at the starting point already evaluated by the caller
+#pragma unroll
+    for (int i = 0; i < NV; i++) rt[i] = rt0[i];
+    scal = scal0;
+    fresh = true;
+  }
   bool jnext = false;       // J1n/J2n hold the Jacobian at the current point
   double J1n[3], J2n[3];
   for (it = 1; it <= 100; it++) {
@@ -572,6 +580,67 @@ __device__ __forceinline__ void mlp_features_raw(const DevParams& P, const doubl
   f[8] = (float)sqrt(Bx * Bx + By * By);
 }
 
+// Mixture density and reduced gravity of a state whose momenta are (mx, my),
+// the real part of the chain in implicit_c (c_phys_var and mixt_var,
+// src/constitutive_2d.f90:1145-1313, 1333-1483) for q1 > eps_sing.
+template <int NS>
+__device__ __forceinline__ void rho_rg_of(const DevParams& P, const double* q, double mx,
+                                          double my, double& rho_m, double& rg) {
+  const double inv_q1 = 1.0 / q[0];
+  double xs_tot = 0.0, dsh = 0.0, dinv = 0.0;
+#pragma unroll
+  for (int i = 0; i < NS; i++) {
+    const double xs = q[4 + i] * inv_q1;
+    xs_tot += xs;
+    dsh += xs * P.sp_heat_s[i];
+    dinv += xs * P.inv_rho_s[i];
+  }
+  const double xc = 1.0 - xs_tot;
+  const double shm = dsh + xc * P.sp_heat_a;
+  double T = P.energy ? (q[3] - 0.5 * (mx * mx + my * my) * inv_q1) / (q[0] * shm)
+                      : q[3] / (q[0] * shm);
+  if (T <= 0.0) T = P.T_ambient;
+  rho_m = 1.0 / (dinv + xc * (P.sp_gas_const_a * T * P.inv_pres));
+  rg = (rho_m - P.rho_a_amb) / rho_m * P.grav;
+}
+
+// A starting point for the implicit Voellmy solve (no reference
+// counterpart: the reference starts Newton at q_si, src/solver_2d.f90:1196).
+// The momenta solve m = m_si + dt a NH(m) with NH = -(rho_m g'/xi) |U| u,
+// u = m/q1 and |U| = |u| sqrt(1 + (u.grad B)^2/|u|^2) with slope correction.
+// m is parallel to m_si, so with the mixture density frozen the speed s =
+// |m| solves s + C c s^2 = |m_si| with C = dt a rho_m g'/(xi q1^2), c the
+// slope factor: s = 2|m_si| / (1 + sqrt(1 + 4 C c |m_si|)).  The density
+// depends on s only through the kinetic energy in the temperature, weakly,
+// so three refreshes of rho_m, g' at the current s make the root exact to
+// rounding.  Returns false where it does not apply (no Voellmy term, thin
+// or lighter-than-air cells, no momentum); Newton then starts at q_si.
+template <int NS>
+__device__ __forceinline__ bool voellmy_closed_form(const DevParams& P, const double* q_si,
+                                                    double a_diag, double dt, double Bx,
+                                                    double By, double* m) {
+  if (!P.rheo || P.rheo_model != 1 || !(q_si[0] > P.eps_sing)) return false;
+  const double mx = q_si[1], my = q_si[2];
+  const double ms = sqrt(mx * mx + my * my);
+  if (!(ms > 0.0)) return false;
+  const double ex = mx / ms, ey = my / ms;
+  const double wn = P.slope ? ex * Bx + ey * By : 0.0;
+  const double c = sqrt(1.0 + wn * wn);
+  const double k0 = dt * a_diag / (P.xi * q_si[0] * q_si[0]) * c;
+  double s = ms;
+#pragma unroll
+  for (int it = 0; it < 4; it++) {
+    double rho_m, rg;
+    rho_rg_of<NS>(P, q_si, s * ex, s * ey, rho_m, rg);
+    const double C = k0 * rho_m * rg;
+    if (!(C > 0.0)) return false;
+    s = 2.0 * ms / (1.0 + sqrt(1.0 + 4.0 * C * ms));
+  }
+  m[0] = s * ex;
+  m[1] = s * ey;
+  return isfinite(m[0]) && isfinite(m[1]);
+}
+
 // The stage-2 predictor (src/solver_2d.f90:1132-1196): q_fv = q0 - dt*D, its
 // physical variables, and the semi-implicit Coulomb step q_si realigned with
 // q_fv.  Returns false when q_fv has no mass (no Newton solve).
@@ -622,7 +691,8 @@ __device__ __forceinline__ int stage2(const DevParams& P, const double* q0,
                                       double Byy, double gc, double* q,
                                       int& nonconv, int& ls_evals,
                                       const double* warm = nullptr, int* warm_used = nullptr,
-                                      float* harvest = nullptr, bool* harvested = nullptr) {
+                                      float* harvest = nullptr, bool* harvested = nullptr,
+                                      bool closed = false) {
   constexpr int NV = 4 + NS;
   double q_fv[NV], qp[NV + 2], q_si[NV];
   nonconv = 0;
@@ -640,26 +710,45 @@ __device__ __forceinline__ int stage2(const DevParams& P, const double* q0,
     q[i] = q_si[i];
   }
   NewtonOut no;
-  if (warm && warm[0] == warm[0] && warm[1] == warm[1]) {
-    // learned starting point for the momenta: Newton starts from whichever
-    // of it and the plain predictor has the smaller residual (the plain one
-    // stays the fallback), and the equation solved is the same
-    double qw[NV], rtw[NV], rtp[NV], sw, sp;
+  double mg[2];
+  bool have = false;
+  if (closed) {
+    have = voellmy_closed_form<NS>(P, q_si, 1.0, dt, Bx, By, mg);
+  } else if (warm && warm[0] == warm[0] && warm[1] == warm[1]) {
+    mg[0] = warm[0];
+    mg[1] = warm[1];
+    have = true;
+  }
+  if (have) {
+    // a starting point for the momenta (learned, or the closed form): Newton
+    // starts from it when it already meets the tolerance or has the smaller
+    // residual of it and the plain predictor (the plain one stays the
+    // fallback), and the equation solved is the same
+    double qw[NV], rtw[NV], rtp[NV], sw, sp = 0.0;
 #pragma unroll
     for (int i = 0; i < NV; i++) qw[i] = q_si[i];
-    qw[1] = warm[0];
-    qw[2] = warm[1];
+    qw[1] = mg[0];
+    qw[2] = mg[1];
     eval_f<NS>(P, qw, q0, Rj, 1.0, dt, Bx, By, rtw, sw);
-    eval_f<NS>(P, q, q0, Rj, 1.0, dt, Bx, By, rtp, sp);
-    const bool use = sw < sp;
+    double mfw = 0.0;
+#pragma unroll
+    for (int i = 0; i < NV; i++) mfw = fmax(mfw, fabs(rtw[i]));
+    bool use = mfw < 1.0e-10;
+    if (!use) {
+      eval_f<NS>(P, q, q0, Rj, 1.0, dt, Bx, By, rtp, sp);
+      use = sw < sp;
+    }
     *warm_used = use ? 1 : 0;
     if (use) {
 #pragma unroll
       for (int i = 0; i < NV; i++) q[i] = qw[i];
     }
-    no = solve_rk_step<NS>(P, q, q0, Rj, 1.0, dt, Bx, By, use ? rtw : rtp, use ? sw : sp);
+    no = (LANES == 4)
+             ? solve_rk_step_quad<NS>(P, q, q0, Rj, 1.0, dt, Bx, By, use ? rtw : rtp,
+                                      use ? sw : sp)
+             : solve_rk_step<NS>(P, q, q0, Rj, 1.0, dt, Bx, By, use ? rtw : rtp, use ? sw : sp);
   } else {
-    if (warm_used) *warm_used = -1;
+    if (warm_used) *warm_used = (closed || warm) ? 0 : -1;
     no = (LANES == 4) ? solve_rk_step_quad<NS>(P, q, q0, Rj, 1.0, dt, Bx, By)
                       : solve_rk_step<NS>(P, q, q0, Rj, 1.0, dt, Bx, By);
   }

@@ -36,12 +36,29 @@ template <int NS>
 void launch_local(imexsw_ctx* c, int ops, bool fused) {
   // work items: listed tile rows, four per workgroup
   const int items = (c->n_tiles * kTileH + 3) / 4;
+  // a Newton starting point for the stage update: the learned one (its
+  // guesses computed first, into the guess planes) or the closed form
+  int start = 0;
+  if ((ops & OP_RK) && c->warm) {
+    static int resg = 0;
+    if (!resg) resg = resident_grid(k_mlp_guess<NS>, 64, c->n_cu, 1 << 30);
+    hipLaunchKernelGGL((k_mlp_guess<NS>), dim3(std::min(c->n_tiles * kTileH, resg)), dim3(64),
+                       0, c->stream, c->P, c->D, c->mlp, (ops & OP_POLICY) ? 1 : 0);
+    start = OP_WARM;
+  } else if ((ops & OP_RK) && c->closed_start) {
+    start = OP_CLOSED;
+  }
   if (fused) {
     // row parts, four per workgroup
-    static int res3 = 0;
+    static int res3 = 0, res4 = 0;
     if (!res3) res3 = resident_grid(k_local<NS, 3>, 256, c->n_cu, 1 << 30);
-    hipLaunchKernelGGL((k_local<NS, 3>), dim3(std::min(kSparseLanes * items, res3)), dim3(256), 0,
-                       c->stream, c->P, c->D, ops & ~(int)OP_SUMS);
+    if (!res4) res4 = resident_grid(k_local<NS, 4>, 256, c->n_cu, 1 << 30);
+    if (start)
+      hipLaunchKernelGGL((k_local<NS, 4>), dim3(std::min(kSparseLanes * items, res4)), dim3(256),
+                         0, c->stream, c->P, c->D, (ops & ~(int)OP_SUMS) | start);
+    else
+      hipLaunchKernelGGL((k_local<NS, 3>), dim3(std::min(kSparseLanes * items, res3)), dim3(256),
+                         0, c->stream, c->P, c->D, ops & ~(int)OP_SUMS);
     return;
   }
   static int res0 = 0, res1 = 0, res2 = 0;
@@ -49,16 +66,9 @@ void launch_local(imexsw_ctx* c, int ops, bool fused) {
   if (!res1) res1 = resident_grid(k_local<NS, 1>, 256, c->n_cu, 1 << 30);
   if (!res2) res2 = resident_grid(k_local<NS, 2>, 256, c->n_cu, 1 << 30);
   if (ops & OP_RK) {
-    int rk = OP_RK | (ops & OP_POLICY);
-    if (c->warm) {
-      static int resg = 0;
-      if (!resg) resg = resident_grid(k_mlp_guess<NS>, 64, c->n_cu, 1 << 30);
-      hipLaunchKernelGGL((k_mlp_guess<NS>), dim3(std::min(c->n_tiles * kTileH, resg)), dim3(64),
-                         0, c->stream, c->P, c->D, c->mlp, (ops & OP_POLICY) ? 1 : 0);
-      rk |= OP_WARM;
-    }
+    int rk = OP_RK | (ops & OP_POLICY) | start;
     if (c->D.harvest) rk |= OP_HARVEST;
-    if (!(rk & (OP_WARM | OP_HARVEST)))
+    if (!(rk & (OP_WARM | OP_HARVEST | OP_CLOSED)))
       hipLaunchKernelGGL((k_local<NS, 0>), dim3(std::min(items, res0)), dim3(256), 0, c->stream,
                          c->P, c->D, rk);
     else

@@ -93,6 +93,7 @@ ABI = {
     "imexsw_harvest": (C.c_int, [C.c_void_p, C.c_int64]),
     "imexsw_harvest_get": (C.c_int, [C.c_void_p, _F, C.c_int64, C.POINTER(C.c_int64)]),
     "imexsw_set_warm_start": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "imexsw_set_closed_form_start": (C.c_int, [C.c_void_p, C.c_int]),
     "imexsw_sync": (C.c_int, [C.c_void_p]),
     "imexsw_q2d_write": (C.c_int, [C.c_char_p, C.c_int32, C.c_int32, C.c_int32, _D, _D, _D]),
     "imexsw_q2d_read": (C.c_int, [C.c_char_p, C.c_int32, C.c_int32, C.c_int32, C.c_int32, _D]),
@@ -293,6 +294,11 @@ class Solver:
         v = np.zeros(1)
         self._chk(self.lib.imexsw_get_dt(self.h, _dp(v)))
         return float(v[0])
+
+    def set_closed_form_start(self, enable=True):
+        """Newton starts from the closed-form Voellmy solution (see
+        imexsw_set_closed_form_start)."""
+        self._chk(self.lib.imexsw_set_closed_form_start(self.h, int(bool(enable))))
 
     def set_tuning(self, max_strip_rows=16, target_strips=2048):
         self._chk(self.lib.imexsw_set_tuning(self.h, int(max_strip_rows), int(target_strips)))

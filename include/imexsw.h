@@ -430,6 +430,17 @@ int imexsw_harvest_get(imexsw_ctx *ctx, float *pairs, int64_t cap, int64_t *n);
  * must outlive its use by the context. */
 int imexsw_set_warm_start(imexsw_ctx *ctx, imexsw_mlp *mlp);
 
+/* Closed-form Newton start (enable != 0) for the following stage-2 solves,
+ * no reference counterpart: the momenta of the Voellmy implicit equation
+ * m = m_si + dt NH(m) are parallel to m_si, and with the mixture density
+ * frozen their magnitude is the root of a quadratic; refreshing the density
+ * (which depends on the speed only through the kinetic energy) makes it
+ * the solution to rounding.  Newton starts there when that point meets the
+ * tolerance or has a smaller residual than the plain predictor q_si
+ * (src/solver_2d.f90:1196), and iterates to the reference tolerance as
+ * always.  Exclusive with the learned warm start; N_RK = 2 only. */
+int imexsw_set_closed_form_start(imexsw_ctx *ctx, int enable);
+
 #ifdef __cplusplus
 }
 #endif

@@ -55,7 +55,8 @@ def test_forward_matches_cpu_reference():
     ref = mlp_ref.forward(p, x.astype(np.float64), quantise=True)
     err = np.abs(y - ref).max() / np.abs(ref).max()
     print("forward max rel err", err, flush=True)
-    assert err < 2e-2
+    # observed 9e-5: the bar is about ten times that
+    assert err < 1e-3
     m.close()
 
 
@@ -72,11 +73,14 @@ def test_gradient_matches_cpu_reference():
     rel = np.linalg.norm(g - gref) / np.linalg.norm(gref)
     cos = float(g @ gref / (np.linalg.norm(g) * np.linalg.norm(gref)))
     print("loss", loss, lref, "grad rel err", rel, "cos", cos, flush=True)
-    assert abs(loss - lref) / lref < 1e-3
-    assert rel < 2e-2 and cos > 0.9998
+    # observed: loss 1.3e-6 relative, gradient 2.3e-6 relative; the bars are
+    # about ten times those
+    assert abs(loss - lref) / lref < 2e-5
+    assert rel < 2.5e-5 and cos > 0.9999999
     for k, (o, n) in mlp_ref.OFF.items():   # every block, not just the norm
         blk = np.linalg.norm(g[o:o + n] - gref[o:o + n])
-        assert blk <= 5e-2 * max(np.linalg.norm(gref[o:o + n]), 1e-12), k
+        print("block", k, blk / max(np.linalg.norm(gref[o:o + n]), 1e-12), flush=True)
+        assert blk <= 1e-4 * max(np.linalg.norm(gref[o:o + n]), 1e-12), k
     m.close()
 
 
@@ -130,3 +134,85 @@ def test_warm_start_harvest_train_and_parity():
         assert err <= 1e-8
     assert abs(warm.t - ref.t) <= 1e-9 * ref.t
     m.close()
+
+
+def _max_rel(a, b):
+    out = []
+    for v in range(b.shape[0]):
+        out.append(float(np.abs(a[v] - b[v]).max() / max(np.abs(b[v]).max(), 1e-300)))
+    return np.array(out)
+
+
+@pytest.fixture(scope="module")
+def volcano_oracle_60():
+    """The 2048^2 volcano after 60 steps in the oracle (the device's fresh
+    interface semantics) and the plain device run's Newton counts."""
+    from oracle import Oracle
+    d = decks.volcano_2048()
+    orc = Oracle(d.params, fresh=True).load_deck(d)
+    assert orc.run(60, dt0=d.dt0) == 60
+    plain = Solver(d.params).load_deck(d)
+    plain.advance(60, dt0=d.dt0)
+    return d, orc.q(), orc.t, plain.stats()
+
+
+def test_learned_warm_start_volcano_2048_against_oracle(volcano_oracle_60):
+    """Configuration 5 at its evaluation size: pairs harvested by the solver
+    from a volcano on another seeded DEM, the MLP trained on the device, and
+    the warm-started solver on the 2048^2 volcano for 60 steps (sparse steps
+    with the guesses computed per step), within 1e-10 of the oracle on every
+    conserved variable; every Newton solve is counted as started from the
+    network or fallen back to the plain predictor."""
+    d, q_orc, t_orc, st_plain = volcano_oracle_60
+    src = decks.volcano_2048(seed=5)
+    h = Solver(src.params).load_deck(src)
+    learned.start_harvest(h, 4_000_000)
+    h.advance(300, dt0=src.dt0)
+    pairs, produced = learned.harvested(h)
+    learned.stop_harvest(h)
+    h.close()
+    print("harvested", pairs.shape[0], "of", produced, flush=True)
+    assert pairs.shape[0] > 100_000
+    m = learned.Mlp(seed=31)
+    hist = m.train(pairs, epochs=3, batch=4096, lr=1e-3, seed=32)
+    print("loss history", hist, flush=True)
+    assert hist[-1] < hist[0]
+    w = Solver(d.params).load_deck(d)
+    learned.set_warm_start(w, m)
+    assert w.advance(60, dt0=d.dt0) == 60
+    s = w.stats()
+    e = _max_rel(w.q(), q_orc)
+    print("warm vs oracle max rel err per var", e, flush=True)
+    print("iters/cell warm %.3f plain %.3f; warm %d fallback %d; sparse steps %d" % (
+        s["newton_iters"] / s["newton_cells"],
+        st_plain["newton_iters"] / st_plain["newton_cells"], s["warm_cells"],
+        s["warm_fallbacks"], s["sparse_steps"]), flush=True)
+    assert np.all(e <= 1e-10), e
+    assert abs(w.t - t_orc) <= 1e-12 * t_orc
+    assert s["warm_cells"] + s["warm_fallbacks"] == s["newton_cells"]
+    assert s["newton_nonconverged"] == 0
+    assert s["sparse_steps"] == 60
+    m.close()
+
+
+def test_closed_form_start_volcano_2048_against_oracle(volcano_oracle_60):
+    """The closed-form Newton start on the 2048^2 volcano, 60 steps: within
+    1e-10 of the oracle on every conserved variable, every solve counted, and
+    at least 1.5 times fewer Newton iterations per cell than the plain
+    predictor."""
+    d, q_orc, t_orc, st_plain = volcano_oracle_60
+    w = Solver(d.params).load_deck(d)
+    w.set_closed_form_start(True)
+    assert w.advance(60, dt0=d.dt0) == 60
+    s = w.stats()
+    e = _max_rel(w.q(), q_orc)
+    it_w = s["newton_iters"] / s["newton_cells"]
+    it_p = st_plain["newton_iters"] / st_plain["newton_cells"]
+    print("closed form vs oracle max rel err per var", e, flush=True)
+    print("iters/cell closed %.3f plain %.3f; started %d fallback %d; hist %s" % (
+        it_w, it_p, s["warm_cells"], s["warm_fallbacks"], s["newton_iter_hist"]), flush=True)
+    assert np.all(e <= 1e-10), e
+    assert abs(w.t - t_orc) <= 1e-12 * t_orc
+    assert s["warm_cells"] + s["warm_fallbacks"] == s["newton_cells"]
+    assert s["newton_nonconverged"] == 0
+    assert it_p / it_w >= 1.5, (it_p, it_w)
