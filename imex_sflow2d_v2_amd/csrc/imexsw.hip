@@ -467,6 +467,8 @@ int imexsw_create(const imexsw_params* params, imexsw_ctx** out) {
   if (p.n_rk < 1 || p.n_rk > kMaxRK) return fail(nullptr, "N_RK must be 1, 2, 3 or 4 (got %d)", p.n_rk);
   if (p.rheology_flag && p.rheology_model != 1)
     return fail(nullptr, "rheology model %d is not supported (only 1, Voellmy-Salm)", p.rheology_model);
+  if (p.vertical_profiles_flag)
+    return fail(nullptr, "vertical profiles are not supported by this build");
   if (p.n_thickness_levels < 0 || p.n_dyn_pres_levels < 0 ||
       p.n_thickness_levels * p.n_dyn_pres_levels > 32 ||
       p.n_thickness_levels > IMEXSW_MAX_LEVELS || p.n_dyn_pres_levels > IMEXSW_MAX_LEVELS)

@@ -19,6 +19,7 @@ MODULE imexsw_c
   INTEGER, PARAMETER :: IMEXSW_MAX_SOLID = 6
   INTEGER, PARAMETER :: IMEXSW_MAX_VARS = 4 + IMEXSW_MAX_SOLID
   INTEGER, PARAMETER :: IMEXSW_MAX_LEVELS = 8
+  INTEGER, PARAMETER :: IMEXSW_MAX_QUAD = 64
 
   !> sides of bc_flag / bc_value
   INTEGER, PARAMETER :: IMEXSW_W = 1, IMEXSW_E = 2, IMEXSW_S = 3, IMEXSW_N = 4
@@ -53,6 +54,8 @@ MODULE imexsw_c
      INTEGER(c_int32_t) :: n_thickness_levels, n_dyn_pres_levels
      REAL(c_double) :: thickness_levels(IMEXSW_MAX_LEVELS)
      REAL(c_double) :: dyn_pres_levels(IMEXSW_MAX_LEVELS)
+     INTEGER(c_int32_t) :: vertical_profiles_flag, bottom_conc_flag, n_quad
+     REAL(c_double) :: vonK, k_s, Sc, z_dyn, friction_factor
   END TYPE imexsw_params
 
   !> struct imexsw_stats: cumulative counters of a context

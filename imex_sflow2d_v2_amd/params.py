@@ -10,6 +10,7 @@ import ctypes as C
 MAX_SOLID = 6
 MAX_VARS = 4 + MAX_SOLID
 MAX_LEVELS = 8
+MAX_QUAD = 64
 
 W, E, S, N = 0, 1, 2, 3
 
@@ -47,6 +48,10 @@ class ImexswParams(C.Structure):
         ("n_thickness_levels", C.c_int32), ("n_dyn_pres_levels", C.c_int32),
         ("thickness_levels", C.c_double * MAX_LEVELS),
         ("dyn_pres_levels", C.c_double * MAX_LEVELS),
+        ("vertical_profiles_flag", C.c_int32), ("bottom_conc_flag", C.c_int32),
+        ("n_quad", C.c_int32),
+        ("vonK", C.c_double), ("k_s", C.c_double), ("Sc", C.c_double),
+        ("z_dyn", C.c_double), ("friction_factor", C.c_double),
     ]
 
     @property
@@ -78,13 +83,17 @@ def make_params(nx, ny, cell_size, n_solid, *, x0=0.0, y0=0.0, n_rk=2,
                 rho_s=(2500.0,), diam_s=(1.0e-3,), sp_heat_s=(1617.0,),
                 erosion_coeff=0.0, erodible_porosity=0.0, erodible_fract=None,
                 T_erodible=300.0, alphastot_min=0.0, maximum_solid_packing=0.64,
-                source=None, thickness_levels=(), dyn_pres_levels=()):
+                source=None, thickness_levels=(), dyn_pres_levels=(),
+                rheology_model=1, friction_factor=-1.0, vertical_profiles=None):
     """Build an ``ImexswParams``.
 
     ``bc`` maps side (W/E/S/N index) to a list of ``(flag, value)`` per
     conservative variable; default is zero-gradient (flag 1, value 0) for all,
     the choice of the shipped decks (e.g. TESTS/TEST_DAMBREAK/*/IMEX_SfloW2D.inp).
     ``source`` is a dict with x, y, r, h, vel, T, alphas, time_param.
+    ``rheology_model`` 8 integrates the turbulent friction ``friction_factor``
+    in closed form.  ``vertical_profiles`` is a dict with vonK, k_s, Sc,
+    n_quad and optionally bottom_conc, z_dyn (VERTICAL_PROFILES_PARAMETERS).
     """
     p = ImexswParams()
     nv = 4 + n_solid
@@ -99,7 +108,15 @@ def make_params(nx, ny, cell_size, n_solid, *, x0=0.0, y0=0.0, n_rk=2,
     p.topo_change_flag = int(topo_change)
     p.radial_source_flag = int(radial_source)
     p.rheology_flag = int(rheology)
-    p.rheology_model = 1 if rheology else 0
+    p.rheology_model = rheology_model if rheology else 0
+    p.friction_factor = friction_factor
+    if vertical_profiles is not None:
+        vp = vertical_profiles
+        p.vertical_profiles_flag = 1
+        p.bottom_conc_flag = int(vp.get("bottom_conc", False))
+        p.n_quad = vp["n_quad"]
+        p.vonK, p.k_s, p.Sc = vp["vonK"], vp["k_s"], vp["Sc"]
+        p.z_dyn = vp.get("z_dyn", -1.0)
     p.settling_flag = int(settling)
     p.entrainment_flag = int(entrainment)
     p.erodible_deposit_flag = int(erodible_deposit)

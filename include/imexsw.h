@@ -53,9 +53,11 @@ enum { IMEXSW_SRC_E = 1, IMEXSW_SRC_W = 2, IMEXSW_SRC_S = 4, IMEXSW_SRC_N = 8 };
  * the library exactly as read_param / init_grid compute them.
  *
  * Supported model subset: gas carrier (GAS_FLAG=T), no liquid, no additional
- * gas components, no vertical profiles, no stochastic or pore-pressure
- * variables, rheology model 1 (Voellmy-Salm) or RHEOLOGY_FLAG=F, solver
- * scheme "KT".  n_vars = 4 + n_solid (src/inpout_2d.f90:1429, 1852).
+ * gas components, no stochastic or pore-pressure variables, no Sutherland
+ * viscosity, rheology model 1 (Voellmy-Salm), model 8 (turbulent friction
+ * integrated in closed form) or RHEOLOGY_FLAG=F, with or without vertical
+ * profiles, solver scheme "KT".  n_vars = 4 + n_solid (src/inpout_2d.f90:1429,
+ * 1852).
  */
 typedef struct imexsw_params {
   /* grid: geometry_2d comp_cells_x/y, x0, y0, cell_size */
@@ -123,7 +125,28 @@ typedef struct imexsw_params {
   int32_t n_dyn_pres_levels;
   double thickness_levels[IMEXSW_MAX_LEVELS];
   double dyn_pres_levels[IMEXSW_MAX_LEVELS];
+
+  /* vertical profiles (VERTICAL_PROFILES_FLAG and the namelist
+   * VERTICAL_PROFILES_PARAMETERS, src/inpout_2d.f90:367-368, 3695-3796):
+   * log velocity / exponential concentration closures of
+   * src/constitutive_2d.f90:797-920, 970-1123, 1976-2107, 2532-2986.
+   * They require ENERGY_FLAG = F and RHEOLOGY_FLAG = F or
+   * RHEOLOGY_MODEL = 8, as read_param does (src/inpout_2d.f90:3699-3718). */
+  int32_t vertical_profiles_flag;
+  int32_t bottom_conc_flag;   /* deposition scaled by the bottom concentration */
+  int32_t n_quad;             /* Gauss-Legendre points, 1..IMEXSW_MAX_QUAD */
+  double vonK;                /* von Karman constant */
+  double k_s;                 /* roughness length [m] */
+  double Sc;                  /* Schmidt number */
+  double z_dyn;               /* read but without effect on the state: the
+                                 profile dynamic pressure is overwritten by
+                                 r_phys_var (src/constitutive_2d.f90:957) */
+  /* RHEOLOGY_PARAMETERS friction_factor: rheology model 8 and the shear
+   * velocity of the profiles (src/inpout_2d.f90:357, 3350-3365) */
+  double friction_factor;
 } imexsw_params;
+
+#define IMEXSW_MAX_QUAD 64
 
 /* Per-context counters (cumulative since create or the last reset). */
 typedef struct imexsw_stats {

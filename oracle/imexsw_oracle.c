@@ -14,9 +14,11 @@
  * quotients use Smith's range-reduced algorithm, SQRT is glibc's csqrt
  * formula and DOT_PRODUCT conjugates its first argument.
  *
- * Supported subset: gas carrier, no liquid / additional gas / vertical
- * profiles / stochastic / pore pressure / bottom or lateral sources,
- * rheology model 1 (Voellmy-Salm) or no rheology, solver scheme KT.
+ * Supported subset: gas carrier, no liquid / additional gas / stochastic /
+ * pore pressure / bottom or lateral sources / Sutherland viscosity,
+ * rheology model 1 (Voellmy-Salm), model 8 (closed-form turbulent
+ * friction) or no rheology, with or without vertical profiles, solver
+ * scheme KT.
  */
 #include "imexsw_oracle.h"
 
@@ -224,6 +226,10 @@ struct orc {
   double rho_a_amb, inv_pres, inv_rho_s[MAXS], rho_c_sub, coeff_porosity;
   double pi_g, h_cs, one_by_h;
   double T_erodible;
+  /* vertical profiles: Gauss-Legendre points on [-1, 1] (z_quad, w_quad of
+   * geometry_2d) and H_crit_rel (src/inpout_2d.f90:3763, 3784-3791) */
+  int vp, nq;
+  double zq[IMEXSW_MAX_QUAD], wq[IMEXSW_MAX_QUAD], h_crit_rel, sqrt_ff;
   double dt_old, dt_old_old; /* host loop state (src/IMEX_SfloW2D.f90:250) */
   int implicit_flag[MAXV + 1]; /* 1-based */
   int n_nh;
@@ -326,7 +332,20 @@ double orc_limit(const orc_t *o, const double *v, const double *z, int lim) {
 /* constitutive_2d                                                          */
 /* ------------------------------------------------------------------------ */
 
-/* r_phys_var, non-profile branch (src/constitutive_2d.f90:541-967).
+/* vertical profiles (defined after settling_velocity) */
+static double vp_velocity(const orc_t *o, double r_h, double q2, double q3,
+                          double r_rho_m, double r_rho_c,
+                          const double *r_alphas);
+static void vp_momentum(const orc_t *o, double r_h, double r_u, double r_v,
+                        double r_rho_c, const double *r_alphas, double *qc2,
+                        double *qc3);
+static void vp_flux_coeffs(const orc_t *o, const double *qpj, double Bx,
+                           double By, double r_rho_c, double r_rho_m,
+                           double *shape);
+static void vp_dep_coeffs(const orc_t *o, const double *qpj, double mod_vel,
+                          double r_rho_c, double *dep);
+
+/* r_phys_var (src/constitutive_2d.f90:541-967), with the profile branch.
  * Outputs h, u, v, alphas(1:ns), rho_m, T, red_grav, p_dyn. */
 static void r_phys_var(const orc_t *o, const double *qj, double *r_h,
                        double *r_u, double *r_v, double *r_alphas,
@@ -399,7 +418,13 @@ static void r_phys_var(const orc_t *o, const double *qj, double *r_h,
   *r_h = qj[0] * r_inv_rhom;
   *r_red_grav = (rho_m - o->rho_a_amb) * r_inv_rhom * P->grav;
   double u, v;
-  if (qj[0] > o->eps_sing) {
+  if (o->vp) {
+    /* profile branch (:797-942): the mean speed of the profiles */
+    double r_rho_c = P->pres / (r_sp_gas_const_c * T);
+    double ua = vp_velocity(o, *r_h, qj[1], qj[2], rho_m, r_rho_c, r_alphas);
+    u = ua * qj[1] / (sqrt(qj[1] * qj[1] + qj[2] * qj[2]));
+    v = ua * qj[2] / (sqrt(qj[1] * qj[1] + qj[2] * qj[2]));
+  } else if (qj[0] > o->eps_sing) {
     u = qj[1] * inv_qj1;
     v = qj[2] * inv_qj1;
   } else {
@@ -506,7 +531,7 @@ void orc_mixt_var(const orc_t *o, const double *qp, double *out6) {
            &out6[5]);
 }
 
-/* qp_to_qc, non-profile no-liquid branch (src/constitutive_2d.f90:1709-2140)*/
+/* qp_to_qc, no-liquid branch (src/constitutive_2d.f90:1709-2140) */
 static void qp_to_qc(const orc_t *o, const double *qp, double *qc) {
   const imexsw_params *P = &o->p;
   const int ns = o->ns;
@@ -534,7 +559,7 @@ static void qp_to_qc(const orc_t *o, const double *qp, double *qc) {
   double r_rho_a = P->pres / (P->sp_gas_const_a * r_T);
   double r_rho_c = ((1.0 - r_alphal - alphas_tot - 0.0) * r_rho_a + 0.0) /
                    (1.0 - r_alphal - alphas_tot);
-  /* the alphas corrections of :1911-1919 do not reach any output */
+  /* the alphas corrections of :1911-1919 reach only the profile branch */
   double r_alphac = 1.0 - alphas_tot;
   double sar = 0.0;
   for (int i = 0; i < ns; i++)
@@ -554,8 +579,18 @@ static void qp_to_qc(const orc_t *o, const double *qp, double *qc) {
     dsh += r_xs[i] * P->sp_heat_s[i];
   double r_sp_heat_mix = dsh + r_xc * r_sp_heat_c;
   qc[0] = r_rho_m * r_h;
-  qc[1] = r_rho_m * r_hu;
-  qc[2] = r_rho_m * r_hv;
+  if (o->vp) {
+    /* the corrections of :1911-1919, which only the profiles read */
+    double ra[MAXS];
+    for (int i = 0; i < ns; i++)
+      ra[i] = (alphas_tot > 1.0)   ? r_alphas[i] / alphas_tot
+              : (alphas_tot < 0.0) ? 0.0
+                                   : r_alphas[i];
+    vp_momentum(o, r_h, r_u, r_v, r_rho_c, ra, &qc[1], &qc[2]);
+  } else {
+    qc[1] = r_rho_m * r_hu;
+    qc[2] = r_rho_m * r_hv;
+  }
   if (P->energy_flag) {
     if (r_h > 0.0)
       qc[3] = r_h * r_rho_m * (r_sp_heat_mix * r_T + 0.5 * (r_u * r_u + r_v * r_v));
@@ -604,30 +639,37 @@ void orc_local_speeds(const orc_t *o, const double *qp, double gc, int dir,
   local_speeds(o, qp, gc, dir, &mm[0], &mm[1]);
 }
 
-/* eval_fluxes, non-profile branch (src/constitutive_2d.f90:2316-2509) */
+/* eval_fluxes (src/constitutive_2d.f90:2316-2509); Bx, By: the slope of
+ * the cell the face state comes from (read by the profile coefficients) */
 static void eval_fluxes(const orc_t *o, const double *qcj, const double *qpj,
-                        double gc, int dir, double *flux) {
+                        double Bx, double By, double gc, int dir,
+                        double *flux) {
   const int ns = o->ns;
   if (qpj[0] > EPS_WP) {
     double r_h = qpj[0], r_u = qpj[o->idx_u - 1], r_v = qpj[o->idx_v - 1];
     double ri, rho_m, rho_c, rg, c1, c2;
     mixt_var(o, qpj, &ri, &rho_m, &rho_c, &rg, 0, &c1, &c2);
+    double sc[MAXV];
+    for (int i = 0; i < o->ne; i++)
+      sc[i] = 1.0;
+    if (o->vp)
+      vp_flux_coeffs(o, qpj, Bx, By, rho_c, rho_m, sc);
     double vel = (dir == 1) ? r_u : r_v;
     double pres_term = 0.5 * rho_m * gc * rg * (r_h * r_h);
-    flux[0] = vel * qcj[0] * 1.0;
+    flux[0] = vel * qcj[0] * sc[0];
     if (dir == 1) {
-      flux[1] = r_u * qcj[1] * 1.0 + pres_term;
-      flux[2] = r_u * qcj[2] * 1.0;
+      flux[1] = r_u * qcj[1] * sc[1] + pres_term;
+      flux[2] = r_u * qcj[2] * sc[2];
     } else {
-      flux[1] = r_v * qcj[1] * 1.0;
-      flux[2] = r_v * qcj[2] * 1.0 + pres_term;
+      flux[1] = r_v * qcj[1] * sc[1];
+      flux[2] = r_v * qcj[2] * sc[2] + pres_term;
     }
     if (o->p.energy_flag)
-      flux[3] = vel * (qcj[3] * 1.0 + pres_term);
+      flux[3] = vel * (qcj[3] * sc[3] + pres_term);
     else
-      flux[3] = vel * qcj[3] * 1.0;
+      flux[3] = vel * qcj[3] * sc[3];
     for (int i = 0; i < ns; i++)
-      flux[4 + i] = vel * qcj[4 + i] * 1.0;
+      flux[4 + i] = vel * qcj[4 + i] * sc[4 + i];
     double sf = 0.0;
     for (int i = 0; i < ns; i++)
       sf += flux[4 + i];
@@ -642,7 +684,7 @@ static void eval_fluxes(const orc_t *o, const double *qcj, const double *qpj,
 }
 void orc_eval_fluxes(const orc_t *o, const double *qc, const double *qp,
                      double gc, int dir, double *flux) {
-  eval_fluxes(o, qc, qp, gc, dir, flux);
+  eval_fluxes(o, qc, qp, 0.0, 0.0, gc, dir, flux);
 }
 
 /* eval_expl_terms, no bottom source (src/constitutive_2d.f90:3015-3245) */
@@ -853,6 +895,439 @@ double orc_settling_velocity(const orc_t *o, double diam, double rhos,
   return settling_velocity(o, diam, rhos, rhoc, ikv);
 }
 
+/* ------------------------------------------------------------------------ */
+/* vertical profiles (VERTICAL_PROFILES_FLAG)                               */
+/* Velocity u(z) ~ log(30 z / k_s + 1) up to a boundary-layer top h0 and    */
+/* constant above it; solid concentration ~ exp(a z) with a = -w_s /        */
+/* epsilon_s (Rouse).  Depth averages use n_quad Gauss-Legendre points on   */
+/* [0, h0] plus the constant part on [h0, h].                               */
+/* ------------------------------------------------------------------------ */
+
+/* Series of W about the branch point -1/e in p = +-sqrt(2 (e x + 1)):
+ * W = sum mu_k p^k (Corless, Gonnet, Hare, Jeffrey & Knuth 1996, "On the
+ * Lambert W function", eqs. 4.22-4.24; the rationals follow from their
+ * recurrence).  Used for |p| < 0.1, where the k = 13 term is below 1e-16. */
+static const double kWmu[13] = {
+    -1.0, 1.0, -1.0 / 3.0, 11.0 / 72.0, -43.0 / 540.0, 769.0 / 17280.0,
+    -221.0 / 8505.0, 680863.0 / 43545600.0, -1963.0 / 204120.0,
+    226287557.0 / 37623398400.0, -5776369.0 / 1515591000.0,
+    169709463197.0 / 69528040243200.0, -1118511313.0 / 709296588000.0};
+#define W_E 2.718281828459045 /* e rounded to double */
+
+static double lw_series(double p) {
+  double s = kWmu[12];
+  for (int k = 11; k >= 0; k--)
+    s = s * p + kWmu[k];
+  return s;
+}
+
+/* Halley's iteration for w e^w = x (Corless et al. 1996, eq. 5.9) from a
+ * starting point on the wanted branch; stops when the step is below
+ * 1e-15 |w| (the iteration is cubic, so the last step already was). */
+static double lw_halley(double x, double w) {
+  for (int it = 0; it < 20; it++) {
+    double ew = det_exp(w);
+    double f = w * ew - x;
+    double wp1 = w + 1.0;
+    double dw = f / (ew * wp1 - (w + 2.0) * f / (2.0 * wp1));
+    w = w - dw;
+    if (!(fabs(dw) > 1.0e-15 * fabs(w)))
+      break;
+  }
+  return w;
+}
+
+/* W_0 (principal branch) for x >= -1/e.  Outside the domain (x < -1/e)
+ * the value is -1, as lambertw0 returns it (src/geometry_2d.f90:1900-1915). */
+double orc_lambertw0(double x) {
+  double p2 = 2.0 * (W_E * x + 1.0);
+  if (!(p2 > 0.0))
+    return -1.0;
+  if (p2 < 0.01)
+    return lw_series(sqrt(p2));
+  double w;
+  if (x < -0.25) {
+    double p = sqrt(p2);
+    w = -1.0 + p * (1.0 + p * (-1.0 / 3.0 + p * (11.0 / 72.0)));
+  } else {
+    /* Winitzki (2003): W ~ L (1 - log(1 + L) / (2 + L)), L = log(1 + x) */
+    double L = det_log(1.0 + x);
+    w = (x == 0.0) ? 0.0 : L * (1.0 - det_log(1.0 + L) / (2.0 + L));
+  }
+  return lw_halley(x, w);
+}
+
+/* W_{-1} for -1/e <= x < 0.  Outside the domain, and for the arguments
+ * above -64 e^-64 that lambertwm1 declares too small, the value is -1 as
+ * there (src/geometry_2d.f90:1985-2010). */
+double orc_lambertwm1(double x) {
+  double p2 = 2.0 * (W_E * x + 1.0);
+  if (!(p2 > 0.0) || !(x < 0.0) || x > -1.0288423856755186e-26)
+    return -1.0;
+  if (p2 < 0.01)
+    return lw_series(-sqrt(p2));
+  double w;
+  if (x < -0.25) {
+    double p = -sqrt(p2);
+    w = -1.0 + p * (1.0 + p * (-1.0 / 3.0 + p * (11.0 / 72.0)));
+  } else {
+    /* asymptotic start W ~ L1 - L2 + L2 / L1, L1 = log(-x), L2 = log(-L1) */
+    double L1 = det_log(-x), L2 = det_log(-L1);
+    w = L1 - L2 + L2 / L1;
+  }
+  return lw_halley(x, w);
+}
+
+/* Gauss-Legendre points and weights on [-1, 1] in ascending order (the
+ * layout gaulegf returns, src/geometry_2d.f90:2407-2449): Newton's method
+ * on P_n from the starting points cos(pi (i - 1/4) / (n + 1/2)), P_n and
+ * P_n' by the three-term recurrence, weights 2 / ((1 - x^2) P_n'(x)^2). */
+void orc_gauss_legendre(int n, double *x, double *w) {
+  const double pi = 3.14159265358979323846;
+  for (int i = 0; i < (n + 1) / 2; i++) {
+    double z = cos(pi * (i + 0.75) / (n + 0.5)), dp = 0.0;
+    for (int it = 0; it < 100; it++) {
+      double pm = 1.0, pc = z; /* P_0, P_1 */
+      for (int m = 2; m <= n; m++) {
+        double pn = ((2.0 * m - 1.0) * z * pc - (m - 1.0) * pm) / m;
+        pm = pc;
+        pc = pn;
+      }
+      if (n == 1) {
+        pc = z;
+        pm = 1.0;
+      }
+      dp = n * (z * pc - pm) / (z * z - 1.0);
+      double dz = pc / dp;
+      z = z - dz;
+      if (!(fabs(dz) > 2.220446049250313e-15))
+        break;
+    }
+    x[i] = -z;
+    x[n - 1 - i] = z;
+    w[i] = 2.0 / ((1.0 - z * z) * dp * dp);
+    w[n - 1 - i] = w[i];
+  }
+}
+
+/* The boundary-layer top h0 of a layer of thickness r_h: for
+ * h/k_s > H_crit_rel the smaller root of b log(c z + 1) - z = a
+ * (z = -b W(-e^d / (b c)) - 1/c on both real branches), else h
+ * (src/constitutive_2d.f90:821-854, 2011-2044, 2697-2730, 2929-2958). */
+static double vp_h0(const orc_t *o, double r_h) {
+  const imexsw_params *P = &o->p;
+  double h_rel = r_h / P->k_s, h0_rel;
+  if (h_rel > o->h_crit_rel) {
+    double a = h_rel * P->vonK / o->sqrt_ff;
+    double b = 1.0 / 30.0 + h_rel, c = 30.0;
+    double d = a / b - 1.0 / (b * c);
+    double arg = -det_exp(d) / (b * c);
+    double h1 = -(b * orc_lambertw0(arg)) - 1.0 / c;
+    double h2 = -(b * orc_lambertwm1(arg)) - 1.0 / c;
+    h0_rel = fmin(h1, h2);
+  } else {
+    h0_rel = h_rel;
+  }
+  return h0_rel * P->k_s;
+}
+double orc_vp_h0(const orc_t *o, double h) { return vp_h0(o, h); }
+
+/* the quadrature of [0, h0] and log(b z + 1) at its points */
+static void vp_quad(const orc_t *o, double h0, double b, double *z,
+                    double *w, double *lz) {
+  for (int i = 0; i < o->nq; i++) {
+    z[i] = 0.5 * h0 * (o->zq[i] + 1.0);
+    w[i] = 0.5 * h0 * o->wq[i];
+    lz[i] = det_log(b * z[i] + 1.0);
+  }
+}
+
+/* depth average of the log profile: (int_0^h0 log(bz+1) + log(b h0+1)(h-h0))/h */
+static double vp_u_log_avg(const orc_t *o, const double *w, const double *lz,
+                           double lh0, double h, double h0) {
+  double s = 0.0;
+  for (int i = 0; i < o->nq; i++)
+    s += w[i] * lz[i];
+  return (s + lh0 * (h - h0)) / h;
+}
+
+/* Rouse numbers w_s / (vonK u*) with the settling velocity in a carrier of
+ * density rho_c (zero without shear) */
+static void vp_rouse(const orc_t *o, double shear_vel, double rho_c,
+                     double *rouse) {
+  const imexsw_params *P = &o->p;
+  double inv_kin_visc = 1.0 / P->kin_visc_c;
+  for (int i = 0; i < o->ns; i++) {
+    double sv = settling_velocity(o, P->diam_s[i], P->rho_s[i], rho_c,
+                                  inv_kin_visc);
+    rouse[i] = (shear_vel > 0.0) ? sv / (P->vonK * shear_vel) : 0.0;
+  }
+}
+
+/* avg_profiles_mix (src/constitutive_2d.f90:970-1123): the depth-averaged
+ * mixture momentum per unit depth of a flow with mean velocity u_guess and
+ * depth-averaged solid partial densities rho_alphas_avg.  Its dynamic
+ * pressure output is not returned: r_phys_var overwrites it (:957). */
+static double vp_avg_mix(const orc_t *o, double h, const double *sv,
+                         const double *rho_alphas_avg, double u_guess,
+                         double h0, double b, double u_rel0, double rho_c) {
+  const imexsw_params *P = &o->p;
+  const int ns = o->ns, nq = o->nq;
+  double shear_vel = u_guess * o->sqrt_ff;
+  double rouse[MAXS];
+  for (int i = 0; i < ns; i++)
+    rouse[i] = (shear_vel > 0.0) ? sv[i] / (P->vonK * shear_vel) : 0.0;
+  double z[IMEXSW_MAX_QUAD], w[IMEXSW_MAX_QUAD], lz[IMEXSW_MAX_QUAD];
+  vp_quad(o, h0, b, z, w, lz);
+  double eps_s = P->Sc * shear_vel * P->vonK * ((h0 / 6.0) + (P->k_s / 60.0));
+  double a_coeff = -(P->vonK * shear_vel / eps_s);
+  double lh0 = det_log(b * h0 + 1.0);
+  double ncu = 1.0 / vp_u_log_avg(o, w, lz, lh0, h, h0);
+  double rua[MAXS];
+  for (int i = 0; i < ns; i++) {
+    double a = a_coeff * rouse[i];
+    double se = 0.0, sq = 0.0;
+    for (int m = 0; m < nq; m++)
+      se += w[m] * det_exp(a * z[m]);
+    double eh0 = det_exp(a * h0);
+    double nca = 1.0 / ((se + eh0 * (h - h0)) / h);
+    for (int m = 0; m < nq; m++)
+      sq += w[m] * (det_exp(a * z[m]) * lz[m]);
+    double d1 = ncu * nca * sq;
+    double ar0 = nca * eh0;
+    double d2 = (h - h0) * u_rel0 * ar0;
+    rua[i] = rho_alphas_avg[i] * u_guess * (d1 + d2) / h;
+  }
+  double s = 0.0;
+  for (int i = 0; i < ns; i++)
+    s += (P->rho_s[i] - rho_c) / P->rho_s[i] * rua[i];
+  return u_guess * rho_c + s;
+}
+
+/* The mean speed of r_phys_var's profile branch (src/constitutive_2d.f90:
+ * 797-942): u_avg such that the profile-averaged momentum of a flow with
+ * mean speed u_avg is |(q2, q3)| / h, found by the fixed-point map
+ * u <- u |q| / (h M(u)) with Aitken's delta-squared acceleration, at most
+ * ten rounds, tolerances 1e-8 (single-precision literals in the reference). */
+static double vp_velocity(const orc_t *o, double r_h, double q2, double q3,
+                          double r_rho_m, double r_rho_c,
+                          const double *r_alphas) {
+  const imexsw_params *P = &o->p;
+  const int ns = o->ns;
+  double sv[MAXS], ra[MAXS];
+  double inv_kin_visc = 1.0 / P->kin_visc_c;
+  for (int i = 0; i < ns; i++) {
+    ra[i] = r_alphas[i] * P->rho_s[i];
+    sv[i] = settling_velocity(o, P->diam_s[i], P->rho_s[i], r_rho_c,
+                              inv_kin_visc);
+  }
+  double h0 = vp_h0(o, r_h);
+  double b = 30.0 / P->k_s;
+  double z[IMEXSW_MAX_QUAD], w[IMEXSW_MAX_QUAD], lz[IMEXSW_MAX_QUAD];
+  vp_quad(o, h0, b, z, w, lz);
+  double lh0 = det_log(b * h0 + 1.0);
+  double ncu = 1.0 / vp_u_log_avg(o, w, lz, lh0, r_h, h0);
+  double u_rel0 = ncu * lh0;
+  double uRho_avg = sqrt(q2 * q2 + q3 * q3) / r_h;
+  double u_guess = uRho_avg / r_rho_m, u_new = u_guess;
+  const double rel_tol = (double)1.e-8f, abs_tol = (double)1.e-8f;
+  for (int it = 0; it < 10; it++) {
+    double x0 = u_guess;
+    double m = vp_avg_mix(o, r_h, sv, ra, u_guess, h0, b, u_rel0, r_rho_c);
+    u_new = u_guess * uRho_avg / m;
+    double x1 = u_new;
+    m = vp_avg_mix(o, r_h, sv, ra, u_new, h0, b, u_rel0, r_rho_c);
+    u_new = u_new * uRho_avg / m;
+    double x2 = u_new;
+    double den = (x2 - x1) - (x1 - x0);
+    if (fabs(den) < (double)0.1f * abs_tol)
+      break;
+    u_new = x2 - ((x2 - x1) * (x2 - x1)) / den;
+    if ((fabs(u_guess - u_new) / u_guess < rel_tol) ||
+        (fabs(u_guess - u_new) < abs_tol))
+      break;
+    u_guess = u_new;
+  }
+  return u_new;
+}
+
+/* qp_to_qc's profile branch (src/constitutive_2d.f90:1976-2107): the
+ * momentum of a flow of mean velocity (r_u, r_v) with the log velocity and
+ * exponential concentration profiles */
+static void vp_momentum(const orc_t *o, double r_h, double r_u, double r_v,
+                        double r_rho_c, const double *r_alphas, double *qc2,
+                        double *qc3) {
+  const imexsw_params *P = &o->p;
+  const int ns = o->ns, nq = o->nq;
+  double mod_vel = sqrt(r_u * r_u + r_v * r_v + 0.0 * 0.0);
+  double shear_vel = o->sqrt_ff * mod_vel;
+  double rouse[MAXS];
+  vp_rouse(o, shear_vel, r_rho_c, rouse);
+  double h0 = vp_h0(o, r_h);
+  double b = 30.0 / P->k_s;
+  double eps_s = P->Sc * shear_vel * P->vonK * ((h0 / 6.0) + (P->k_s / 60.0));
+  double a_coeff = -(P->vonK * shear_vel / eps_s);
+  double z[IMEXSW_MAX_QUAD], w[IMEXSW_MAX_QUAD], lz[IMEXSW_MAX_QUAD];
+  vp_quad(o, h0, b, z, w, lz);
+  double lh0 = det_log(b * h0 + 1.0);
+  double ncu = 1.0 / vp_u_log_avg(o, w, lz, lh0, r_h, h0);
+  double u_rel0 = ncu * lh0;
+  double rua[MAXS];
+  for (int i = 0; i < ns; i++) {
+    double a = a_coeff * rouse[i];
+    double se = 0.0, sq = 0.0;
+    for (int m = 0; m < nq; m++)
+      se += w[m] * det_exp(a * z[m]);
+    double eh0 = det_exp(a * h0);
+    double nca = 1.0 / ((se + eh0 * (r_h - h0)) / r_h);
+    for (int m = 0; m < nq; m++)
+      sq += w[m] * (det_exp(a * z[m]) * lz[m]);
+    double d1 = (mod_vel * ncu) * (r_alphas[i] * nca) * sq;
+    double ar0 = nca * eh0;
+    double d2 = (r_h - h0) * (mod_vel * u_rel0) * (ar0 * r_alphas[i]);
+    rua[i] = P->rho_s[i] * (d1 + d2) / r_h;
+  }
+  double s = 0.0;
+  for (int i = 0; i < ns; i++)
+    s += (P->rho_s[i] - r_rho_c) / P->rho_s[i] * rua[i];
+  double uRho_avg = mod_vel * r_rho_c + s;
+  *qc2 = r_h * uRho_avg * r_u / mod_vel;
+  *qc3 = r_h * uRho_avg * r_v / mod_vel;
+}
+
+/* eval_flux_coeffs (src/constitutive_2d.f90:2532-2823): the ratios of the
+ * profile-integrated advective fluxes to those of uniform profiles, per
+ * equation (mass, momentum, temperature, solids) */
+static void vp_flux_coeffs(const orc_t *o, const double *qpj, double Bx,
+                           double By, double r_rho_c, double r_rho_m,
+                           double *shape) {
+  const imexsw_params *P = &o->p;
+  const int ns = o->ns, nq = o->nq;
+  for (int i = 0; i < o->ne; i++)
+    shape[i] = 1.0;
+  double r_h = qpj[0], r_hu = qpj[1], r_hv = qpj[2];
+  double r_u = qpj[o->idx_u - 1], r_v = qpj[o->idx_v - 1];
+  double r_alphas[MAXS];
+  for (int i = 0; i < ns; i++)
+    r_alphas[i] = P->alpha_flag ? qpj[4 + i] : qpj[4 + i] / qpj[0];
+  double r_hw = 0.0, r_w = 0.0;
+  if (P->slope_correction_flag) {
+    r_hw = r_hu * Bx + r_hv * By;
+    r_w = r_u * Bx + r_v * By;
+  }
+  double mod_hvel = sqrt(r_hu * r_hu + r_hv * r_hv + r_hw * r_hw);
+  double mod_vel = sqrt(r_u * r_u + r_v * r_v + r_w * r_w);
+  double shear_vel = o->sqrt_ff * mod_vel;
+  double rouse[MAXS];
+  vp_rouse(o, shear_vel, r_rho_c, rouse);
+  double h0 = vp_h0(o, r_h);
+  double b = 30.0 / P->k_s;
+  double z[IMEXSW_MAX_QUAD], w[IMEXSW_MAX_QUAD], lz[IMEXSW_MAX_QUAD];
+  vp_quad(o, h0, b, z, w, lz);
+  double lh0 = det_log(b * h0 + 1.0);
+  double log_term_h0 = lh0 * lh0;
+  double ncu = 1.0 / vp_u_log_avg(o, w, lz, lh0, r_h, h0);
+  double u_rel0 = ncu * lh0;
+  double eps_s = P->Sc * shear_vel * P->vonK * ((h0 / 6.0) + (P->k_s / 60.0));
+  double a_coeff = -(P->vonK * shear_vel / eps_s);
+  double irhv = 0.0, rua[MAXS];
+  for (int i = 0; i < ns; i++) {
+    double a = a_coeff * rouse[i];
+    double se = 0.0, sq = 0.0, sl = 0.0;
+    for (int m = 0; m < nq; m++)
+      se += w[m] * det_exp(a * z[m]);
+    double eh0 = det_exp(a * h0);
+    double nca = 1.0 / ((se + eh0 * (r_h - h0)) / r_h);
+    for (int m = 0; m < nq; m++)
+      sq += w[m] * (det_exp(a * z[m]) * lz[m]);
+    double d1 = (mod_vel * ncu) * (r_alphas[i] * nca) * sq;
+    double ar0 = nca * eh0;
+    double d2 = (r_h - h0) * (mod_vel * u_rel0) * (r_alphas[i] * ar0);
+    rua[i] = P->rho_s[i] * (d1 + d2) / r_h;
+    for (int m = 0; m < nq; m++)
+      sl += det_exp(a * z[m]) * (w[m] * (lz[m] * lz[m]));
+    d2 = (r_h - h0) * eh0 * log_term_h0;
+    irhv = irhv + (P->rho_s[i] - r_rho_c) * (nca * r_alphas[i]) *
+                      (mod_hvel * mod_vel * (ncu * ncu)) * (sl + d2);
+  }
+  double s = 0.0;
+  for (int i = 0; i < ns; i++)
+    s += (P->rho_s[i] - r_rho_c) / P->rho_s[i] * rua[i];
+  double uRho_avg = mod_vel * r_rho_c + s;
+  double s2 = 0.0;
+  for (int m = 0; m < nq; m++)
+    s2 += w[m] * (lz[m] * lz[m]);
+  double ihv = mod_hvel * mod_vel * (ncu * ncu) * (s2 + (r_h - h0) * log_term_h0);
+  double rhom_vel_vel = (irhv + r_rho_c * ihv) / (r_h * r_h);
+  shape[0] = uRho_avg / (mod_vel * r_rho_m);
+  shape[1] = rhom_vel_vel / (uRho_avg * mod_vel);
+  shape[2] = shape[1];
+  shape[3] = shape[0];
+  for (int i = 0; i < ns; i++)
+    shape[4 + i] = rua[i] / (P->rho_s[i] * r_alphas[i] * mod_vel);
+}
+void orc_flux_coeffs(const orc_t *o, const double *qp, double Bx, double By,
+                     double *shape) {
+  double ri, rho_m, rho_c, rg, c1, c2;
+  mixt_var(o, qp, &ri, &rho_m, &rho_c, &rg, 0, &c1, &c2);
+  vp_flux_coeffs(o, qp, Bx, By, rho_c, rho_m, shape);
+}
+
+/* eval_dep_coeffs (src/constitutive_2d.f90:2844-2986): ratio of the bottom
+ * to the depth-averaged concentration of each class.  (The reference fills
+ * dep_coeff(1:n_eqns) of an n_solid array first; only n_solid entries are
+ * written here, and the loop overwrites all of them.) */
+static void vp_dep_coeffs(const orc_t *o, const double *qpj, double mod_vel,
+                          double r_rho_c, double *dep) {
+  const imexsw_params *P = &o->p;
+  const int ns = o->ns, nq = o->nq;
+  double r_h = qpj[0];
+  double shear_vel = o->sqrt_ff * mod_vel;
+  double rouse[MAXS];
+  vp_rouse(o, shear_vel, r_rho_c, rouse);
+  double h0 = vp_h0(o, r_h);
+  double eps_s = P->Sc * shear_vel * P->vonK * ((h0 / 6.0) + (P->k_s / 60.0));
+  double a_coeff = -(P->vonK * shear_vel / eps_s);
+  double z[IMEXSW_MAX_QUAD], w[IMEXSW_MAX_QUAD];
+  for (int m = 0; m < nq; m++) {
+    z[m] = 0.5 * h0 * (o->zq[m] + 1.0);
+    w[m] = 0.5 * h0 * o->wq[m];
+  }
+  for (int i = 0; i < ns; i++) {
+    double a = a_coeff * rouse[i];
+    double se = 0.0;
+    for (int m = 0; m < nq; m++)
+      se += w[m] * det_exp(a * z[m]);
+    dep[i] = 1.0 / ((se + det_exp(a * h0) * (r_h - h0)) / r_h);
+  }
+}
+void orc_dep_coeffs(const orc_t *o, const double *qp, double mod_vel,
+                    double *dep) {
+  double ri, rho_m, rho_c, rg, c1, c2;
+  mixt_var(o, qp, &ri, &rho_m, &rho_c, &rg, 0, &c1, &c2);
+  vp_dep_coeffs(o, qp, mod_vel, rho_c, dep);
+}
+
+/* integrate_friction_term (src/constitutive_2d.f90:3252-3306), rheology
+ * model 8: the speed after dt of du/dt = -f u^2 / h in closed form, the
+ * direction kept, through qc_to_qp and back through qp_to_qc */
+static void integrate_friction(const orc_t *o, double *qj, double dt) {
+  if (qj[0] == 0.0)
+    return;
+  double qp[MAXV + 2], pd;
+  qc_to_qp(o, qj, qp, &pd);
+  double r_h = qp[0], r_u = qp[o->idx_u - 1], r_v = qp[o->idx_v - 1];
+  double mvh = sqrt(r_u * r_u + r_v * r_v);
+  double mv = 1.0 / (1.0 / mvh + o->p.friction_factor / r_h * dt);
+  r_u = r_u * (mv / mvh);
+  r_v = r_v * (mv / mvh);
+  qp[1] = r_h * r_u;
+  qp[2] = r_h * r_v;
+  qp[o->idx_u - 1] = r_u;
+  qp[o->idx_v - 1] = r_v;
+  qp_to_qc(o, qp, qj);
+}
+
 /* eval_mass_exchange_terms, supported branches
  * (src/constitutive_2d.f90:4424-5003) */
 static void mass_exchange_terms(orc_t *o, const double *qpj, double Bx,
@@ -907,11 +1382,17 @@ static void mass_exchange_terms(orc_t *o, const double *qpj, double Bx,
   mixt_var(o, qpj, &ri, &rho_m, &rho_c, &rg, 1, &shc, &shm);
   double inv_kin_visc = 1.0 / P->kin_visc_c;
   double rhoc = rho_c;
+  double dep_coeff[MAXS];
+  if (o->vp && P->bottom_conc_flag)
+    vp_dep_coeffs(o, qpj, mod_vel, rho_c, dep_coeff);
+  else
+    for (int i = 0; i < ns; i++)
+      dep_coeff[i] = 1.0;
   for (int i = 0; i < ns; i++) {
     if ((r_alphas[i] > 0.0) && P->settling_flag) {
       double sv = settling_velocity(o, P->diam_s[i], P->rho_s[i], rhoc,
                                     inv_kin_visc);
-      deposition_term[i] = r_alphas[i] * sv * 1.0;
+      deposition_term[i] = r_alphas[i] * sv * dep_coeff[i];
       deposition_term[i] =
           deposition_term[i] *
           det_pow(1.0 - fmin(1.0, alphas_tot / alpha_max), hind_exp);
@@ -1074,6 +1555,20 @@ orc_t *orc_create(const imexsw_params *p) {
   o->coeff_porosity = (P->erosion_coeff > 0.0)
                           ? P->erodible_porosity / (1.0 - P->erodible_porosity)
                           : 0.0;
+  /* vertical profiles: quadrature and H_crit_rel (src/inpout_2d.f90:
+   * 3761-3791); the shear velocity reads SQRT(friction_factor) */
+  o->vp = P->vertical_profiles_flag;
+  o->sqrt_ff = sqrt(P->friction_factor);
+  o->nq = 0;
+  o->h_crit_rel = 0.0;
+  if (o->vp) {
+    o->nq = P->n_quad;
+    orc_gauss_legendre(o->nq, o->zq, o->wq);
+    double a = P->vonK / o->sqrt_ff + 1.0;
+    double arg = -a * det_exp(-a);
+    double l0 = orc_lambertw0(arg), l1 = orc_lambertwm1(arg);
+    o->h_crit_rel = 1.0 / 30.0 * (-(a / fmax(l0, l1)) - 1.0);
+  }
   /* complex step (src/solver_2d.f90:273-274) */
   o->h_cs = o->nv * EPS_WP;
   o->one_by_h = 1.0 / o->h_cs;
@@ -1958,8 +2453,12 @@ static void eval_flux_KT(orc_t *o) {
       double fL[MAXV], fR[MAXV], avg[MAXV];
       double *H = &o->Hx[(size_t)nv * f];
       const double *qL = &o->qL[(size_t)nv * f], *qR = &o->qR[(size_t)nv * f];
-      eval_fluxes(o, qL, &o->qpL[(size_t)nvp * f], o->gsx[f], 1, fL);
-      eval_fluxes(o, qR, &o->qpR[(size_t)nvp * f], o->gsx[f], 1, fR);
+      size_t cl = I2(j > 1 ? j - 1 : 1, k < o->ny ? k : o->ny);
+      size_t cr = I2(j < o->nx ? j : o->nx, k < o->ny ? k : o->ny);
+      eval_fluxes(o, qL, &o->qpL[(size_t)nvp * f], o->Bx[cl], o->By[cl],
+                  o->gsx[f], 1, fL);
+      eval_fluxes(o, qR, &o->qpR[(size_t)nvp * f], o->Bx[cr], o->By[cr],
+                  o->gsx[f], 1, fR);
       double aN = o->axNeg[f], aP = o->axPos[f];
       for (int i = 0; i < nv; i++)
         avg[i] = (aN == aP) ? 0.0 : (aP * fL[i] - aN * fR[i]) / (aP - aN);
@@ -1995,8 +2494,12 @@ static void eval_flux_KT(orc_t *o) {
       double fB[MAXV], fT[MAXV], avg[MAXV];
       double *H = &o->Hy[(size_t)nv * f];
       const double *qB = &o->qB[(size_t)nv * f], *qT = &o->qT[(size_t)nv * f];
-      eval_fluxes(o, qB, &o->qpB[(size_t)nvp * f], o->gsy[f], 2, fB);
-      eval_fluxes(o, qT, &o->qpT[(size_t)nvp * f], o->gsy[f], 2, fT);
+      size_t cb = I2(j < o->nx ? j : o->nx, k > 1 ? k - 1 : 1);
+      size_t ct = I2(j < o->nx ? j : o->nx, k < o->ny ? k : o->ny);
+      eval_fluxes(o, qB, &o->qpB[(size_t)nvp * f], o->Bx[cb], o->By[cb],
+                  o->gsy[f], 2, fB);
+      eval_fluxes(o, qT, &o->qpT[(size_t)nvp * f], o->Bx[ct], o->By[ct],
+                  o->gsy[f], 2, fT);
       double aN = o->ayNeg[f], aP = o->ayPos[f];
       for (int i = 0; i < nv; i++)
         avg[i] = (aN == aP) ? 0.0 : (aP * fB[i] - aN * fT[i]) / (aP - aN);
@@ -2493,16 +2996,24 @@ void orc_imex_rk(orc_t *o, double t, double dt) {
           nc.dt = dt;
           nc.Bx = o->Bx[c];
           nc.By = o->By[c];
-          int conv;
-          n_iters += solve_rk_step(o, &nc, q_guess, &conv);
-          n_cells++;
-          if (!conv)
-            n_nc++;
+          const int fric8 = P->rheology_flag && P->rheology_model == 8;
+          if (fric8) {
+            /* solve_rk_step returns after integrate_friction_term
+             * (src/solver_2d.f90:1734-1739) */
+            integrate_friction(o, q_guess, dt);
+          } else {
+            int conv;
+            n_iters += solve_rk_step(o, &nc, q_guess, &conv);
+            n_cells++;
+            if (!conv)
+              n_nc++;
+          }
           if (o->ny == 1)
             q_guess[2] = 0.0;
           if (o->nx == 1)
             q_guess[1] = 0.0;
-          {
+          /* model 8: NH from the update alone, no realignment (:1226-1230) */
+          if (!fric8) {
             double nh[MAXV];
             implicit_terms_r(o, q_guess, o->Bx[c], o->By[c], nh);
             for (int i = 1; i <= ne; i++)

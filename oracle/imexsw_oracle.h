@@ -97,6 +97,15 @@ double orc_det_pow(double x, double y);
 double orc_det_cos(double x);
 void orc_source_bdry(const orc_t *o, double time, double vx, double vy,
                      double *out);
+/* vertical profiles: special functions and closures */
+double orc_lambertw0(double x);
+double orc_lambertwm1(double x);
+void orc_gauss_legendre(int n, double *x, double *w);
+double orc_vp_h0(const orc_t *o, double h);
+void orc_flux_coeffs(const orc_t *o, const double *qp, double Bx, double By,
+                     double *shape);
+void orc_dep_coeffs(const orc_t *o, const double *qp, double mod_vel,
+                    double *dep);
 
 #ifdef __cplusplus
 }

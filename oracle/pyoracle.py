@@ -73,6 +73,12 @@ def load_oracle():
         "orc_semi_impl_terms": (None, [C.c_void_p, _D, _D, C.c_double, _D]),
         "orc_local_update": (C.c_int, [C.c_void_p, _D, _D, C.c_double, _D]),
         "orc_source_bdry": (None, [C.c_void_p, C.c_double, C.c_double, C.c_double, _D]),
+        "orc_lambertw0": (C.c_double, [C.c_double]),
+        "orc_lambertwm1": (C.c_double, [C.c_double]),
+        "orc_gauss_legendre": (None, [C.c_int, _D, _D]),
+        "orc_vp_h0": (C.c_double, [C.c_void_p, C.c_double]),
+        "orc_flux_coeffs": (None, [C.c_void_p, _D, C.c_double, C.c_double, _D]),
+        "orc_dep_coeffs": (None, [C.c_void_p, _D, C.c_double, _D]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)

@@ -48,6 +48,11 @@ PROGRAM imexsw_c_check
   CALL off('n_dyn_pres_levels', C_LOC(p%n_dyn_pres_levels))
   CALL off('thickness_levels', C_LOC(p%thickness_levels))
   CALL off('dyn_pres_levels', C_LOC(p%dyn_pres_levels))
+  CALL off('vertical_profiles_flag', C_LOC(p%vertical_profiles_flag))
+  CALL off('n_quad', C_LOC(p%n_quad))
+  CALL off('vonK', C_LOC(p%vonK))
+  CALL off('z_dyn', C_LOC(p%z_dyn))
+  CALL off('friction_factor', C_LOC(p%friction_factor))
   ! bc_flag(i, side) is C bc_flag[side][i]: side 2 (east), variable 3
   CALL off('bc_flag[1][2]', C_LOC(p%bc_flag(3, 2)))
   CALL off('bc_value[3][0]', C_LOC(p%bc_value(1, 4)))
