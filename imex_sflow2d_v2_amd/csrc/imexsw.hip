@@ -340,6 +340,50 @@ unsigned long long sum_slots(const unsigned long long* v) {
   return s;
 }
 
+// Gauss-Legendre points and weights on [-1, 1] in ascending order (the
+// layout gaulegf returns, src/geometry_2d.f90:2407-2449): Newton's method on
+// P_n from cos(pi (i - 1/4) / (n + 1/2)), P_n and P_n' by the three-term
+// recurrence, weights 2 / ((1 - x^2) P_n'(x)^2).
+void gauss_legendre(int n, double* x, double* w) {
+  const double pi = 3.14159265358979323846;
+  for (int i = 0; i < (n + 1) / 2; i++) {
+    double z = std::cos(pi * (i + 0.75) / (n + 0.5)), dp = 0.0;
+    for (int it = 0; it < 100; it++) {
+      double pm = 1.0, pc = z;  // P_0, P_1
+      for (int m = 2; m <= n; m++) {
+        const double pn = ((2.0 * m - 1.0) * z * pc - (m - 1.0) * pm) / m;
+        pm = pc;
+        pc = pn;
+      }
+      if (n == 1) {
+        pc = z;
+        pm = 1.0;
+      }
+      dp = n * (z * pc - pm) / (z * z - 1.0);
+      const double dz = pc / dp;
+      z = z - dz;
+      if (!(std::fabs(dz) > 2.220446049250313e-15)) break;
+    }
+    x[i] = -z;
+    x[n - 1 - i] = z;
+    w[i] = 2.0 / ((1.0 - z * z) * dp * dp);
+    w[n - 1 - i] = w[i];
+  }
+}
+
+// H_crit_rel (src/inpout_2d.f90:3784-3791) with the device's Lambert W and
+// exponential, so that host and device compare h / k_s against one value
+int vp_h_crit(imexsw_ctx* c) {
+  double* d = nullptr;
+  if (dalloc(c, &d, 1)) return -1;
+  hipLaunchKernelGGL(k_vp_hcrit<>, dim3(1), dim3(1), 0, c->stream, c->P.vonK, c->P.sqrt_ff, d);
+  if (hipMemcpyAsync(&c->P.h_crit_rel, d, sizeof(double), hipMemcpyDeviceToHost, c->stream) !=
+          hipSuccess ||
+      hipStreamSynchronize(c->stream) != hipSuccess)
+    return fail(c, "H_crit evaluation failed");
+  return 0;
+}
+
 // The IMEX Runge-Kutta tableaux of allocate_solver_variables
 // (src/solver_2d.f90:398-486): a_tilde_ij and omega_tilde of the explicit
 // part, a_dirk_ij and omega of the implicit part.  N_RK = 3 is
@@ -465,10 +509,34 @@ int imexsw_create(const imexsw_params* params, imexsw_ctx** out) {
   if (p.nx < 2 || p.ny < 1) return fail(nullptr, "grid must have nx >= 2 and ny >= 1");
   if (p.n_solid < 0 || p.n_solid > IMEXSW_MAX_SOLID) return fail(nullptr, "n_solid out of range");
   if (p.n_rk < 1 || p.n_rk > kMaxRK) return fail(nullptr, "N_RK must be 1, 2, 3 or 4 (got %d)", p.n_rk);
-  if (p.rheology_flag && p.rheology_model != 1)
-    return fail(nullptr, "rheology model %d is not supported (only 1, Voellmy-Salm)", p.rheology_model);
-  if (p.vertical_profiles_flag)
-    return fail(nullptr, "vertical profiles are not supported by this build");
+  if (p.rheology_flag && p.rheology_model != 1 && p.rheology_model != 8)
+    return fail(nullptr, "rheology model %d is not supported (1, Voellmy-Salm, or 8)",
+                p.rheology_model);
+  if (p.rheology_flag && p.rheology_model == 8 && !(p.friction_factor >= 0.0))
+    return fail(nullptr, "rheology model 8 needs FRICTION_FACTOR >= 0 (got %g)", p.friction_factor);
+  if (p.vertical_profiles_flag) {
+    // src/inpout_2d.f90:3699-3775
+    if (p.rheology_flag && p.rheology_model != 8)
+      return fail(nullptr, "vertical profiles can be used only with RHEOLOGY_FLAG = F or "
+                           "RHEOLOGY_MODEL = 8");
+    if (p.energy_flag) return fail(nullptr, "vertical profiles can be used only with ENERGY_FLAG = F");
+    if (!(p.vonK > 0.0)) return fail(nullptr, "vertical profiles: vonK must be > 0");
+    if (!(p.k_s > 0.0)) return fail(nullptr, "vertical profiles: k_s must be > 0");
+    if (!(p.Sc > 0.0)) return fail(nullptr, "vertical profiles: Sc must be > 0");
+    if (p.n_quad < 1 || p.n_quad > IMEXSW_MAX_QUAD)
+      return fail(nullptr, "vertical profiles: n_quad must be 1..%d (got %d)", IMEXSW_MAX_QUAD,
+                  p.n_quad);
+    // the shear velocity is SQRT(friction_factor) |u|: without a positive
+    // friction factor every profile quantity is NaN
+    if (!(p.friction_factor > 0.0))
+      return fail(nullptr, "vertical profiles need FRICTION_FACTOR > 0 (got %g)",
+                  p.friction_factor);
+    // the ring overrides put a neighbour's face state on a face side whose
+    // slope the profile coefficients read by position: not reproduced
+    if (p.radial_source_flag && p.slope_correction_flag)
+      return fail(nullptr, "vertical profiles with a radial source and slope correction are "
+                           "not supported");
+  }
   if (p.n_thickness_levels < 0 || p.n_dyn_pres_levels < 0 ||
       p.n_thickness_levels * p.n_dyn_pres_levels > 32 ||
       p.n_thickness_levels > IMEXSW_MAX_LEVELS || p.n_dyn_pres_levels > IMEXSW_MAX_LEVELS)
@@ -565,6 +633,16 @@ int imexsw_create(const imexsw_params* params, imexsw_ctx** out) {
   }
   P.h_cs = (4 + p.n_solid) * kEps;  // src/solver_2d.f90:273-274
   P.one_by_h = 1.0 / P.h_cs;
+  P.ff = p.friction_factor;
+  P.sqrt_ff = std::sqrt(p.friction_factor);
+  P.vp = p.vertical_profiles_flag ? 1 : 0;
+  P.bottom_conc = p.bottom_conc_flag ? 1 : 0;
+  P.nq = P.vp ? p.n_quad : 0;
+  P.vonK = p.vonK;
+  P.k_s = p.k_s;
+  P.Sc = p.Sc;
+  P.h_crit_rel = 0.0;
+  if (P.vp) gauss_legendre(P.nq, P.zq, P.wq);
 
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
     fail(nullptr, "hipStreamCreate failed");
@@ -609,9 +687,11 @@ int imexsw_create(const imexsw_params* params, imexsw_ctx** out) {
   c->grid_mask = dim3((P.nx + kStripW - 1) / kStripW, (P.ny + kTileH - 1) / kTileH);
   c->grid_stag = dim3((P.nx + 1 + kTileX - 1) / kTileX, (P.ny + 1 + kTileY - 1) / kTileY);
   c->n_tiles = (int)(c->grid_mask.x * c->grid_mask.y);
-  if (p.n_rk != 2) {
-    // the tableau-driven stage pipeline: stage states, residuals and
-    // implicit terms of every stage, two byte planes for the mask dilation
+  if (p.n_rk != 2 || P.vp || (P.rheo && P.rheo_model == 8)) {
+    // the tableau-driven stage pipeline (N_RK != 2, and every N_RK with
+    // the profile closures or rheology model 8): stage states, residuals
+    // and implicit terms of every stage, two byte planes for the mask
+    // dilation
     c->general = true;
     fill_tableau(p.n_rk, c->tab);
     const size_t sp = (size_t)c->nv * pl * p.n_rk;
@@ -639,6 +719,7 @@ int imexsw_create(const imexsw_params* params, imexsw_ctx** out) {
       if (!rc && resolve_scratch(c, dev)) rc = 1;
     }
   }
+  if (!rc && P.vp && vp_h_crit(c)) rc = 1;
   if (rc) {
     g_create_error = c->err;
     imexsw_destroy(c);
@@ -1093,6 +1174,84 @@ int imexsw_get_dt(imexsw_ctx* c, double* dt) {
   return 0;
 }
 
+// n packed states through k_convert on the context's stream, with device
+// buffers that live for the call (host set-up helpers, not step-path code)
+static int convert_points(imexsw_ctx* c, int64_t n, const double* in, const double* bx,
+                          const double* by, double* out, int mode) {
+  if (!c) return fail(c, "null argument");
+  if (n < 0) return fail(c, "n must be >= 0 (got %lld)", (long long)n);
+  if (n == 0) return 0;
+  if (!in || !out || (mode == 2 && (!bx || !by))) return fail(c, "null argument");
+  const size_t nin = (size_t)n * (mode == 1 ? c->nv : c->nv + 2);
+  const size_t nout = (size_t)n * (mode == 1 ? c->nv + 2 : c->nv);
+  const size_t nb = mode == 2 ? (size_t)n : 0;
+  double* buf = nullptr;
+  HIPCHK(c, hipMalloc((void**)&buf, (nin + nout + 2 * nb) * sizeof(double)));
+  double *din = buf, *dout = buf + nin, *dbx = buf + nin + nout, *dby = dbx + nb;
+  int rc = 0;
+  if (hipMemcpyAsync(din, in, nin * sizeof(double), hipMemcpyHostToDevice, c->stream) !=
+          hipSuccess ||
+      (nb && hipMemcpyAsync(dbx, bx, nb * sizeof(double), hipMemcpyHostToDevice, c->stream) !=
+                 hipSuccess) ||
+      (nb && hipMemcpyAsync(dby, by, nb * sizeof(double), hipMemcpyHostToDevice, c->stream) !=
+                 hipSuccess)) {
+    rc = fail(c, "upload of the states failed");
+  } else {
+#define CALL(N) launch_convert<N>(c->P, c->stream, c->n_cu, n, din, dbx, dby, dout, mode)
+    NS_SWITCH(c->ns, CALL)
+#undef CALL
+    if (check_launch(c, "k_convert") ||
+        hipMemcpyAsync(out, dout, nout * sizeof(double), hipMemcpyDeviceToHost, c->stream) !=
+            hipSuccess ||
+        hipStreamSynchronize(c->stream) != hipSuccess)
+      rc = fail(c, "point conversion failed");
+  }
+  (void)hipFree(buf);
+  return rc;
+}
+
+int imexsw_qp_to_qc(imexsw_ctx* c, int64_t n, const double* qp, double* qc) {
+  return convert_points(c, n, qp, nullptr, nullptr, qc, 0);
+}
+
+int imexsw_qc_to_qp(imexsw_ctx* c, int64_t n, const double* qc, double* qp) {
+  return convert_points(c, n, qc, nullptr, nullptr, qp, 1);
+}
+
+int imexsw_flux_coeffs(imexsw_ctx* c, int64_t n, const double* qp, const double* Bx,
+                       const double* By, double* shape) {
+  return convert_points(c, n, qp, Bx, By, shape, 2);
+}
+
+int imexsw_lambertw(int32_t branch, int64_t n, const double* x, double* w) {
+  if (branch != 0 && branch != -1) return fail(nullptr, "branch must be 0 or -1");
+  if (n < 0 || (n > 0 && (!x || !w))) return fail(nullptr, "bad argument");
+  if (n == 0) return 0;
+  double* buf = nullptr;
+  if (hipMalloc((void**)&buf, 2 * (size_t)n * sizeof(double)) != hipSuccess)
+    return fail(nullptr, "hipMalloc failed");
+  int rc = 0;
+  const int grid = (int)std::min<int64_t>((n + 255) / 256, 1024);
+  if (hipMemcpy(buf, x, (size_t)n * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) {
+    rc = fail(nullptr, "upload failed");
+  } else {
+    hipLaunchKernelGGL(k_lambertw<>, dim3(grid), dim3(256), 0, nullptr, (int)branch,
+                       (long long)n, buf, buf + n);
+    if (hipGetLastError() != hipSuccess ||
+        hipMemcpy(w, buf + n, (size_t)n * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
+      rc = fail(nullptr, "Lambert W evaluation failed");
+  }
+  (void)hipFree(buf);
+  return rc;
+}
+
+int imexsw_gauss_legendre(int32_t n, double* x, double* w) {
+  if (n < 1 || n > IMEXSW_MAX_QUAD || !x || !w)
+    return fail(nullptr, "n must be 1..%d and x, w non-null", IMEXSW_MAX_QUAD);
+  gauss_legendre(n, x, w);
+  return 0;
+}
+
 int imexsw_stats_get(imexsw_ctx* c, imexsw_stats* st, int reset) {
   if (!c || !st) return fail(c, "null argument");
   std::vector<unsigned long long> raw((size_t)C_NCOUNTERS * kSlots);
@@ -1380,6 +1539,9 @@ int imexsw_ensemble_create(int32_t n_members, const imexsw_params* params,
     if (p.n_solid != ns) return efail(nullptr, "member %d: all members need the same n_solid", m);
     if (p.ny < 2) return efail(nullptr, "member %d: ensembles need 2-D grids", m);
     if (p.n_rk != 2) return efail(nullptr, "member %d: ensembles run N_RK = 2", m);
+    if (p.vertical_profiles_flag || (p.rheology_flag && p.rheology_model == 8))
+      return efail(nullptr, "member %d: ensembles run without vertical profiles and with "
+                            "rheology model 1", m);
     if (p.topo_change_flag || p.radial_source_flag || p.slope_correction_flag || p.curvature_term_flag)
       return efail(nullptr, "member %d: ensembles run the flat-source-free model (no topography "
                             "change, radial source, slope correction or curvature)", m);

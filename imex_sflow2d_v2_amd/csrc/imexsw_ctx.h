@@ -136,7 +136,13 @@ void launch_local(imexsw_ctx* c, int ops, bool fused);
 // qp of rows k0 .. k1-1 into dst in the host layout
 template <int NS>
 void launch_qp(imexsw_ctx* c, double* dst, int k0, int k1);
-// N_RK != 2: stage `st` of the tableau and the assembly (imexsw_stages.h)
+// point conversions of n packed states (k_convert): mode 0 qp_to_qc,
+// 1 qc_to_qp, 2 the profile flux coefficients
+template <int NS>
+void launch_convert(const DevParams& P, hipStream_t s, int n_cu, long long n, const double* in,
+                    const double* bx, const double* by, double* out, int mode);
+// the tableau-driven path (N_RK != 2, vertical profiles, rheology model 8):
+// stage `st` of the tableau and the assembly (imexsw_stages.h)
 template <int NS>
 void launch_stage(imexsw_ctx* c, int st, int policy);
 template <int NS>

@@ -795,7 +795,7 @@ __device__ __forceinline__ int stage2(const DevParams& P, const double* q0,
 // solid classes' settling velocities are split between them (class
 // LANES*s + r on lane r, the same instructions on every lane) and
 // exchanged, so the cell waits for about 1/LANES of them.
-template <int NS, int LANES = 1>
+template <int NS, int LANES = 1, bool VP = false>
 __device__ __forceinline__ void mass_exchange(const DevParams& P, const double* qp,
                                               double Bx, double By,
                                               const double* erodible, double dt,
@@ -838,6 +838,11 @@ __device__ __forceinline__ void mass_exchange(const DevParams& P, const double* 
   double r_T = qp[3];
   Mixt m = mixt<NS, true>(P, qp);
   double inv_kin_visc = 1.0 / P.kin_visc_c;
+  // bottom over mean concentration with the profiles (:4676-4683)
+  double dcf[NS > 0 ? NS : 1];
+#pragma unroll
+  for (int i = 0; i < NS; i++) dcf[i] = 1.0;
+  if (VP && P.bottom_conc) vp_dep_coeffs<NS>(P, r_h, mv, m.rho_c, dcf);
   if (P.settling) {
     // hindered settling factor, the same for every class
     const double hind = det_pow(1.0 - fmin(1.0, at / alpha_max), hind_exp);
@@ -878,7 +883,7 @@ __device__ __forceinline__ void mass_exchange(const DevParams& P, const double* 
         const double sv = (LANES > 1) ? svs[i]
                                       : settling_velocity(P, P.diam_s[i], P.rho_s[i], m.rho_c,
                                                           inv_kin_visc);
-        double d = al[i] * sv * 1.0;
+        double d = al[i] * sv * dcf[i];
         d = d * hind;
         dep[i] = fmin(d, r_h * al[i] / dt);
       }

@@ -18,6 +18,20 @@ void launch_stencil(imexsw_ctx* c, const DevPtrs& D, int compute_dt, bool rows) 
     if (!res) res = resident_grid(k_stencil_rows<NS>, 384, c->n_cu, 1 << 30);
     hipLaunchKernelGGL((k_stencil_rows<NS>), dim3(std::min(c->n_tiles * kTileH, res)),
                        dim3(384), 0, c->stream, c->P, D, compute_dt);
+  } else if (c->P.vp) {
+    // the profile closures (general path only)
+    static int res2 = 0, res1 = 0;
+    if (c->P.ny > 1) {
+      if (!res2) res2 = resident_grid(k_stencil<NS, true, true>, 64, c->n_cu, 1 << 30);
+      hipLaunchKernelGGL((k_stencil<NS, true, true>), dim3(std::min(items, res2)), dim3(64), 0,
+                         c->stream, c->P, D, c->row_unit, c->strip_h, c->target_strips,
+                         compute_dt);
+    } else {
+      if (!res1) res1 = resident_grid(k_stencil<NS, false, true>, 64, c->n_cu, 1 << 30);
+      hipLaunchKernelGGL((k_stencil<NS, false, true>), dim3(std::min(items, res1)), dim3(64), 0,
+                         c->stream, c->P, D, c->row_unit, c->strip_h, c->target_strips,
+                         compute_dt);
+    }
   } else if (c->P.ny > 1) {
     static int res = 0;
     if (!res) res = resident_grid(k_stencil<NS, true>, 64, c->n_cu, 1 << 30);
@@ -75,31 +89,62 @@ void launch_local(imexsw_ctx* c, int ops, bool fused) {
       hipLaunchKernelGGL((k_local<NS, 2>), dim3(std::min(items, res2)), dim3(256), 0, c->stream,
                          c->P, c->D, rk);
   }
-  if (ops & (OP_EROSION | OP_POST | OP_SUMS))
+  if ((ops & (OP_EROSION | OP_POST | OP_SUMS)) && c->P.vp) {
+    static int res1v = 0;
+    if (!res1v) res1v = resident_grid(k_local<NS, 1, true>, 256, c->n_cu, 1 << 30);
+    hipLaunchKernelGGL((k_local<NS, 1, true>), dim3(std::min(items, res1v)), dim3(256), 0,
+                       c->stream, c->P, c->D, ops & ~(int)OP_RK);
+  } else if (ops & (OP_EROSION | OP_POST | OP_SUMS)) {
     hipLaunchKernelGGL((k_local<NS, 1>), dim3(std::min(items, res1)), dim3(256), 0, c->stream,
                        c->P, c->D, ops & ~(int)OP_RK);
+  }
 }
 template <int NS>
 void launch_qp(imexsw_ctx* c, double* dst, int k0, int k1) {
   const dim3 grid(c->grid_tiles.x, (k1 - k0 + kTileY - 1) / kTileY);
-  hipLaunchKernelGGL((k_qp_out<NS>), grid, dim3(256), 0, c->stream, c->P, c->D.q, dst, k0, k1);
+  if (c->P.vp)
+    hipLaunchKernelGGL((k_qp_out<NS, true>), grid, dim3(256), 0, c->stream, c->P, c->D.q, dst,
+                       k0, k1);
+  else
+    hipLaunchKernelGGL((k_qp_out<NS>), grid, dim3(256), 0, c->stream, c->P, c->D.q, dst, k0, k1);
+}
+template <int NS>
+void launch_convert(const DevParams& P, hipStream_t s, int n_cu, long long n, const double* in,
+                    const double* bx, const double* by, double* out, int mode) {
+  const int grid = (int)std::min<long long>((n + 255) / 256, (long long)n_cu * 8);
+  if (P.vp)
+    hipLaunchKernelGGL((k_convert<NS, true>), dim3(grid), dim3(256), 0, s, P, n, in, bx, by, out,
+                       mode);
+  else
+    hipLaunchKernelGGL((k_convert<NS, false>), dim3(grid), dim3(256), 0, s, P, n, in, bx, by,
+                       out, mode);
 }
 
-template <int NS>
-void launch_stage(imexsw_ctx* c, int st, int policy) {
+template <int NS, bool VP>
+void launch_stage_v(imexsw_ctx* c, int st, int policy) {
   const int items = (c->n_tiles * kTileH + 3) / 4;
   static int res = 0;
-  if (!res) res = resident_grid(kg_stage<NS>, 256, c->n_cu, 1 << 30);
-  hipLaunchKernelGGL((kg_stage<NS>), dim3(std::min(items, res)), dim3(256), 0, c->stream, c->P,
-                     c->D, c->G, c->tab, st, policy);
+  if (!res) res = resident_grid(kg_stage<NS, VP>, 256, c->n_cu, 1 << 30);
+  hipLaunchKernelGGL((kg_stage<NS, VP>), dim3(std::min(items, res)), dim3(256), 0, c->stream,
+                     c->P, c->D, c->G, c->tab, st, policy);
+}
+template <int NS>
+void launch_stage(imexsw_ctx* c, int st, int policy) {
+  if (c->P.vp) launch_stage_v<NS, true>(c, st, policy);
+  else launch_stage_v<NS, false>(c, st, policy);
+}
+template <int NS, bool VP>
+void launch_assemble_v(imexsw_ctx* c, int policy) {
+  const int items = (c->n_tiles * kTileH + 3) / 4;
+  static int res = 0;
+  if (!res) res = resident_grid(kg_assemble<NS, VP>, 256, c->n_cu, 1 << 30);
+  hipLaunchKernelGGL((kg_assemble<NS, VP>), dim3(std::min(items, res)), dim3(256), 0, c->stream,
+                     c->P, c->D, c->G, c->tab, policy);
 }
 template <int NS>
 void launch_assemble(imexsw_ctx* c, int policy) {
-  const int items = (c->n_tiles * kTileH + 3) / 4;
-  static int res = 0;
-  if (!res) res = resident_grid(kg_assemble<NS>, 256, c->n_cu, 1 << 30);
-  hipLaunchKernelGGL((kg_assemble<NS>), dim3(std::min(items, res)), dim3(256), 0, c->stream,
-                     c->P, c->D, c->G, c->tab, policy);
+  if (c->P.vp) launch_assemble_v<NS, true>(c, policy);
+  else launch_assemble_v<NS, false>(c, policy);
 }
 
 template <int NS>
@@ -129,6 +174,9 @@ template void launch_stage<IMEXSW_NS>(imexsw_ctx*, int, int);
 template void launch_assemble<IMEXSW_NS>(imexsw_ctx*, int);
 template void launch_local<IMEXSW_NS>(imexsw_ctx*, int, bool);
 template void launch_qp<IMEXSW_NS>(imexsw_ctx*, double*, int, int);
+template void launch_convert<IMEXSW_NS>(const DevParams&, hipStream_t, int, long long,
+                                        const double*, const double*, const double*, double*,
+                                        int);
 #if IMEXSW_NS >= 1 && IMEXSW_NS <= 3
 template void launch_batch_stencil<IMEXSW_NS>(imexsw_ensemble*);
 template void launch_batch_local<IMEXSW_NS>(imexsw_ensemble*);

@@ -47,6 +47,11 @@ struct DevParams {
       alphas_source[IMEXSW_MAX_SOLID], time_param[4], pi_g;
   double thickness_levels[IMEXSW_MAX_LEVELS], dyn_pres_levels[IMEXSW_MAX_LEVELS];
   double h_cs, one_by_h;  // complex step n_vars*EPSILON and its reciprocal
+  // vertical profiles: Gauss-Legendre rule on [-1, 1], the closure constants,
+  // H_crit_rel (src/inpout_2d.f90:3761-3791); SQRT(friction_factor)
+  int vp, bottom_conc, nq;
+  double vonK, k_s, Sc, sqrt_ff, ff, h_crit_rel;
+  double zq[IMEXSW_MAX_QUAD], wq[IMEXSW_MAX_QUAD];
 };
 
 // ---------------------------------------------------------------------------
@@ -141,12 +146,38 @@ __device__ __forceinline__ double limit_(const DevParams& P, double v0, double v
   return c;  // 5: centred
 }
 
+// settling_velocity (src/constitutive_2d.f90:5148-5214)
+__device__ __forceinline__ double settling_velocity(const DevParams& P, double diam,
+                                                    double rhos, double rhoc,
+                                                    double inv_kin_visc) {
+  if (rhos <= rhoc) return 0.0;
+  double cp = sqrt(0.75 * (rhos / rhoc - 1.0) * diam * P.grav);
+  double sv = cp * 1.0;
+  double Rey = diam * sv * inv_kin_visc;
+  if (Rey <= 1000.0) {
+    for (int i = 1; i <= 20; i++) {
+      double old = sv;
+      double isc = sqrt(Rey / (24.0 * (1.0 + 0.15 * det_pow(Rey, 0.687))));
+      sv = cp * isc;
+      if (fabs(old - sv) / old < 1.0e-6) return sv;
+      Rey = diam * sv * inv_kin_visc;
+    }
+  }
+  return sv;
+}
+
+}  // namespace imexsw
+
+#include "imexsw_profiles.h"
+
+namespace imexsw {
+
 // ---------------------------------------------------------------------------
 // variable transforms (src/constitutive_2d.f90:541-967, 1333-2179)
 // ---------------------------------------------------------------------------
-// qc_to_qp (r_phys_var, non-profile, gas carrier).  qp = h, hu, hv, T,
-// (h*)alphas, u, v; returns p_dyn.
-template <int NS>
+// qc_to_qp (r_phys_var, gas carrier; VP: the profile branch's mean speed).
+// qp = h, hu, hv, T, (h*)alphas, u, v; returns p_dyn.
+template <int NS, bool VP = false>
 __device__ __forceinline__ double phys(const DevParams& P, const double* qj,
                                        double* qp) {
   constexpr int NV = 4 + NS;
@@ -199,7 +230,15 @@ __device__ __forceinline__ double phys(const DevParams& P, const double* qj,
   double rho_m = 1.0 / inv_rhom;
   double h = qj[0] * inv_rhom;
   double u, v;
-  if (qj[0] > P.eps_sing) {
+  if (VP) {
+    double al[NS > 0 ? NS : 1];
+#pragma unroll
+    for (int i = 0; i < NS; i++) al[i] = xs[i] * rho_m * P.inv_rho_s[i];
+    const double rho_c = P.pres / (rgc * T);
+    const double ua = vp_velocity<NS>(P, h, qj[1], qj[2], rho_m, rho_c, al);
+    u = ua * qj[1] / (sqrt(qj[1] * qj[1] + qj[2] * qj[2]));
+    v = ua * qj[2] / (sqrt(qj[1] * qj[1] + qj[2] * qj[2]));
+  } else if (qj[0] > P.eps_sing) {
     u = qj[1] * inv_qj1;
     v = qj[2] * inv_qj1;
   } else {
@@ -222,11 +261,11 @@ __device__ __forceinline__ double phys(const DevParams& P, const double* qj,
 }
 
 // "phys of q if q1 > 0 else the dry state" as the solver loops write it
-template <int NS>
+template <int NS, bool VP = false>
 __device__ __forceinline__ double phys_or_dry(const DevParams& P, const double* q,
                                               double* qp) {
   constexpr int NV = 4 + NS;
-  if (q[0] > 0.0) return phys<NS>(P, q, qp);
+  if (q[0] > 0.0) return phys<NS, VP>(P, q, qp);
 #pragma unroll
   for (int i = 0; i < NV + 2; i++) qp[i] = 0.0;
   qp[3] = P.T_ambient;
@@ -290,8 +329,9 @@ __device__ __forceinline__ Mixt mixt(const DevParams& P, const double* qp) {
   return m;
 }
 
-// qp_to_qc (src/constitutive_2d.f90:1709-2140, non-profile, no liquid)
-template <int NS>
+// qp_to_qc (src/constitutive_2d.f90:1709-2140, no liquid; VP: the profile
+// momentum)
+template <int NS, bool VP = false>
 __device__ __forceinline__ void qp2qc(const DevParams& P, const double* qp,
                                       double* qc) {
   constexpr int NV = 4 + NS;
@@ -302,12 +342,12 @@ __device__ __forceinline__ void qp2qc(const DevParams& P, const double* qp,
     return;
   }
   double r_hu = qp[1], r_hv = qp[2], r_u = qp[NV], r_v = qp[NV + 1], r_T = qp[3];
-  double ars[NS > 0 ? NS : 1], at = 0.0;
+  double ars[NS > 0 ? NS : 1], al[NS > 0 ? NS : 1], at = 0.0;
 #pragma unroll
   for (int i = 0; i < NS; i++) {
-    double al = P.alpha ? qp[4 + i] : qp[4 + i] / qp[0];
-    at += al;
-    ars[i] = al * P.rho_s[i];
+    al[i] = P.alpha ? qp[4 + i] : qp[4 + i] / qp[0];
+    at += al[i];
+    ars[i] = al[i] * P.rho_s[i];
   }
   double rho_a = P.pres / (P.sp_gas_const_a * r_T);
   double rho_c = ((1.0 - 0.0 - at - 0.0) * rho_a + 0.0) / (1.0 - 0.0 - at);
@@ -327,8 +367,16 @@ __device__ __forceinline__ void qp2qc(const DevParams& P, const double* qp,
   for (int i = 0; i < NS; i++) dsh += xs[i] * P.sp_heat_s[i];
   double shm = dsh + xc * shc;
   qc[0] = rho_m * r_h;
-  qc[1] = rho_m * r_hu;
-  qc[2] = rho_m * r_hv;
+  if (VP) {
+    // the alphas corrections of :1911-1919 (only the profiles read them)
+    double ra[NS > 0 ? NS : 1];
+#pragma unroll
+    for (int i = 0; i < NS; i++) ra[i] = (at > 1.0) ? al[i] / at : (at < 0.0) ? 0.0 : al[i];
+    vp_momentum<NS>(P, r_h, r_u, r_v, rho_c, ra, qc[1], qc[2]);
+  } else {
+    qc[1] = rho_m * r_hu;
+    qc[2] = rho_m * r_hv;
+  }
   if (P.energy)
     qc[3] = (r_h > 0.0) ? r_h * rho_m * (shm * r_T + 0.5 * (r_u * r_u + r_v * r_v))
                         : 0.0;
@@ -358,9 +406,12 @@ __device__ __forceinline__ void speeds(const DevParams& P, const double* qp,
   }
 }
 
-template <int NS, int DIR>
+// eval_fluxes; VP: with the profile shape coefficients, which read the slope
+// (Bx, By) of the cell the face state belongs to
+template <int NS, int DIR, bool VP = false>
 __device__ __forceinline__ void fluxes(const DevParams& P, const double* qc,
-                                       const double* qp, double gc, double* f) {
+                                       const double* qp, double gc, double* f,
+                                       double Bx = 0.0, double By = 0.0) {
   constexpr int NV = 4 + NS;
   if (!(qp[0] > kEps)) {
 #pragma unroll
@@ -369,19 +420,23 @@ __device__ __forceinline__ void fluxes(const DevParams& P, const double* qc,
   }
   double r_h = qp[0], r_u = qp[NV], r_v = qp[NV + 1];
   Mixt m = mixt<NS, false>(P, qp);
+  double sc[NV];
+#pragma unroll
+  for (int i = 0; i < NV; i++) sc[i] = 1.0;
+  if (VP) vp_flux_coeffs<NS>(P, qp, Bx, By, m.rho_c, m.rho_m, sc);
   double vel = (DIR == 1) ? r_u : r_v;
   double pt = 0.5 * m.rho_m * gc * m.rg * (r_h * r_h);
-  f[0] = vel * qc[0] * 1.0;
+  f[0] = vel * qc[0] * sc[0];
   if (DIR == 1) {
-    f[1] = r_u * qc[1] * 1.0 + pt;
-    f[2] = r_u * qc[2] * 1.0;
+    f[1] = r_u * qc[1] * sc[1] + pt;
+    f[2] = r_u * qc[2] * sc[2];
   } else {
-    f[1] = r_v * qc[1] * 1.0;
-    f[2] = r_v * qc[2] * 1.0 + pt;
+    f[1] = r_v * qc[1] * sc[1];
+    f[2] = r_v * qc[2] * sc[2] + pt;
   }
-  f[3] = P.energy ? vel * (qc[3] * 1.0 + pt) : vel * qc[3] * 1.0;
+  f[3] = P.energy ? vel * (qc[3] * sc[3] + pt) : vel * qc[3] * sc[3];
 #pragma unroll
-  for (int i = 0; i < NS; i++) f[4 + i] = vel * qc[4 + i] * 1.0;
+  for (int i = 0; i < NS; i++) f[4 + i] = vel * qc[4 + i] * sc[4 + i];
   double sf = 0.0;
 #pragma unroll
   for (int i = 0; i < NS; i++) sf += f[4 + i];
@@ -547,26 +602,6 @@ __device__ __forceinline__ void implicit_c(const DevParams& P, const cpx* cq,
     n1 = csub(n1, cmul(turb, cdiv(u, mv)));
     n2 = csub(n2, cmul(turb, cdiv(v, mv)));
   }
-}
-
-// settling_velocity (src/constitutive_2d.f90:5148-5214)
-__device__ __forceinline__ double settling_velocity(const DevParams& P, double diam,
-                                                    double rhos, double rhoc,
-                                                    double inv_kin_visc) {
-  if (rhos <= rhoc) return 0.0;
-  double cp = sqrt(0.75 * (rhos / rhoc - 1.0) * diam * P.grav);
-  double sv = cp * 1.0;
-  double Rey = diam * sv * inv_kin_visc;
-  if (Rey <= 1000.0) {
-    for (int i = 1; i <= 20; i++) {
-      double old = sv;
-      double isc = sqrt(Rey / (24.0 * (1.0 + 0.15 * det_pow(Rey, 0.687))));
-      sv = cp * isc;
-      if (fabs(old - sv) / old < 1.0e-6) return sv;
-      Rey = diam * sv * inv_kin_visc;
-    }
-  }
-  return sv;
 }
 
 // eval_source_bdry (src/constitutive_2d.f90:5024-5130), split into the

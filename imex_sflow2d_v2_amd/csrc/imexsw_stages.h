@@ -139,6 +139,28 @@ __global__ void kg_stage_init(DevParams P, const double* scal, const double* q, 
     qrk[e] = q[e % per];
 }
 
+// integrate_friction_term (src/constitutive_2d.f90:3252-3306), rheology
+// model 8: the speed after dt of du/dt = -f u^2 / h in closed form, the
+// direction kept, through qc_to_qp and back through qp_to_qc
+template <int NS, bool VP>
+__device__ __forceinline__ void integrate_friction(const DevParams& P, double* qj, double dt) {
+  constexpr int NV = 4 + NS;
+  if (qj[0] == 0.0) return;
+  double qp[NV + 2];
+  phys<NS, VP>(P, qj, qp);
+  const double r_h = qp[0];
+  double r_u = qp[NV], r_v = qp[NV + 1];
+  const double mvh = sqrt(r_u * r_u + r_v * r_v);
+  const double mv = 1.0 / (1.0 / mvh + P.ff / r_h * dt);
+  r_u = r_u * (mv / mvh);
+  r_v = r_v * (mv / mvh);
+  qp[1] = r_h * r_u;
+  qp[2] = r_h * r_v;
+  qp[NV] = r_u;
+  qp[NV + 1] = r_v;
+  qp2qc<NS, VP>(P, qp, qj);
+}
+
 // ---------------------------------------------------------------------------
 // one stage of imex_RK_solver for one cell (src/solver_2d.f90:1105-1322)
 // ---------------------------------------------------------------------------
@@ -146,7 +168,7 @@ __global__ void kg_stage_init(DevParams P, const double* scal, const double* q, 
 // summed over the stages before this one, in stage order from 0.0 (row i_RK
 // of the tableau without its diagonal); out: the stage state and NH + SI_NH
 // of the stage.  Returns the Newton iteration count (0: no Newton).
-template <int NS>
+template <int NS, bool VP = false>
 __device__ __forceinline__ int stage_cell(const DevParams& P, const double* q0,
                                           const double* s1p, const double* s2p,
                                           double a_diag, double dt, double Bx, double By,
@@ -168,7 +190,7 @@ __device__ __forceinline__ int stage_cell(const DevParams& P, const double* q0,
     q_guess[i] = q0[i];
     si_nh[i] = 0.0;
   }
-  phys<NS>(P, q_fv, qp);
+  phys<NS, VP>(P, q_fv, qp);
   int iters = 0;
   if (a_diag != 0.0) {
     if (q_fv[0] > 0.0) {
@@ -198,15 +220,24 @@ __device__ __forceinline__ int stage_cell(const DevParams& P, const double* q0,
         q_guess[i] = q_si[i];
         Rj[i] = Rj[i] - a_diag * si_nh[i];
       }
-      const NewtonOut no = solve_rk_step<NS>(P, q_guess, q0, Rj, a_diag, dt, Bx, By);
-      iters = no.iters;
-      nonconv = !no.conv;
-      ls_evals = no.ls_evals;
+      // rheology model 8: solve_rk_step integrates the friction in closed
+      // form and returns (src/solver_2d.f90:1734-1739)
+      const bool fric8 = P.rheo && P.rheo_model == 8;
+      if (fric8) {
+        integrate_friction<NS, VP>(P, q_guess, dt);
+      } else {
+        const NewtonOut no = solve_rk_step<NS>(P, q_guess, q0, Rj, a_diag, dt, Bx, By);
+        iters = no.iters;
+        nonconv = !no.conv;
+        ls_evals = no.ls_evals;
+      }
       if (P.ny == 1) q_guess[2] = 0.0;
       if (P.nx == 1) q_guess[1] = 0.0;
       // (the implicit-term refresh of :1234-1236 is overwritten at
-      // :1271-1277 below and is not evaluated)
-      if (q_si[1] * q_si[1] + q_si[2] * q_si[2] == 0.0) {
+      // :1271-1277 below and is not evaluated; model 8 keeps the update
+      // as it is, :1226-1230)
+      if (fric8) {
+      } else if (q_si[1] * q_si[1] + q_si[2] * q_si[2] == 0.0) {
         q_guess[1] = 0.0;
         q_guess[2] = 0.0;
       } else if ((q_guess[1] * q_si[1] <= 0.0) && (q_guess[2] * q_si[2] <= 0.0)) {
@@ -248,7 +279,7 @@ struct StageCounters {
   int imax;
 };
 
-template <int NS>
+template <int NS, bool VP = false>
 __global__ void __launch_bounds__(256) kg_stage(DevParams P, DevPtrs D, RkPtrs G, RkTableau T,
                                                 int st, int policy) {
   constexpr int NV = 4 + NS;
@@ -301,8 +332,8 @@ __global__ void __launch_bounds__(256) kg_stage(DevParams P, DevPtrs D, RkPtrs G
     const double bxx = D.Bxx ? D.Bxx[c] : 0.0, bxy = D.Bxy ? D.Bxy[c] : 0.0,
                  byy = D.Byy ? D.Byy[c] : 0.0, gc = D.gcoef ? D.gcoef[c] : 1.0;
     int nonconv, ls;
-    const int its = stage_cell<NS>(P, q0, s1, s2, a_diag, dt, D.Bx[c], D.By[c], bxx, bxy, byy,
-                                   gc, qr, im, nonconv, ls);
+    const int its = stage_cell<NS, VP>(P, q0, s1, s2, a_diag, dt, D.Bx[c], D.By[c], bxx, bxy,
+                                       byy, gc, qr, im, nonconv, ls);
 #pragma unroll
     for (int i = 0; i < NV; i++) {
       G.qrk[st * sp + i * pl + c] = qr[i];
@@ -336,7 +367,7 @@ __global__ void __launch_bounds__(256) kg_stage(DevParams P, DevPtrs D, RkPtrs G
 // assemble_sol (src/solver_2d.f90:1341-1626): q = q_rk(N_RK) for a stiffly
 // accurate tableau, else q0 - dt * residual with residual =
 // (divFlux - expl) . omega_tilde - (NH + SI_NH) . omega; then the checks
-template <int NS>
+template <int NS, bool VP = false>
 __global__ void __launch_bounds__(256) kg_assemble(DevParams P, DevPtrs D, RkPtrs G, RkTableau T,
                                                    int policy) {
   constexpr int NV = 4 + NS;
@@ -376,7 +407,7 @@ __global__ void __launch_bounds__(256) kg_assemble(DevParams P, DevPtrs D, RkPtr
       for (int r = 0; r < T.nrk; r++) s2 = s2 + G.Im[r * sp + i * pl + c] * T.omega[r];
       q[i] = T.stiff ? G.qrk[(T.nrk - 1) * sp + i * pl + c] : D.q[i * pl + c] - dt * (s1 - s2);
     }
-    assembly_checks<NS>(P, D, c, q, qp, cnt[w], true);
+    assembly_checks<NS, VP>(P, D, c, q, qp, cnt[w], true);
     bool zero = true;
 #pragma unroll
     for (int i = 0; i < NV; i++) {

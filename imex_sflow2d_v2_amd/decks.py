@@ -428,6 +428,70 @@ def small_2d(n=48, ny=None, dx=10.0, seed=0, energy=True, alpha=False,
                 description="small 2D test deck")
 
 
+def profile_channel(nx=64, ny=48, dx=5.0, n_rk=2, n_solid=2, rheology_model=8,
+                    vertical_profiles=True, bottom_conc=True, n_quad=20,
+                    slope_correction=True, settling=True, seed=11, convert=None):
+    """A channel of moving flow for the vertical-profile closures (and
+    rheology model 8), which need every wet cell to move: the profile
+    momentum, speed and coefficients are NaN for a layer at rest, and so is
+    model 8's friction step, as in the reference.  A 3 degree incline along
+    x with seeded bumps; h ~ 1.5 m, u ~ 3 m/s, v ~ +-0.4 m/s everywhere;
+    Dirichlet inflow on the west (h = 2 m, u = 4 m/s), zero-gradient
+    outflow elsewhere; two classes (20 and 40 um, 2500 and 1800 kg/m3)
+    settling with the bottom-concentration coefficient; friction factor
+    0.01, vonK 0.4, k_s 0.05 m, Sc 0.9.  ``convert(params, qp) -> qc``
+    converts the initial physical state (qp_to_qc with the profiles, which
+    the host restatement qp_to_qc_np does not implement); by default the
+    closures must be off."""
+    rng = np.random.default_rng(seed)
+    rho = (2500.0, 1800.0, 2200.0)[:n_solid]
+    dia = (2e-5, 4e-5, 3e-5)[:n_solid]
+    shs = (1617.0, 1200.0, 1300.0)[:n_solid]
+    ab = (0.03, 0.01, 0.005)[:n_solid]
+    nv = 4 + n_solid
+    hb, ub, Tb = 2.0, 4.0, 400.0
+    bcw = [(0, hb), (0, hb * ub), (0, 0.0), (0, Tb)] + [(0, a * hb) for a in ab]
+    bc = {0: bcw}
+    for side in (1, 2, 3):
+        bc[side] = [(1, 0.0)] * nv
+    vp = None
+    if vertical_profiles:
+        vp = dict(vonK=0.4, k_s=0.05, Sc=0.9, n_quad=n_quad, bottom_conc=bottom_conc)
+    p = make_params(nx, ny, dx, n_solid, n_rk=n_rk, energy_flag=False, cfl=0.24, max_dt=1.0,
+                    rho_s=rho, diam_s=dia, sp_heat_s=shs, settling=settling,
+                    slope_correction=slope_correction, bc=bc,
+                    rheology=rheology_model is not None,
+                    rheology_model=rheology_model if rheology_model is not None else 1,
+                    friction_factor=0.01, vertical_profiles=vp)
+    xc, yc, _, _ = cell_centres(p)
+    X, Y = np.meshgrid(xc, yc, indexing="ij")
+    L = nx * dx
+    B = (L - X) * math.tan(math.radians(3.0)) + 20.0
+    for _ in range(6):
+        x0, y0 = rng.uniform(0, L), rng.uniform(0, ny * dx)
+        B = B + rng.uniform(0.2, 0.8) * np.exp(-((X - x0) ** 2 + (Y - y0) ** 2)
+                                               / (2.0 * (rng.uniform(3, 8) * dx) ** 2))
+    h = 1.5 + 0.3 * np.sin(X / (9 * dx)) * np.cos(Y / (7 * dx))
+    u = 3.0 + 0.5 * np.cos(Y / (6 * dx))
+    v = 0.4 * np.sin(X / (8 * dx))
+    T = 400.0 + 20.0 * rng.random(h.shape)
+    al = [a * (0.8 + 0.4 * rng.random(h.shape)) for a in (0.02, 0.01, 0.005)[:n_solid]]
+    qp = np.zeros((nv + 2,) + h.shape)
+    qp[0], qp[1], qp[2], qp[3] = h, h * u, h * v, T
+    for i, a in enumerate(al):
+        qp[4 + i] = a * h
+    qp[nv], qp[nv + 1] = u, v
+    if convert is None:
+        if vertical_profiles:
+            raise ValueError("the profile closures need a qp_to_qc with profiles (convert=)")
+        q = qp_to_qc_np(p, qp)
+    else:
+        q = convert(p, qp)
+    name = "profile_channel" if vertical_profiles else "friction8_channel"
+    return Deck(name, p, B, q, steps=30, dt0=1e-3,
+                description="%s %dx%d dx=%g N_RK=%d" % (name, nx, ny, dx, n_rk))
+
+
 CONFIGS = {
     "dambreak_1d": dambreak_1d,
     "incline_2d": incline_2d,

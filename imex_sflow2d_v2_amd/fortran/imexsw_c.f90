@@ -270,6 +270,55 @@ MODULE imexsw_c
        REAL(c_double), INTENT(OUT) :: q(*)
      END FUNCTION imexsw_q2d_read
 
+     !> qp_to_qc / qc_to_qp of n packed states with the context's model
+     !> (init_2d's conversion of the initial physical states)
+     INTEGER(c_int) FUNCTION imexsw_qp_to_qc(ctx, n, qp, qc)                     &
+          BIND(C, name='imexsw_qp_to_qc')
+       IMPORT :: c_int, c_ptr, c_int64_t, c_double
+       TYPE(c_ptr), VALUE :: ctx
+       INTEGER(c_int64_t), VALUE :: n
+       REAL(c_double), INTENT(IN) :: qp(*)
+       REAL(c_double), INTENT(OUT) :: qc(*)
+     END FUNCTION imexsw_qp_to_qc
+
+     INTEGER(c_int) FUNCTION imexsw_qc_to_qp(ctx, n, qc, qp)                     &
+          BIND(C, name='imexsw_qc_to_qp')
+       IMPORT :: c_int, c_ptr, c_int64_t, c_double
+       TYPE(c_ptr), VALUE :: ctx
+       INTEGER(c_int64_t), VALUE :: n
+       REAL(c_double), INTENT(IN) :: qc(*)
+       REAL(c_double), INTENT(OUT) :: qp(*)
+     END FUNCTION imexsw_qc_to_qp
+
+     !> eval_flux_coeffs of n physical states (read_param's MFR_SOURCE to
+     !> H_SOURCE with vertical profiles)
+     INTEGER(c_int) FUNCTION imexsw_flux_coeffs(ctx, n, qp, Bx, By, shape)      &
+          BIND(C, name='imexsw_flux_coeffs')
+       IMPORT :: c_int, c_ptr, c_int64_t, c_double
+       TYPE(c_ptr), VALUE :: ctx
+       INTEGER(c_int64_t), VALUE :: n
+       REAL(c_double), INTENT(IN) :: qp(*), Bx(*), By(*)
+       REAL(c_double), INTENT(OUT) :: shape(*)
+     END FUNCTION imexsw_flux_coeffs
+
+     !> lambertw0 (branch 0) / lambertwm1 (branch -1) of n arguments
+     INTEGER(c_int) FUNCTION imexsw_lambertw(branch, n, x, w)                   &
+          BIND(C, name='imexsw_lambertw')
+       IMPORT :: c_int, c_int32_t, c_int64_t, c_double
+       INTEGER(c_int32_t), VALUE :: branch
+       INTEGER(c_int64_t), VALUE :: n
+       REAL(c_double), INTENT(IN) :: x(*)
+       REAL(c_double), INTENT(OUT) :: w(*)
+     END FUNCTION imexsw_lambertw
+
+     !> gaulegf(-1, 1, x, w, n)
+     INTEGER(c_int) FUNCTION imexsw_gauss_legendre(n, x, w)                      &
+          BIND(C, name='imexsw_gauss_legendre')
+       IMPORT :: c_int, c_int32_t, c_double
+       INTEGER(c_int32_t), VALUE :: n
+       REAL(c_double), INTENT(OUT) :: x(*), w(*)
+     END FUNCTION imexsw_gauss_legendre
+
   END INTERFACE
 
 CONTAINS

@@ -95,6 +95,11 @@ ABI = {
     "imexsw_set_warm_start": (C.c_int, [C.c_void_p, C.c_void_p]),
     "imexsw_set_closed_form_start": (C.c_int, [C.c_void_p, C.c_int]),
     "imexsw_sync": (C.c_int, [C.c_void_p]),
+    "imexsw_qp_to_qc": (C.c_int, [C.c_void_p, C.c_int64, _D, _D]),
+    "imexsw_qc_to_qp": (C.c_int, [C.c_void_p, C.c_int64, _D, _D]),
+    "imexsw_flux_coeffs": (C.c_int, [C.c_void_p, C.c_int64, _D, _D, _D, _D]),
+    "imexsw_lambertw": (C.c_int, [C.c_int32, C.c_int64, _D, _D]),
+    "imexsw_gauss_legendre": (C.c_int, [C.c_int32, _D, _D]),
     "imexsw_q2d_write": (C.c_int, [C.c_char_p, C.c_int32, C.c_int32, C.c_int32, _D, _D, _D]),
     "imexsw_q2d_read": (C.c_int, [C.c_char_p, C.c_int32, C.c_int32, C.c_int32, C.c_int32, _D]),
 }
@@ -129,6 +134,25 @@ def _dp(a):
 
 def _f(a, dtype=np.float64):
     return np.asfortranarray(a, dtype=dtype)
+
+
+def lambertw(x, branch=0):
+    """Lambert W on branch 0 or -1 as the device evaluates it."""
+    lib = load_library()
+    x = np.ascontiguousarray(np.asarray(x, dtype=np.float64).ravel())
+    w = np.zeros_like(x)
+    if lib.imexsw_lambertw(int(branch), x.size, _dp(x), _dp(w)) != 0:
+        raise ImexswError(lib.imexsw_last_error(None).decode())
+    return w
+
+
+def gauss_legendre(n):
+    """Gauss-Legendre points and weights on [-1, 1] (the library's rule)."""
+    lib = load_library()
+    x, w = np.zeros(n), np.zeros(n)
+    if lib.imexsw_gauss_legendre(int(n), _dp(x), _dp(w)) != 0:
+        raise ImexswError(lib.imexsw_last_error(None).decode())
+    return x, w
 
 
 def q2d_write(path, q, x_comp, y_comp):
@@ -302,6 +326,38 @@ class Solver:
 
     def set_tuning(self, max_strip_rows=16, target_strips=2048):
         self._chk(self.lib.imexsw_set_tuning(self.h, int(max_strip_rows), int(target_strips)))
+
+    # ------------------------------------------------- point conversions
+    def qp_to_qc(self, qp):
+        """qp_to_qc of states qp(n_vars+2, ...) with the context's model
+        (vertical profiles included), on the device."""
+        qp = np.asarray(qp, dtype=np.float64)
+        nv = self.params.n_vars
+        flat = np.ascontiguousarray(qp.reshape(nv + 2, -1).T)
+        out = np.zeros((flat.shape[0], nv))
+        self._chk(self.lib.imexsw_qp_to_qc(self.h, flat.shape[0], _dp(flat), _dp(out)))
+        return out.T.reshape((nv,) + qp.shape[1:])
+
+    def qc_to_qp(self, qc):
+        """qc_to_qp of states qc(n_vars, ...) on the device."""
+        qc = np.asarray(qc, dtype=np.float64)
+        nv = self.params.n_vars
+        flat = np.ascontiguousarray(qc.reshape(nv, -1).T)
+        out = np.zeros((flat.shape[0], nv + 2))
+        self._chk(self.lib.imexsw_qc_to_qp(self.h, flat.shape[0], _dp(flat), _dp(out)))
+        return out.T.reshape((nv + 2,) + qc.shape[1:])
+
+    def flux_coeffs(self, qp, Bx=None, By=None):
+        """eval_flux_coeffs of states qp(n_vars+2, ...) (1 without profiles)."""
+        qp = np.asarray(qp, dtype=np.float64)
+        nv = self.params.n_vars
+        flat = np.ascontiguousarray(qp.reshape(nv + 2, -1).T)
+        n = flat.shape[0]
+        bx = np.zeros(n) if Bx is None else np.ascontiguousarray(np.asarray(Bx, float).ravel())
+        by = np.zeros(n) if By is None else np.ascontiguousarray(np.asarray(By, float).ravel())
+        out = np.zeros((n, nv))
+        self._chk(self.lib.imexsw_flux_coeffs(self.h, n, _dp(flat), _dp(bx), _dp(by), _dp(out)))
+        return out.T.reshape((nv,) + qp.shape[1:])
 
     def sync(self):
         self._chk(self.lib.imexsw_sync(self.h))

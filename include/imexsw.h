@@ -336,6 +336,37 @@ int imexsw_set_tuning(imexsw_ctx *ctx, int max_strip_rows, int target_strips);
 int imexsw_sync(imexsw_ctx *ctx);
 
 /* ---------------------------------------------------------------------------
+ * Point conversions with the context's model (vertical profiles included),
+ * evaluated on the device.  States are packed one after another:
+ * qp(n_vars+2) = h, hu, hv, T, (h*)alphas, u, v; qc(n_vars).  For the
+ * host's set-up work (initial states, source parameters), not per step.
+ * ------------------------------------------------------------------------- */
+
+/* qp_to_qc of n states (src/constitutive_2d.f90:1709-2140), as init_2d
+ * converts the initial physical states (src/init_2d.f90:53, 149-153). */
+int imexsw_qp_to_qc(imexsw_ctx *ctx, int64_t n, const double *qp, double *qc);
+
+/* qc_to_qp of n states (src/constitutive_2d.f90:1628-1681). */
+int imexsw_qc_to_qp(imexsw_ctx *ctx, int64_t n, const double *qc, double *qp);
+
+/* eval_flux_coeffs of n physical states with slopes Bx[n], By[n]
+ * (src/constitutive_2d.f90:2532-2823): shape(n_vars) per state, all 1
+ * without vertical profiles or for a dry state.  read_param derives the
+ * source thickness from MFR_SOURCE with shape(1) of the source state
+ * (src/inpout_2d.f90:4440-4480). */
+int imexsw_flux_coeffs(imexsw_ctx *ctx, int64_t n, const double *qp, const double *Bx,
+                       const double *By, double *shape);
+
+/* Lambert W of n arguments on branch 0 or -1 as the device evaluates it
+ * (lambertw0 / lambertwm1, src/geometry_2d.f90:1830-2038; -1 outside
+ * [-1/e, 0) on branch -1 and below -1/e on branch 0, as there). */
+int imexsw_lambertw(int32_t branch, int64_t n, const double *x, double *w);
+
+/* Gauss-Legendre points and weights on [-1, 1], ascending, n in
+ * 1..IMEXSW_MAX_QUAD (gaulegf, src/geometry_2d.f90:2407-2449); host only. */
+int imexsw_gauss_legendre(int32_t n, double *x, double *w);
+
+/* ---------------------------------------------------------------------------
  * .q_2d state files (host only; no context, no device work)
  * ------------------------------------------------------------------------- */
 

@@ -150,6 +150,36 @@ class Oracle:
         self.lib.orc_get_qp(self.h, _dp(a))
         return a
 
+    def qp_to_qc(self, qp):
+        """qp_to_qc of states qp(n_vars+2, ...), cell by cell."""
+        qp = np.asarray(qp, dtype=np.float64)
+        flat = np.ascontiguousarray(qp.reshape(self.nv + 2, -1).T)
+        out = np.zeros((flat.shape[0], self.nv))
+        for i in range(flat.shape[0]):
+            self.lib.orc_qp_to_qc(self.h, _dp(flat[i]), _dp(out[i]))
+        return out.T.reshape((self.nv,) + qp.shape[1:])
+
+    def qc_to_qp(self, qc):
+        """qc_to_qp of states qc(n_vars, ...), cell by cell."""
+        qc = np.asarray(qc, dtype=np.float64)
+        flat = np.ascontiguousarray(qc.reshape(self.nv, -1).T)
+        out = np.zeros((flat.shape[0], self.nv + 2))
+        pd = np.zeros(1)
+        for i in range(flat.shape[0]):
+            self.lib.orc_qc_to_qp(self.h, _dp(flat[i]), _dp(out[i]), _dp(pd))
+        return out.T.reshape((self.nv + 2,) + qc.shape[1:])
+
+    def flux_coeffs(self, qp, Bx, By):
+        """eval_flux_coeffs of states qp(n_vars+2, ...) with slopes Bx, By."""
+        qp = np.asarray(qp, dtype=np.float64)
+        flat = np.ascontiguousarray(qp.reshape(self.nv + 2, -1).T)
+        bx, by = np.ravel(Bx), np.ravel(By)
+        out = np.zeros((flat.shape[0], self.nv))
+        for i in range(flat.shape[0]):
+            self.lib.orc_flux_coeffs(self.h, _dp(flat[i]), float(bx[i]), float(by[i]),
+                                     _dp(out[i]))
+        return out.T.reshape((self.nv,) + qp.shape[1:])
+
     def set_q(self, q):
         q = _f(q)
         self.lib.orc_set_q(self.h, _dp(q))

@@ -14,7 +14,7 @@ PROGRAM imexsw_c_check
   TYPE(imexsw_stats), TARGET :: st
   TYPE(c_ptr) :: ctx
   INTEGER(c_int) :: rc
-  REAL(c_double) :: q(5, 3, 2), q2(5, 3, 2), x(3), y(2)
+  REAL(c_double) :: q(5, 3, 2), q2(5, 3, 2), x(3), y(2), gx(5), gw(5)
   INTEGER :: i, j, k
   CHARACTER(len=*), PARAMETER :: path = 'imexsw_c_check.q_2d'
 
@@ -94,6 +94,14 @@ PROGRAM imexsw_c_check
   WRITE(*, '(A,I0)') 'q2d_read rc ', rc
   q(:, 2, 2) = 0.0_c_double
   WRITE(*, '(A,ES12.4)') 'q2d max abs diff ', MAXVAL(ABS(q2 - q))
+
+  ! the quadrature rule of the vertical profiles (host only), printed to the
+  ! last bit
+  rc = imexsw_gauss_legendre(5, gx, gw)
+  WRITE(*, '(A,I0)') 'gauss_legendre rc ', rc
+  DO i = 1, 5
+     WRITE(*, '(A,I0,A,Z16.16,A,Z16.16)') 'gl ', i, ' ', gx(i), ' ', gw(i)
+  END DO
 
 CONTAINS
 

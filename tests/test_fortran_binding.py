@@ -84,3 +84,25 @@ def test_host_only_entry_points_from_fortran(check_output):
     assert "q2d_write rc 0" in lines and "q2d_read rc 0" in lines
     diff = [ln for ln in lines if ln.startswith("q2d max abs diff")][0]
     assert float(diff.split()[-1]) == 0.0
+
+
+def test_quadrature_rule_from_fortran_matches_oracle(check_output):
+    """imexsw_gauss_legendre called from Fortran returns the oracle's
+    Gauss-Legendre rule bit for bit (the rule the profile closures use)."""
+    import struct
+
+    import numpy as np
+    from oracle.pyoracle import load_oracle
+
+    lines = check_output.splitlines()
+    assert "gauss_legendre rc 0" in lines
+    got = [ln.split() for ln in lines if ln.startswith("gl ")]
+    assert len(got) == 5
+    L = load_oracle()
+    x, w = np.zeros(5), np.zeros(5)
+    D = L.orc_gauss_legendre.argtypes[1]
+    L.orc_gauss_legendre(5, x.ctypes.data_as(D), w.ctypes.data_as(D))
+    for i, (_, _, hx, hw) in enumerate(got):
+        assert struct.unpack(">d", bytes.fromhex(hx))[0] == x[i]
+        assert struct.unpack(">d", bytes.fromhex(hw))[0] == w[i]
+
