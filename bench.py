@@ -10,7 +10,8 @@ cells, dx = 5 m, cone + 32 bumps + fractal noise, radial source 1e6 kg/s,
 three solid classes, erosion/deposition with topography change, a run of
 1000 timesteps from t = 0.  ``--workload wet`` runs the fully wet 2048^2
 incline variant (every cell active) that exposes the stencil and Newton
-kernels.
+kernels; ``--workload profiles`` a 2048^2 channel of moving flow with the
+vertical-profile closures and rheology model 8 (the tableau-driven path).
 
 Timing.  W warm-up steps run on the context that is timed (code objects,
 caches, the captured step graphs); the deck is then loaded again and the run
@@ -112,6 +113,18 @@ def make_deck(workload):
         return decks.volcano_2048()
     if workload == "wet":
         return decks.wet_2048()
+    if workload == "profiles":
+        # the vertical-profile closures on a 2048^2 channel of moving flow;
+        # the initial state converted by the library's own qp_to_qc
+        from imex_sflow2d_v2_amd.solver import Solver
+        holder = {}
+
+        def convert(p, qp):
+            s = holder.setdefault("s", Solver(p))
+            return s.qp_to_qc(qp)
+        d = decks.profile_channel(nx=2048, ny=2048, dx=5.0, convert=convert)
+        holder["s"].close()
+        return d
     raise SystemExit("unknown workload %r" % workload)
 
 
@@ -443,7 +456,7 @@ def main():
                     help="steps of the fully wet 2048^2 grid run for its roofline")
     ap.add_argument("--no-wet", action="store_true")
     ap.add_argument("--workload", default="volcano",
-                    choices=["volcano", "wet", "ensemble", "warmstart"])
+                    choices=["volcano", "wet", "profiles", "ensemble", "warmstart"])
     ap.add_argument("--ens-steps", type=int, default=500,
                     help="ensemble workload: steps per member")
     ap.add_argument("--harvest-incline-steps", type=int, default=150,
@@ -609,7 +622,10 @@ def main():
     }
     if not args.no_cpu:
         threads = args.cpu_threads or min(16, len(os.sched_getaffinity(0)))
-        out["cpu_baseline"] = cpu_baseline(args.workload, max(args.cpu_steps, K), threads,
+        # the profile closures make the oracle's cell-updates about 30x
+        # dearer than the plain path's: its sample is --cpu-steps steps
+        csteps = args.cpu_steps if args.workload == "profiles" else max(args.cpu_steps, K)
+        out["cpu_baseline"] = cpu_baseline(args.workload, csteps, threads,
                                            start=start_state if develop else None)
     print(json.dumps(out), flush=True)
     return 0

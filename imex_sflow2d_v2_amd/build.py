@@ -22,7 +22,7 @@ NS_VALUES = range(0, 7)
 SOURCES = ["imexsw.hip", "imexsw_ns.hip"]
 HEADERS = ["imexsw_math.h", "imexsw_physics.h", "imexsw_newton.h", "imexsw_kernels.h",
            "imexsw_batch.h", "imexsw_mlp.h", "imexsw_q2d.h", "imexsw_ctx.h",
-           "imexsw_stages.h"]
+           "imexsw_stages.h", "imexsw_profiles.h"]
 ARCH = os.environ.get("IMEXSW_ARCH", "gfx950")
 
 HIPCC_FLAGS = [
