@@ -509,11 +509,14 @@ int imexsw_create(const imexsw_params* params, imexsw_ctx** out) {
   if (p.nx < 2 || p.ny < 1) return fail(nullptr, "grid must have nx >= 2 and ny >= 1");
   if (p.n_solid < 0 || p.n_solid > IMEXSW_MAX_SOLID) return fail(nullptr, "n_solid out of range");
   if (p.n_rk < 1 || p.n_rk > kMaxRK) return fail(nullptr, "N_RK must be 1, 2, 3 or 4 (got %d)", p.n_rk);
-  if (p.rheology_flag && p.rheology_model != 1 && p.rheology_model != 8)
-    return fail(nullptr, "rheology model %d is not supported (1, Voellmy-Salm, or 8)",
+  if (p.rheology_flag && p.rheology_model != 1 && p.rheology_model != 6 && p.rheology_model != 8)
+    return fail(nullptr, "rheology model %d is not supported (1, Voellmy-Salm; 6; 8)",
                 p.rheology_model);
-  if (p.rheology_flag && p.rheology_model == 8 && !(p.friction_factor >= 0.0))
-    return fail(nullptr, "rheology model 8 needs FRICTION_FACTOR >= 0 (got %g)", p.friction_factor);
+  // src/inpout_2d.f90:3350-3365
+  if (p.rheology_flag && (p.rheology_model == 6 || p.rheology_model == 8) &&
+      !(p.friction_factor >= 0.0))
+    return fail(nullptr, "rheology model %d needs FRICTION_FACTOR >= 0 (got %g)", p.rheology_model,
+                p.friction_factor);
   if (p.vertical_profiles_flag) {
     // src/inpout_2d.f90:3699-3775
     if (p.rheology_flag && p.rheology_model != 8)
@@ -687,11 +690,11 @@ int imexsw_create(const imexsw_params* params, imexsw_ctx** out) {
   c->grid_mask = dim3((P.nx + kStripW - 1) / kStripW, (P.ny + kTileH - 1) / kTileH);
   c->grid_stag = dim3((P.nx + 1 + kTileX - 1) / kTileX, (P.ny + 1 + kTileY - 1) / kTileY);
   c->n_tiles = (int)(c->grid_mask.x * c->grid_mask.y);
-  if (p.n_rk != 2 || P.vp || (P.rheo && P.rheo_model == 8)) {
+  if (p.n_rk != 2 || P.vp || (P.rheo && P.rheo_model != 1)) {
     // the tableau-driven stage pipeline (N_RK != 2, and every N_RK with
-    // the profile closures or rheology model 8): stage states, residuals
-    // and implicit terms of every stage, two byte planes for the mask
-    // dilation
+    // the profile closures or rheology models 6 and 8): stage states,
+    // residuals and implicit terms of every stage, two byte planes for the
+    // mask dilation
     c->general = true;
     fill_tableau(p.n_rk, c->tab);
     const size_t sp = (size_t)c->nv * pl * p.n_rk;
@@ -1539,9 +1542,9 @@ int imexsw_ensemble_create(int32_t n_members, const imexsw_params* params,
     if (p.n_solid != ns) return efail(nullptr, "member %d: all members need the same n_solid", m);
     if (p.ny < 2) return efail(nullptr, "member %d: ensembles need 2-D grids", m);
     if (p.n_rk != 2) return efail(nullptr, "member %d: ensembles run N_RK = 2", m);
-    if (p.vertical_profiles_flag || (p.rheology_flag && p.rheology_model == 8))
+    if (p.vertical_profiles_flag || (p.rheology_flag && p.rheology_model != 1))
       return efail(nullptr, "member %d: ensembles run without vertical profiles and with "
-                            "rheology model 1", m);
+                            "rheology model 1 or none", m);
     if (p.topo_change_flag || p.radial_source_flag || p.slope_correction_flag || p.curvature_term_flag)
       return efail(nullptr, "member %d: ensembles run the flat-source-free model (no topography "
                             "change, radial source, slope correction or curvature)", m);

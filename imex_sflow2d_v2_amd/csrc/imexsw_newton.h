@@ -26,7 +26,7 @@ __device__ __forceinline__ void finish_f(const double* qj, const double* qold,
   for (int i = 0; i < NV; i++) s += f[i] * f[i];
   scal = 0.5 * s;
 }
-template <int NS>
+template <int NS, int RM = 1>
 __device__ __forceinline__ void eval_f(const DevParams& P, const double* qj,
                                        const double* qold, const double* Rj,
                                        double a_diag, double dt, double Bx,
@@ -36,7 +36,7 @@ __device__ __forceinline__ void eval_f(const DevParams& P, const double* qj,
 #pragma unroll
   for (int i = 0; i < NV; i++) cq[i] = mk(qj[i], 0.0);
   cpx n1, n2;
-  implicit_c<NS>(P, cq, Bx, By, n1, n2);
+  implicit_c<NS, RM>(P, cq, Bx, By, n1, n2);
   finish_f<NS>(qj, qold, Rj, a_diag, dt, n1.re, n2.re, f, scal);
 }
 
@@ -67,12 +67,13 @@ struct NewtonOut {
   int ls_evals;  // residual evaluations inside the line search
 };
 
-// solve_rk_step with eval_jacobian and lnsrch (src/solver_2d.f90:1660-2365).
+// solve_rk_step with eval_jacobian and lnsrch (src/solver_2d.f90:1660-2365);
+// RM = 6: the implicit terms of rheology model 6.
 // The implicit unknowns are the momenta (implicit_flag = F,T,T,F,...,
 // src/constitutive_2d.f90:478-480): the Jacobian has the 2x2 momentum block
 // plus the diagonal scaling of the other unknowns, which is what the
 // reference's packed partitioned solve reduces to.
-template <int NS>
+template <int NS, int RM = 1>
 __device__ __forceinline__ NewtonOut solve_rk_step(const DevParams& P, double* qj,
                                                   const double* qold,
                                                   const double* Rj, double a_diag,
@@ -103,7 +104,7 @@ __device__ __forceinline__ NewtonOut solve_rk_step(const DevParams& P, double* q
     fresh = true;
   }
   for (it = 1; it <= 100; it++) {
-    if (!fresh) eval_f<NS>(P, qj, qold, Rj, a_diag, dt, Bx, By, rt, scal);
+    if (!fresh) eval_f<NS, RM>(P, qj, qold, Rj, a_diag, dt, Bx, By, rt, scal);
     bool restored = false;
     double mf = 0.0;
 #pragma unroll
@@ -123,7 +124,7 @@ __device__ __forceinline__ NewtonOut solve_rk_step(const DevParams& P, double* q
 #pragma unroll
       for (int m = 0; m < NV; m++) cq[m] = cmulr(cq[m], org[m]);
       cpx n1, n2;
-      implicit_c<NS>(P, cq, Bx, By, n1, n2);
+      implicit_c<NS, RM>(P, cq, Bx, By, n1, n2);
       J1[1] = 1.0 * n1.im * P.one_by_h;
       J1[2] = 1.0 * n2.im * P.one_by_h;
     }
@@ -135,7 +136,7 @@ __device__ __forceinline__ NewtonOut solve_rk_step(const DevParams& P, double* q
 #pragma unroll
       for (int m = 0; m < NV; m++) cq[m] = cmulr(cq[m], org[m]);
       cpx n1, n2;
-      implicit_c<NS>(P, cq, Bx, By, n1, n2);
+      implicit_c<NS, RM>(P, cq, Bx, By, n1, n2);
       J2[1] = 1.0 * n1.im * P.one_by_h;
       J2[2] = 1.0 * n2.im * P.one_by_h;
     }
@@ -220,7 +221,7 @@ __device__ __forceinline__ NewtonOut solve_rk_step(const DevParams& P, double* q
           for (int i = 0; i < NV; i++) rel[i] = old[i] + alam * desc[i];
 #pragma unroll
           for (int i = 0; i < NV; i++) q[i] = rel[i] * org[i];
-          eval_f<NS>(P, q, qold, Rj, a_diag, dt, Bx, By, rt, scal);
+          eval_f<NS, RM>(P, q, qold, Rj, a_diag, dt, Bx, By, rt, scal);
           out.ls_evals++;
           if (scal <= 0.9 * scal_old) break;
           if (alam < alamin) {
@@ -258,7 +259,7 @@ __device__ __forceinline__ NewtonOut solve_rk_step(const DevParams& P, double* q
       for (int i = 0; i < NV; i++) rel[i] = old[i] + desc[i];
 #pragma unroll
       for (int i = 0; i < NV; i++) qj[i] = rel[i] * org[i];
-      eval_f<NS>(P, qj, qold, Rj, a_diag, dt, Bx, By, rt, scal);
+      eval_f<NS, RM>(P, qj, qold, Rj, a_diag, dt, Bx, By, rt, scal);
     }
 #pragma unroll
     for (int i = 0; i < NV; i++) qj[i] = rel[i] * org[i];

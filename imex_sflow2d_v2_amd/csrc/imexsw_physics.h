@@ -330,10 +330,10 @@ __device__ __forceinline__ Mixt mixt(const DevParams& P, const double* qp) {
 }
 
 // qp_to_qc (src/constitutive_2d.f90:1709-2140, no liquid; VP: the profile
-// momentum)
+// momentum, with the state's h0 and settling speeds from pre when given)
 template <int NS, bool VP = false>
 __device__ __forceinline__ void qp2qc(const DevParams& P, const double* qp,
-                                      double* qc) {
+                                      double* qc, const VpPre<NS>* pre = nullptr) {
   constexpr int NV = 4 + NS;
   double r_h = qp[0];
   if (!(r_h > kEps)) {
@@ -372,7 +372,8 @@ __device__ __forceinline__ void qp2qc(const DevParams& P, const double* qp,
     double ra[NS > 0 ? NS : 1];
 #pragma unroll
     for (int i = 0; i < NS; i++) ra[i] = (at > 1.0) ? al[i] / at : (at < 0.0) ? 0.0 : al[i];
-    vp_momentum<NS>(P, r_h, r_u, r_v, rho_c, ra, qc[1], qc[2]);
+    const VpPre<NS> pr = pre ? *pre : vp_pre<NS>(P, r_h, rho_c);
+    vp_momentum<NS>(P, r_h, r_u, r_v, pr, rho_c, ra, qc[1], qc[2]);
   } else {
     qc[1] = rho_m * r_hu;
     qc[2] = rho_m * r_hv;
@@ -411,7 +412,8 @@ __device__ __forceinline__ void speeds(const DevParams& P, const double* qp,
 template <int NS, int DIR, bool VP = false>
 __device__ __forceinline__ void fluxes(const DevParams& P, const double* qc,
                                        const double* qp, double gc, double* f,
-                                       double Bx = 0.0, double By = 0.0) {
+                                       double Bx = 0.0, double By = 0.0,
+                                       const VpPre<NS>* pre = nullptr) {
   constexpr int NV = 4 + NS;
   if (!(qp[0] > kEps)) {
 #pragma unroll
@@ -423,7 +425,10 @@ __device__ __forceinline__ void fluxes(const DevParams& P, const double* qc,
   double sc[NV];
 #pragma unroll
   for (int i = 0; i < NV; i++) sc[i] = 1.0;
-  if (VP) vp_flux_coeffs<NS>(P, qp, Bx, By, m.rho_c, m.rho_m, sc);
+  if (VP) {
+    const VpPre<NS> pr = pre ? *pre : vp_pre<NS>(P, r_h, m.rho_c);
+    vp_flux_coeffs<NS>(P, qp, Bx, By, pr, m.rho_c, m.rho_m, sc);
+  }
   double vel = (DIR == 1) ? r_u : r_v;
   double pt = 0.5 * m.rho_m * gc * m.rg * (r_h * r_h);
   f[0] = vel * qc[0] * sc[0];
@@ -535,10 +540,11 @@ __device__ __forceinline__ void semi_impl(const DevParams& P, double Bx, double 
   }
 }
 
-// Voellmy-Salm turbulent drag, implicit, complex arguments
-// (eval_implicit_terms + c_phys_var, src/constitutive_2d.f90:3327-3823,
-// 1145-1313).  Only the momentum components are non-zero.
-template <int NS>
+// Turbulent drag, implicit, complex arguments (eval_implicit_terms +
+// c_phys_var, src/constitutive_2d.f90:3327-3823, 1145-1313): Voellmy-Salm
+// (rheology model 1), or with RM = 6 the quadratic drag rho_m f |u|^2 of
+// model 6 (:3639-3650).  Only the momentum components are non-zero.
+template <int NS, int RM = 1>
 __device__ __forceinline__ void implicit_c(const DevParams& P, const cpx* cq,
                                            double Bx, double By, cpx& n1,
                                            cpx& n2) {
@@ -597,7 +603,13 @@ __device__ __forceinline__ void implicit_c(const DevParams& P, const cpx* cq,
   cpx w = P.slope ? cadd(cmulr(u, Bx), cmulr(v, By)) : mk(0.0, 0.0);
   cpx mv2 = cadd(cadd(cmul(u, u), cmul(v, v)), cmul(w, w));
   cpx mv = csqrt_(mv2);
-  if (P.rheo_model == 1 && mv.re != 0.0) {
+  if (RM == 6) {
+    const cpx mvh = csqrt_(cadd(cmul(u, u), cmul(v, v)));
+    if (mv.re != 0.0) {
+      n1 = csub(n1, cmul(cmulr(cmul(rho_m, cdiv(u, mvh)), P.ff), mv2));
+      n2 = csub(n2, cmul(cmulr(cmul(rho_m, cdiv(v, mvh)), P.ff), mv2));
+    }
+  } else if (P.rheo_model == 1 && mv.re != 0.0) {
     cpx turb = cmul(cdivr(cmul(rho_m, rg), P.xi), mv2);
     n1 = csub(n1, cmul(turb, cdiv(u, mv)));
     n2 = csub(n2, cmul(turb, cdiv(v, mv)));

@@ -119,17 +119,31 @@ __device__ __forceinline__ void vp_point(const DevParams& P, int m, double h0, d
   lz = det_log(b * z + 1.0);
 }
 
-// Rouse numbers w_s / (vonK u*) (zero without shear); the settling speed in a
-// carrier of density rho_c
+// What the closures of one state share whatever its velocity: the
+// boundary-layer top (from h) and the settling speeds (from the carrier
+// density), so that a face side's momentum and flux coefficients form
+// them once
 template <int NS>
-__device__ __forceinline__ void vp_rouse(const DevParams& P, double shear_vel, double rho_c,
-                                         double* rouse) {
+struct VpPre {
+  double h0;
+  double sv[NS > 0 ? NS : 1];
+};
+template <int NS>
+__device__ __forceinline__ VpPre<NS> vp_pre(const DevParams& P, double r_h, double rho_c) {
+  VpPre<NS> r;
+  r.h0 = vp_h0(P, r_h);
   const double ikv = 1.0 / P.kin_visc_c;
 #pragma unroll
-  for (int i = 0; i < NS; i++) {
-    const double sv = settling_velocity(P, P.diam_s[i], P.rho_s[i], rho_c, ikv);
-    rouse[i] = (shear_vel > 0.0) ? sv / (P.vonK * shear_vel) : 0.0;
-  }
+  for (int i = 0; i < NS; i++) r.sv[i] = settling_velocity(P, P.diam_s[i], P.rho_s[i], rho_c, ikv);
+  return r;
+}
+
+// Rouse numbers w_s / (vonK u*) (zero without shear)
+template <int NS>
+__device__ __forceinline__ void vp_rouse(const DevParams& P, double shear_vel, const double* sv,
+                                         double* rouse) {
+#pragma unroll
+  for (int i = 0; i < NS; i++) rouse[i] = (shear_vel > 0.0) ? sv[i] / (P.vonK * shear_vel) : 0.0;
 }
 
 // The sums every closure starts from, for the solid exponents a_i:
@@ -240,16 +254,17 @@ __device__ __forceinline__ double vp_velocity(const DevParams& P, double r_h, do
   return u_new;
 }
 
-// qp_to_qc's profile momentum (src/constitutive_2d.f90:1976-2107)
+// qp_to_qc's profile momentum (src/constitutive_2d.f90:1976-2107); pre: the
+// state's h0 and settling speeds
 template <int NS>
 __device__ __forceinline__ void vp_momentum(const DevParams& P, double r_h, double r_u,
-                                            double r_v, double r_rho_c, const double* r_alphas,
-                                            double& qc2, double& qc3) {
+                                            double r_v, const VpPre<NS>& pre, double r_rho_c,
+                                            const double* r_alphas, double& qc2, double& qc3) {
   const double mod_vel = sqrt(r_u * r_u + r_v * r_v + 0.0 * 0.0);
   const double shear_vel = P.sqrt_ff * mod_vel;
   double rouse[NS > 0 ? NS : 1];
-  vp_rouse<NS>(P, shear_vel, r_rho_c, rouse);
-  const double h0 = vp_h0(P, r_h);
+  vp_rouse<NS>(P, shear_vel, pre.sv, rouse);
+  const double h0 = pre.h0;
   const double b = 30.0 / P.k_s;
   const double eps_s = P.Sc * shear_vel * P.vonK * ((h0 / 6.0) + (P.k_s / 60.0));
   const double a_coeff = -(P.vonK * shear_vel / eps_s);
@@ -276,11 +291,12 @@ __device__ __forceinline__ void vp_momentum(const DevParams& P, double r_h, doub
   qc3 = r_h * uRho_avg * r_v / mod_vel;
 }
 
-// eval_flux_coeffs (src/constitutive_2d.f90:2532-2823); sc[0..NV-1]
+// eval_flux_coeffs (src/constitutive_2d.f90:2532-2823); sc[0..NV-1]; pre:
+// the state's h0 and settling speeds
 template <int NS>
 __device__ __forceinline__ void vp_flux_coeffs(const DevParams& P, const double* qp, double Bx,
-                                               double By, double r_rho_c, double r_rho_m,
-                                               double* sc) {
+                                               double By, const VpPre<NS>& pre, double r_rho_c,
+                                               double r_rho_m, double* sc) {
   constexpr int NV = 4 + NS;
   const double r_h = qp[0], r_hu = qp[1], r_hv = qp[2], r_u = qp[NV], r_v = qp[NV + 1];
   double al[NS > 0 ? NS : 1];
@@ -292,8 +308,8 @@ __device__ __forceinline__ void vp_flux_coeffs(const DevParams& P, const double*
   const double mod_vel = sqrt(r_u * r_u + r_v * r_v + r_w * r_w);
   const double shear_vel = P.sqrt_ff * mod_vel;
   double rouse[NS > 0 ? NS : 1];
-  vp_rouse<NS>(P, shear_vel, r_rho_c, rouse);
-  const double h0 = vp_h0(P, r_h);
+  vp_rouse<NS>(P, shear_vel, pre.sv, rouse);
+  const double h0 = pre.h0;
   const double b = 30.0 / P.k_s;
   const double eps_s = P.Sc * shear_vel * P.vonK * ((h0 / 6.0) + (P.k_s / 60.0));
   const double a_coeff = -(P.vonK * shear_vel / eps_s);
@@ -338,9 +354,10 @@ template <int NS>
 __device__ __forceinline__ void vp_dep_coeffs(const DevParams& P, double r_h, double mod_vel,
                                               double r_rho_c, double* dep) {
   const double shear_vel = P.sqrt_ff * mod_vel;
+  const VpPre<NS> pre = vp_pre<NS>(P, r_h, r_rho_c);
   double rouse[NS > 0 ? NS : 1];
-  vp_rouse<NS>(P, shear_vel, r_rho_c, rouse);
-  const double h0 = vp_h0(P, r_h);
+  vp_rouse<NS>(P, shear_vel, pre.sv, rouse);
+  const double h0 = pre.h0;
   const double eps_s = P.Sc * shear_vel * P.vonK * ((h0 / 6.0) + (P.k_s / 60.0));
   const double a_coeff = -(P.vonK * shear_vel / eps_s);
   double a[NS > 0 ? NS : 1], se[NS > 0 ? NS : 1];

@@ -190,12 +190,16 @@ __device__ __forceinline__ int stage_cell(const DevParams& P, const double* q0,
     q_guess[i] = q0[i];
     si_nh[i] = 0.0;
   }
-  phys<NS, VP>(P, q_fv, qp);
+  // qp of q_fv (:1138) feeds only the semi-implicit Coulomb terms, which
+  // rheology model 1 alone has: it is formed only for them
   int iters = 0;
   if (a_diag != 0.0) {
     if (q_fv[0] > 0.0) {
-      double s1, s2;
-      semi_impl<NS>(P, Bx, By, Bxx, Bxy, Byy, gc, qp, s1, s2);
+      double s1 = 0.0, s2 = 0.0;
+      if (P.rheo && P.rheo_model == 1) {
+        phys<NS, VP>(P, q_fv, qp);
+        semi_impl<NS>(P, Bx, By, Bxx, Bxy, Byy, gc, qp, s1, s2);
+      }
 #pragma unroll
       for (int i = 0; i < NV; i++) {
         const double si = (i == 1) ? s1 : (i == 2) ? s2 : 0.0;
@@ -226,7 +230,9 @@ __device__ __forceinline__ int stage_cell(const DevParams& P, const double* q0,
       if (fric8) {
         integrate_friction<NS, VP>(P, q_guess, dt);
       } else {
-        const NewtonOut no = solve_rk_step<NS>(P, q_guess, q0, Rj, a_diag, dt, Bx, By);
+        const NewtonOut no = (P.rheo && P.rheo_model == 6)
+                                 ? solve_rk_step<NS, 6>(P, q_guess, q0, Rj, a_diag, dt, Bx, By)
+                                 : solve_rk_step<NS>(P, q_guess, q0, Rj, a_diag, dt, Bx, By);
         iters = no.iters;
         nonconv = !no.conv;
         ls_evals = no.ls_evals;

@@ -385,7 +385,7 @@ def small_2d(n=48, ny=None, dx=10.0, seed=0, energy=True, alpha=False,
              settling=True, erosion=False, source=False, topo_change=False,
              slope_correction=False, curvature=False, n_solid=2, rheology=True,
              bc=None, limiter=1, n_rk=2, wet_floor=0.0, entrainment=True,
-             source_time=(1e9, 1e9, 0.0, 1e9)):
+             source_time=(1e9, 1e9, 0.0, 1e9), rheology_model=1, friction_factor=-1.0):
     """Small seeded 2D decks for the parity tests (seconds on the oracle).
     ``source_time`` is the radial source's time_param (period, duration,
     ramp time, end time; src/constitutive_2d.f90:5041-5088)."""
@@ -407,6 +407,7 @@ def small_2d(n=48, ny=None, dx=10.0, seed=0, energy=True, alpha=False,
                     topo_change=topo_change, radial_source=source,
                     slope_correction=slope_correction, curvature=curvature,
                     rheology=rheology, bc=bc, limiter=limiter, n_rk=n_rk,
+                    rheology_model=rheology_model, friction_factor=friction_factor,
                     source=src, entrainment=entrainment,
                     thickness_levels=(0.1, 1.0), dyn_pres_levels=(10.0, 1000.0))
     xc, yc, _, _ = cell_centres(p)
@@ -490,6 +491,34 @@ def profile_channel(nx=64, ny=48, dx=5.0, n_rk=2, n_solid=2, rheology_model=8,
     name = "profile_channel" if vertical_profiles else "friction8_channel"
     return Deck(name, p, B, q, steps=30, dt0=1e-3,
                 description="%s %dx%d dx=%g N_RK=%d" % (name, nx, ny, dx, n_rk))
+
+
+def reference_box(q_path):
+    """The reference's own TESTS/TEST_BOX run (its IMEX_SfloW2D.inp): a
+    400 x 1 grid of 125 m cells on a flat bed, one solid class (2000 kg/m3,
+    0.1 mm) with settling and air entrainment, the energy equation,
+    rheology model 6 with FRICTION_FACTOR 1e-3, N_RK = 2, the KT scheme with
+    limiter 1 on every variable, THETA 1.3, CFL 0.24 as the single-precision
+    literal 0.23999999463558197, MAX_DT 0.2, DT0 1e-4, T_END 200 s; a wall
+    (hu = 0) on the west and zero-gradient outflow on the east.  The state
+    is its restart file INIT_BOX_0000.q_2d (``q_path``), read as
+    read_solution reads it."""
+    from .solver import q2d_read
+    nx, ny, ns = 400, 1, 1
+    nv = 4 + ns
+    # (flag, value) per variable: h, hu, hv, T, h alpha
+    bcw = [(1, 0.0), (0, 0.0), (0, 0.0), (1, 0.0), (1, 0.0)]
+    bce = [(1, 0.0), (1, 0.0), (0, 0.0), (1, 0.0), (1, 0.0)]
+    p = make_params(nx, ny, 125.0, ns, n_rk=2, energy_flag=True, cfl=0.23999999463558197,
+                    max_dt=0.2, theta=1.3, reconstr_coeff=1.0, limiter=1,
+                    bc={0: bcw, 1: bce}, grav=9.81, pres=101300.0, T_ambient=300.0,
+                    sp_heat_a=998.0, sp_gas_const_a=287.051, kin_visc_a=1.48e-5,
+                    rho_s=(2000.0,), diam_s=(1e-4,), sp_heat_s=(1617.0,), settling=True,
+                    entrainment=True, erosion_coeff=0.0, rheology=True, rheology_model=6,
+                    friction_factor=1e-3)
+    q = q2d_read(q_path, nv, ns, nx, ny)
+    return Deck("reference_box", p, np.zeros((nx, ny)), q, t_end=200.0, dt0=1e-4,
+                description="TESTS/TEST_BOX 400x1 dx=125 model 6")
 
 
 CONFIGS = {

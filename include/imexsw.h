@@ -54,9 +54,9 @@ enum { IMEXSW_SRC_E = 1, IMEXSW_SRC_W = 2, IMEXSW_SRC_S = 4, IMEXSW_SRC_N = 8 };
  *
  * Supported model subset: gas carrier (GAS_FLAG=T), no liquid, no additional
  * gas components, no stochastic or pore-pressure variables, no Sutherland
- * viscosity, rheology model 1 (Voellmy-Salm), model 8 (turbulent friction
- * integrated in closed form) or RHEOLOGY_FLAG=F, with or without vertical
- * profiles, solver scheme "KT".  n_vars = 4 + n_solid (src/inpout_2d.f90:1429,
+ * viscosity, rheology model 1 (Voellmy-Salm), 6 (turbulent drag), 8
+ * (turbulent friction integrated in closed form) or RHEOLOGY_FLAG=F, with or
+ * without vertical profiles, solver scheme "KT".  n_vars = 4 + n_solid (src/inpout_2d.f90:1429,
  * 1852).
  */
 typedef struct imexsw_params {
@@ -76,7 +76,9 @@ typedef struct imexsw_params {
   int32_t topo_change_flag;
   int32_t radial_source_flag;
   int32_t rheology_flag;
-  int32_t rheology_model;   /* 1 = Voellmy-Salm */
+  int32_t rheology_model;   /* 1 = Voellmy-Salm, 6 = turbulent drag
+                               rho f |u|^2, 8 = the same drag integrated in
+                               closed form */
   int32_t settling_flag;
   int32_t entrainment_flag;
   int32_t erodible_deposit_flag;
@@ -141,7 +143,7 @@ typedef struct imexsw_params {
   double z_dyn;               /* read but without effect on the state: the
                                  profile dynamic pressure is overwritten by
                                  r_phys_var (src/constitutive_2d.f90:957) */
-  /* RHEOLOGY_PARAMETERS friction_factor: rheology model 8 and the shear
+  /* RHEOLOGY_PARAMETERS friction_factor: rheology models 6 and 8, and the shear
    * velocity of the profiles (src/inpout_2d.f90:357, 3350-3365) */
   double friction_factor;
 } imexsw_params;

@@ -16,9 +16,9 @@
  *
  * Supported subset: gas carrier, no liquid / additional gas / stochastic /
  * pore pressure / bottom or lateral sources / Sutherland viscosity,
- * rheology model 1 (Voellmy-Salm), model 8 (closed-form turbulent
- * friction) or no rheology, with or without vertical profiles, solver
- * scheme KT.
+ * rheology model 1 (Voellmy-Salm), 6 (turbulent drag), 8 (closed-form
+ * turbulent friction) or no rheology, with or without vertical profiles,
+ * solver scheme KT.
  */
 #include "imexsw_oracle.h"
 
@@ -780,8 +780,9 @@ static void c_phys_var(const orc_t *o, const cx *cq, cx *rho_m, cx *u, cx *v) {
   }
 }
 
-/* eval_implicit_terms, Voellmy branch (src/constitutive_2d.f90:3327-3823,
- * model 1 at :3474-3497); complex in, complex out.  The real-argument call
+/* eval_implicit_terms, Voellmy (model 1, :3474-3497) and turbulent-drag
+ * (model 6, :3639-3650) branches (src/constitutive_2d.f90:3327-3823);
+ * complex in, complex out.  The real-argument call
  * of the reference goes through the same complex arithmetic with zero
  * imaginary parts (:3431-3441). */
 static void implicit_terms_cs(const orc_t *o, const cx *cq, double Bx,
@@ -803,6 +804,18 @@ static void implicit_terms_cs(const orc_t *o, const cx *cq, double Bx,
       cx turb = cmul(cdivr(cmul(rho_m, red_grav), P->xi), mod_vel2);
       nh[1] = csub(nh[1], cmul(turb, cdiv(u, mod_vel)));
       nh[2] = csub(nh[2], cmul(turb, cdiv(v, mod_vel)));
+    }
+  } else if (P->rheology_model == 6) {
+    /* turbulent drag rho_m f |u|^2 along the horizontal velocity
+     * (:3639-3650); the real factor multiplies both components */
+    cx mod_vel_hor = csqrt_f(cadd(cmul(u, u), cmul(v, v)));
+    if (mod_vel.re != 0.0) {
+      nh[1] = csub(nh[1], cmul(cmulr(cmul(rho_m, cdiv(u, mod_vel_hor)),
+                                     P->friction_factor),
+                               mod_vel2));
+      nh[2] = csub(nh[2], cmul(cmulr(cmul(rho_m, cdiv(v, mod_vel_hor)),
+                                     P->friction_factor),
+                               mod_vel2));
     }
   }
 }

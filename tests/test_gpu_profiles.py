@@ -1,6 +1,7 @@
-"""Device versus oracle for the vertical-profile closures and rheology model 8
-(src/constitutive_2d.f90:797-1123, 1976-2107, 2532-2986, 3252-3306,
-src/geometry_2d.f90:1776-2038, 2407-2449).
+"""Device versus oracle for the vertical-profile closures and rheology models
+6 and 8 (src/constitutive_2d.f90:797-1123, 1976-2107, 2532-2986, 3252-3306,
+3639-3650, src/geometry_2d.f90:1776-2038, 2407-2449), including the
+reference's own TESTS/TEST_BOX run.
 
 The device evaluates the closures with the same operation sequence as the
 oracle (deterministic exp/log, the same Lambert W iteration, the same
@@ -124,6 +125,45 @@ def test_profile_channel_matches_oracle(kw):
         dev.close()
 
 
+@pytest.mark.parametrize("tuning", [(16, 1), (4, 1), (2, 1)])
+def test_profile_channel_strips_match_oracle(tuning):
+    """The stencil's multi-row strips (a 2048^2 grid runs 16-row strips; the
+    small decks above run one-row strips): forced on the 64 x 48 channel."""
+    d = decks.profile_channel(convert=_conv_oracle)
+    print("case strips", tuning, flush=True)
+    orc = Oracle(d.params, fresh=True).load_deck(d)
+    dev = Solver(d.params).load_deck(d)
+    try:
+        dev.set_tuning(*tuning)
+        n1 = orc.run(10, dt0=d.dt0)
+        n2 = dev.advance(10, dt0=d.dt0)
+        assert n1 == n2 == 10
+        assert dev.stats()["nan_events"] == 0
+        _check(orc, dev, d.params)
+    finally:
+        dev.close()
+
+
+@pytest.mark.parametrize("kind", ["nrk3", "model8"])
+def test_tableau_path_strips_match_oracle(kind):
+    """The tableau-driven path without the profiles on multi-row strips."""
+    if kind == "nrk3":
+        d = decks.small_2d(n=40, ny=36, seed=5, n_rk=3)
+    else:
+        d = decks.profile_channel(vertical_profiles=False)
+    print("case tableau strips", kind, flush=True)
+    orc = Oracle(d.params, fresh=True).load_deck(d)
+    dev = Solver(d.params).load_deck(d)
+    try:
+        dev.set_tuning(16, 1)
+        n1 = orc.run(10, dt0=d.dt0)
+        n2 = dev.advance(10, dt0=d.dt0)
+        assert n1 == n2 == 10
+        _check(orc, dev, d.params)
+    finally:
+        dev.close()
+
+
 def test_profile_channel_1d_matches_oracle():
     d = decks.profile_channel(nx=200, ny=1, convert=_conv_oracle)
     orc, dev = _run_pair(d, 60)
@@ -152,5 +192,44 @@ def test_model8_without_profiles_matches_oracle(n_rk):
     orc, dev = _run_pair(d, 40)
     try:
         _check(orc, dev, d.params)
+    finally:
+        dev.close()
+
+
+# --------------------------------------------------------------------------
+# rheology model 6 (src/constitutive_2d.f90:3639-3650)
+# --------------------------------------------------------------------------
+def test_reference_box_run_matches_oracle():
+    """The reference's own TESTS/TEST_BOX run (model 6, settling and
+    entrainment, energy on) from its restart file to T_END = 200 s: 1103
+    steps, device against oracle."""
+    import os
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden",
+                        "INIT_BOX_0000.q_2d")
+    d = decks.reference_box(path)
+    print("case", d.name, flush=True)
+    orc = Oracle(d.params, fresh=True).load_deck(d)
+    dev = Solver(d.params).load_deck(d)
+    try:
+        n1 = orc.run(100000, dt0=d.dt0, t_end=d.t_end)
+        n2 = dev.advance(100000, dt0=d.dt0, t_end=d.t_end)
+        assert n1 == n2 and orc.t == dev.t == d.t_end, (n1, n2, orc.t, dev.t)
+        _check(orc, dev, d.params)
+        so, sd = orc.counters(), dev.stats()
+        assert sd["newton_iters"] == so["newton_iters"]
+        assert sd["newton_cells"] == so["newton_cells"]
+    finally:
+        dev.close()
+
+
+@pytest.mark.parametrize("n_rk", [2, 3])
+@pytest.mark.parametrize("slope", [False, True])
+def test_model6_small_2d_matches_oracle(n_rk, slope):
+    d = decks.small_2d(n=40, ny=36, seed=5, n_rk=n_rk, rheology_model=6, friction_factor=2e-3,
+                       slope_correction=slope, curvature=slope)
+    orc, dev = _run_pair(d, 30)
+    try:
+        _check(orc, dev, d.params)
+        assert np.array_equal(dev.mask(), orc.mask())
     finally:
         dev.close()

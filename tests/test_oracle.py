@@ -216,6 +216,41 @@ def test_voellmy_stage_closed_form(mu, xi):
         assert np.allclose(out[[0, 3, 4, 5]], q0[[0, 3, 4, 5]], rtol=1e-15)
 
 
+@pytest.mark.parametrize("ff", [1e-3, 0.05])
+def test_turbulent_drag_stage_closed_form(ff):
+    """Rheology model 6 (src/constitutive_2d.f90:3639-3650), N_RK = 2 stage
+    with zero explicit residual: no semi-implicit part, the implicit drag
+    rho_m f |u|^2 along the velocity, so the momentum magnitude solves
+    s + k s^2 = s0 with k = dt rho_m f / q1^2, direction kept."""
+    p = make_params(8, 8, 1.0, 2, energy_flag=False, rheology_model=6, friction_factor=ff,
+                    rho_s=(2500.0, 2200.0), diam_s=(1e-3, 1e-3), sp_heat_s=(1617.0, 1500.0))
+    o = Oracle(p)
+    rng = np.random.default_rng(6)
+    D0 = np.zeros(p.n_vars)
+    Dt = o.lib.orc_local_update.argtypes[1]
+    for _ in range(40):
+        h = 10 ** rng.uniform(-1, 1.3)
+        a = rng.uniform(0.005, 0.05, 2)
+        T = rng.uniform(300, 800)
+        u, v = rng.normal(size=2) * 10
+        dt = 10 ** rng.uniform(-3, -0.5)
+        qp = np.array([h, h * u, h * v, T, *(a * h), u, v])
+        q0 = _qp_to_qc(o, qp)
+        out = np.zeros(p.n_vars)
+        it = o.lib.orc_local_update(o.h, q0.ctypes.data_as(Dt), D0.ctypes.data_as(Dt), dt,
+                                    out.ctypes.data_as(Dt))
+        rho_m, _ = _mix_rho_g(p, h, a, T)
+        q1 = q0[0]
+        s0 = math.hypot(q0[1], q0[2])
+        k = dt * rho_m * ff / (q1 * q1)
+        s = (-1.0 + math.sqrt(1.0 + 4.0 * k * s0)) / (2.0 * k)
+        assert it >= 1
+        assert math.isclose(math.hypot(out[1], out[2]), s, rel_tol=1e-10), (
+            math.hypot(out[1], out[2]), s)
+        assert math.isclose(out[1] * q0[2], out[2] * q0[1], rel_tol=1e-9, abs_tol=1e-9 * s0 * s0)
+        assert np.allclose(out[[0, 3, 4, 5]], q0[[0, 3, 4, 5]], rtol=1e-15)
+
+
 def test_complex_step_matches_finite_differences():
     for energy in (True, False):
         p = make_params(8, 8, 1.0, 2, energy_flag=energy, mu=0.1, xi=500.0,
