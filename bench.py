@@ -153,11 +153,14 @@ def host_cpu():
     return model, (len(cores) or None)
 
 
-def cpu_baseline(workload, steps, threads, start=None):
+def cpu_baseline(workload, steps, threads, start=None, device_q=None):
     """Time the oracle (reference interface semantics) on the host cores
     over `steps` timesteps of the workload: from t = 0, or from the state
     `start` (q, erodible, B_cent, t, dt) that the device reached, so that
-    the sample covers the same window as the timed device steps."""
+    the sample covers the same window as the timed device steps.  With
+    `device_q` (the device's state after the same steps from t = 0) the
+    oracle also checks it: the largest relative difference per conserved
+    variable is reported."""
     from oracle.pyoracle import Oracle
     d = make_deck(workload)
     o = Oracle(d.params, fresh=False, threads=threads).load_deck(d)
@@ -175,6 +178,11 @@ def cpu_baseline(workload, steps, threads, start=None):
     n = o.run(steps, dt0=dt0)
     el = time.perf_counter() - t0
     c = o.counters()
+    diff = None
+    if device_q is not None:
+        qo = o.q()
+        diff = max(float(np.abs(device_q[v] - qo[v]).max() / max(np.abs(qo[v]).max(), 1e-300))
+                   for v in range(qo.shape[0]))
     o.close()
     cells = d.params.nx * d.params.ny
     model, phys = host_cpu()
@@ -192,6 +200,7 @@ def cpu_baseline(workload, steps, threads, start=None):
         "host_cpu": model,
         "host_physical_cores": phys,
         "active_cell_updates_per_s": c["active_cell_updates"] / el,
+        "max_rel_diff_device_vs_oracle": diff,
     }
 
 
@@ -625,7 +634,14 @@ def main():
         # the profile closures make the oracle's cell-updates about 30x
         # dearer than the plain path's: its sample is --cpu-steps steps
         csteps = args.cpu_steps if args.workload == "profiles" else max(args.cpu_steps, K)
-        out["cpu_baseline"] = cpu_baseline(args.workload, csteps, threads,
+        dq = None
+        if args.workload == "profiles":
+            # the device's state after the sample's steps, for the oracle to check
+            v = Solver(p).load_deck(d)
+            v.advance(csteps, dt0=d.dt0)
+            dq = v.q()
+            v.close()
+        out["cpu_baseline"] = cpu_baseline(args.workload, csteps, threads, device_q=dq,
                                            start=start_state if develop else None)
     print(json.dumps(out), flush=True)
     return 0
