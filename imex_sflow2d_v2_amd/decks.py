@@ -521,6 +521,149 @@ def reference_box(q_path):
                 description="TESTS/TEST_BOX 400x1 dx=125 model 6")
 
 
+def _open_text(path):
+    import gzip
+    if str(path).endswith(".gz"):
+        return gzip.open(path, "rt")
+    return open(path)
+
+
+def read_esri_dem(path):
+    """An ESRI ASCII grid as the reference reads a DEM
+    (src/inpout_2d.f90:3480-3603): the header values and the node heights
+    f[ncols, nrows], row k = 1 at the bottom (the file starts at the top
+    row).  ``path`` may be gzip-compressed."""
+    with _open_text(path) as fh:
+        lines = fh.read().split("\n")
+    hdr = {}
+    for ln in lines[:6]:
+        key, val = ln.split()[:2]
+        hdr[key.lower()] = float(val)
+    ncols, nrows = int(hdr["ncols"]), int(hdr["nrows"])
+    vals = np.array(" ".join(lines[6:]).split(), dtype=np.float64)
+    if vals.size != ncols * nrows:
+        raise ValueError("%s: %d heights for a %d x %d grid" % (path, vals.size, ncols, nrows))
+    rows = vals.reshape(nrows, ncols)
+    f = np.empty((ncols, nrows))
+    for r in range(nrows):
+        f[:, nrows - 1 - r] = rows[r]
+    return hdr, f
+
+
+def _interp_index(x1, x2):
+    """Cell index and weight of interp_2d_scalar along one axis
+    (src/geometry_2d.f90:1174-1226), 0-based."""
+    n = x1.size
+    if n == 1:
+        return np.zeros(x2.shape, dtype=np.int64), np.zeros(x2.shape)
+    ix = np.floor((x2 - x1[0]) / (x1[1] - x1[0])).astype(np.int64)
+    ix = np.minimum(ix, n - 2)
+    alfa = (x1[ix + 1] - x2) / (x1[ix + 1] - x1[ix])
+    return ix, alfa
+
+
+def dem_to_b_cent(hdr, f, nx, ny, x0, y0, cell_size):
+    """B_cent(nx, ny) and B_nodata(nx, ny) from a DEM as init_grid makes them
+    (src/geometry_2d.f90:173-365): cell centres from the staggered nodes,
+    the nodata test on the four surrounding heights (interp_2d_nodata,
+    :1126-1172), then bilinear interpolation (interp_2d_scalar, :1174-1226)
+    of the heights clipped at zero."""
+    ncols, nrows = f.shape
+    cs, xll, yll = hdr["cellsize"], hdr["xllcorner"], hdr["yllcorner"]
+    x1 = xll + (np.arange(1, ncols + 1) - 0.5) * cs
+    y1 = yll + (np.arange(1, nrows + 1) - 0.5) * cs
+    dx = cell_size if nx > 1 else 1.0
+    dy = cell_size if ny > 1 else 1.0
+    xs = x0 + np.arange(nx + 1) * dx
+    ys = y0 + np.arange(ny + 1) * dy
+    xc = 0.5 * (xs[:-1] + xs[1:])
+    yc = 0.5 * (ys[:-1] + ys[1:])
+    ix, ax = _interp_index(x1, xc)
+    iy, ay = _interp_index(y1, yc)
+    I, J = ix[:, None], iy[None, :]
+    AX, AY = ax[:, None], ay[None, :]
+    nod = hdr.get("nodata_value", -9999.0)
+    if ncols == 1:
+        nodata = (f[I, J] == nod) | (f[I, J + 1] == nod)
+    elif nrows == 1:
+        nodata = (f[I, J] == nod) | (f[I + 1, J] == nod)
+    else:
+        nodata = ((f[I, J] == nod) | (f[I, J + 1] == nod) | (f[I + 1, J] == nod)
+                  | (f[I + 1, J + 1] == nod))
+    g = np.maximum(0.0, f)
+    if ncols == 1:
+        b = AY * g[I, J] + (1.0 - AY) * g[I, J + 1]
+    elif nrows == 1:
+        b = AX * g[I, J] + (1.0 - AX) * g[I + 1, J]
+    else:
+        b = (AX * (AY * g[I, J] + (1.0 - AY) * g[I, J + 1])
+             + (1.0 - AX) * (AY * g[I + 1, J] + (1.0 - AY) * g[I + 1, J + 1]))
+    return np.asfortranarray(b), np.asfortranarray(nodata.astype(np.uint8))
+
+
+def _read_q2d_any(path, nv, ns, nx, ny):
+    from .solver import q2d_read
+    if not str(path).endswith(".gz"):
+        return q2d_read(path, nv, ns, nx, ny)
+    import gzip
+    import tempfile
+    with gzip.open(path, "rb") as g, tempfile.NamedTemporaryFile(suffix=".q_2d") as t:
+        t.write(g.read())
+        t.flush()
+        return q2d_read(t.name, nv, ns, nx, ny)
+
+
+def reference_bw(q_path, dem_path):
+    """The reference's own TESTS/TEST_BW run: TEST_BOX's deck (400 x 1 cells
+    of 125 m, model 6 with FRICTION_FACTOR 1e-3, settling of one 2000 kg/m3
+    class and air entrainment, the energy equation, N_RK = 2, limiter 1,
+    THETA 1.3, CFL 0.23999999463558197, MAX_DT 0.2, DT0 1e-4) with 1 mm
+    particles on its DEM (a 3-degree incline, ``dem_path``), a Dirichlet
+    inflow on the west (h = 400 m, hu = 10^4 m2/s, h alpha = 0.1, T = 900 K)
+    and zero-gradient outflow on the east, to T_END = 400 s from its restart
+    file INIT_BW_0000.q_2d (``q_path``)."""
+    nx, ny, ns = 400, 1, 1
+    nv = 4 + ns
+    bcw = [(0, 400.0), (0, 10000.0), (0, 0.0), (0, 900.0), (0, 0.1)]
+    bce = [(1, 0.0), (1, 0.0), (0, 0.0), (1, 0.0), (1, 0.0)]
+    p = make_params(nx, ny, 125.0, ns, n_rk=2, energy_flag=True, cfl=0.23999999463558197,
+                    max_dt=0.2, theta=1.3, reconstr_coeff=1.0, limiter=1,
+                    bc={0: bcw, 1: bce}, grav=9.81, pres=101300.0, T_ambient=300.0,
+                    sp_heat_a=998.0, sp_gas_const_a=287.051, kin_visc_a=1.48e-5,
+                    rho_s=(2000.0,), diam_s=(1e-3,), sp_heat_s=(1617.0,), settling=True,
+                    entrainment=True, erosion_coeff=0.0, rheology=True, rheology_model=6,
+                    friction_factor=1e-3)
+    hdr, f = read_esri_dem(dem_path)
+    B, _ = dem_to_b_cent(hdr, f, nx, ny, 0.0, 0.0, 125.0)
+    q = _read_q2d_any(q_path, nv, ns, nx, ny)
+    return Deck("reference_bw", p, B, q, t_end=400.0, dt0=1e-4,
+                description="TESTS/TEST_BW 400x1 dx=125 incline, inflow, model 6")
+
+
+def reference_2d(q_path, dem_path):
+    """The reference's own TESTS/TEST_2D run: 100 x 68 cells of 0.3 m from
+    (0, -10.2) on its DEM (``dem_path``), one solid class (2000 kg/m3,
+    0.1 mm) with settling and air entrainment, the energy equation, rheology
+    model 6 with FRICTION_FACTOR 1e-3, N_RK = 2, limiter 1, THETA 1.3,
+    CFL 0.24, MAX_DT 1, DT0 1e-4, zero-gradient boundaries on every side,
+    to T_END = 1.5 s from its restart file INIT_2D_0000.q_2d (``q_path``):
+    a pile at rest released on the bed."""
+    nx, ny, ns = 100, 68, 1
+    nv = 4 + ns
+    p = make_params(nx, ny, 0.3, ns, x0=0.0, y0=-10.2, n_rk=2, energy_flag=True, cfl=0.24,
+                    max_dt=1.0, theta=1.3, reconstr_coeff=1.0, limiter=1,
+                    grav=9.81, pres=101300.0, T_ambient=300.0,
+                    sp_heat_a=998.0, sp_gas_const_a=287.051, kin_visc_a=1.48e-5,
+                    rho_s=(2000.0,), diam_s=(1e-4,), sp_heat_s=(1617.0,), settling=True,
+                    entrainment=True, erosion_coeff=0.0, rheology=True, rheology_model=6,
+                    friction_factor=1e-3)
+    hdr, f = read_esri_dem(dem_path)
+    B, _ = dem_to_b_cent(hdr, f, nx, ny, 0.0, -10.2, 0.3)
+    q = _read_q2d_any(q_path, nv, ns, nx, ny)
+    return Deck("reference_2d", p, B, q, t_end=1.5, dt0=1e-4,
+                description="TESTS/TEST_2D 100x68 dx=0.3 DEM, model 6")
+
+
 CONFIGS = {
     "dambreak_1d": dambreak_1d,
     "incline_2d": incline_2d,

@@ -27,6 +27,25 @@ CASES = [
 ]
 
 
+# Input data files of the reference's TESTS decks that the device-versus-oracle
+# runs start from (restart states and their DEMs), kept gzip-compressed.
+INPUT_FILES = [
+    ("TESTS/TEST_BW/INIT_BW_0000.q_2d", "INIT_BW_0000.q_2d.gz"),
+    ("TESTS/TEST_BW/topography_dem.asc", "TEST_BW_topography_dem.asc.gz"),
+    ("TESTS/TEST_2D/INIT_2D_0000.q_2d", "INIT_2D_0000.q_2d.gz"),
+    ("TESTS/TEST_2D/topography_dem.asc", "TEST_2D_topography_dem.asc.gz"),
+]
+
+
+def copy_inputs(here):
+    import gzip
+    for src, dst in INPUT_FILES:
+        with open(os.path.join(REF, src), "rb") as f:
+            data = f.read()
+        with gzip.GzipFile(os.path.join(here, dst), "wb", compresslevel=9, mtime=0) as g:
+            g.write(data)
+
+
 def main():
     out = []
     for c in CASES:
@@ -44,6 +63,7 @@ def main():
     with open(os.path.join(here, "fixtures.json"), "w") as f:
         json.dump(out, f, indent=1)
     print(json.dumps(out, indent=1))
+    copy_inputs(here)
 
 
 if __name__ == "__main__":

@@ -222,6 +222,46 @@ def test_reference_box_run_matches_oracle():
         dev.close()
 
 
+def _golden(name):
+    import os
+    return os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", name)
+
+
+@pytest.mark.parametrize("which", ["2d", "bw"])
+def test_reference_decks_run_to_t_end_match_oracle(which):
+    """The reference's own TESTS/TEST_2D (a pile released on its 100 x 68
+    DEM, model 6, settling and entrainment, to T_END = 1.5 s) and TESTS/TEST_BW
+    (a hot dilute Dirichlet inflow on a 1-D 3-degree incline, to T_END =
+    400 s: the inflow is lighter than the ambient air and lifts off in the
+    first cell every step, as the reference's erosion-deposition update
+    prescribes), device against oracle from their restart files."""
+    if which == "2d":
+        d = decks.reference_2d(_golden("INIT_2D_0000.q_2d.gz"),
+                               _golden("TEST_2D_topography_dem.asc.gz"))
+    else:
+        d = decks.reference_bw(_golden("INIT_BW_0000.q_2d.gz"),
+                               _golden("TEST_BW_topography_dem.asc.gz"))
+    print("case", d.name, flush=True)
+    orc = Oracle(d.params, fresh=True).load_deck(d)
+    dev = Solver(d.params).load_deck(d)
+    try:
+        n1 = orc.run(100000, dt0=d.dt0, t_end=d.t_end)
+        n2 = dev.advance(100000, dt0=d.dt0, t_end=d.t_end)
+        assert n1 == n2 and orc.t == dev.t == d.t_end, (n1, n2, orc.t, dev.t)
+        so, sd = orc.counters(), dev.stats()
+        assert sd["newton_iters"] == so["newton_iters"]
+        assert sd["newton_cells"] == so["newton_cells"]
+        assert sd["nan_events"] == 0
+        if which == "2d":
+            _check(orc, dev, d.params)
+            assert np.array_equal(dev.mask(), orc.mask())
+            assert (orc.q()[0] > 0).sum() > 1000          # the pile has spread
+        else:
+            assert np.array_equal(dev.q(), orc.q())
+    finally:
+        dev.close()
+
+
 @pytest.mark.parametrize("n_rk", [2, 3])
 @pytest.mark.parametrize("slope", [False, True])
 def test_model6_small_2d_matches_oracle(n_rk, slope):

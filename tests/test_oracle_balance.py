@@ -352,3 +352,41 @@ def test_mass_balance_radial_source_and_deposition(settling):
     if settling:
         assert deposited > 0
     assert math.isclose(dm, dt * inflow - deposited, rel_tol=1e-11), (dm, dt * inflow, deposited)
+
+
+# --------------------------------------------------------------------------
+# lift-off: a cell whose mixture is lighter than the ambient air (g' <= 0)
+# is emptied by the erosion-deposition update (src/solver_2d.f90:2570-2587)
+# --------------------------------------------------------------------------
+@pytest.mark.parametrize("T", [600.0, 900.0])
+def test_buoyant_cells_are_emptied(T):
+    """Two cells of hot, dilute flow on either side of the solid fraction
+    alpha* = (rho_a - rho_g(T)) / (rho_s - rho_g(T)) at which the mixture is
+    as dense as the ambient air: the lighter one is emptied, the denser one
+    kept (nothing else changes it: no settling, and erosion of an empty
+    erodible layer)."""
+    n = 8
+    p = make_params(n, n, 10.0, 1, energy_flag=True, rho_s=(2000.0,), diam_s=(1e-4,),
+                    erosion_coeff=1e-3, rheology=False)
+    rho_g = p.pres / (p.sp_gas_const_a * T)
+    rho_a = p.pres / (p.sp_gas_const_a * p.T_ambient)
+    a_star = (rho_a - rho_g) / (p.rho_s[0] - rho_g)
+    h = 5.0
+    hh = np.zeros((n, n))
+    a = np.zeros((n, n))
+    hh[3, 3] = hh[5, 5] = h
+    a[3, 3] = a_star * (1.0 - 1e-6)
+    a[5, 5] = a_star * (1.0 + 1e-6)
+    qp = np.zeros((p.n_vars + 2, n, n))
+    qp[0] = hh
+    qp[3] = T
+    qp[4] = a * hh
+    q = np.asfortranarray(Oracle(p).qp_to_qc(qp))
+    for c in ((3, 3), (5, 5)):
+        assert abs(_rho_m_gp(p, [a[c]], T)[2]) < 1e-4       # near the threshold
+    o = Oracle(p).load_deck(decks.Deck("buoyant", p, np.zeros((n, n)), q, dt0=1e-3))
+    o.check_solve()
+    o.erosion_deposition(1e-3)
+    q1 = o.q()
+    assert np.all(q1[:, 3, 3] == 0.0)
+    assert np.array_equal(q1[:, 5, 5], q[:, 5, 5])

@@ -7,6 +7,7 @@ numpy's parser; the writer against Fortran E20.12 output strings fixed by the
 language's E editing rules (0.d mantissa, 12 digits, E+dd or +ddd exponent,
 right-justified), and both by round trips.
 """
+import math
 import os
 
 import numpy as np
@@ -116,3 +117,104 @@ def test_writer_non_finite_values(tmp_path):
     q = q2d_read(str(tmp_path / "one.q_2d"), 4, 0, 1, 1)
     assert np.isnan(q[0, 0, 0]) and q[1, 0, 0] == np.inf and q[2, 0, 0] == -np.inf
     assert q[3, 0, 0] == 1.5
+
+
+# --------------------------------------------------------------------------
+# the reference's TEST_2D and TEST_BW inputs: restart states and ESRI DEMs
+# (decks.read_esri_dem / dem_to_b_cent, host set-up of init_grid)
+# --------------------------------------------------------------------------
+GOLD = os.path.join(HERE, "golden")
+
+
+def test_reference_2d_restart_fixture_reads_like_numpy():
+    import gzip
+    from imex_sflow2d_v2_amd import decks
+    path = os.path.join(GOLD, "INIT_2D_0000.q_2d.gz")
+    q = decks._read_q2d_any(path, 5, 1, 100, 68)
+    with gzip.open(path, "rt") as f:
+        ref = np.loadtxt(f)
+    assert ref.shape == (6800, 7)
+    # rows run over x fastest, then y (output_solution's loop order)
+    expect = ref[:, 2:].T.reshape(5, 68, 100).transpose(0, 2, 1)
+    expect[:, expect[0] <= 0.0] = 0.0
+    assert np.array_equal(q, expect)
+    assert (q[0] > 0).sum() == 118
+    # cell centres of the 100 x 68 grid of 0.3 m cells from (0, -10.2)
+    np.testing.assert_allclose(ref[:100, 0], 0.15 + 0.3 * np.arange(100), rtol=0, atol=1e-9)
+    np.testing.assert_allclose(ref[::100, 1], -10.05 + 0.3 * np.arange(68), rtol=0, atol=1e-9)
+
+
+def _write_dem(tmp_path, f, xll, yll, cs, nodata=-9999.0):
+    ncols, nrows = f.shape
+    lines = ["ncols %d" % ncols, "nrows %d" % nrows, "xllcorner %r" % xll,
+             "yllcorner %r" % yll, "cellsize %r" % cs, "NODATA_value %r" % nodata]
+    for r in range(nrows):
+        lines.append(" ".join(repr(float(v)) for v in f[:, nrows - 1 - r]))
+    path = os.path.join(str(tmp_path), "dem.asc")
+    with open(path, "w") as fh:
+        fh.write("\n".join(lines) + "\n")
+    return path
+
+
+def test_dem_bilinear_interpolation_is_exact_on_bilinear_fields(tmp_path):
+    """Bilinear interpolation reproduces a + b x + c y + d x y at any point
+    inside the DEM; the DEM rows are stored top row first; heights are
+    clipped at zero before interpolation and nodata marks the cells whose
+    four surrounding nodes include one."""
+    from imex_sflow2d_v2_amd import decks
+    ncols, nrows, cs, xll, yll = 23, 17, 2.0, -1.0, 3.0
+    xn = xll + (np.arange(1, ncols + 1) - 0.5) * cs
+    yn = yll + (np.arange(1, nrows + 1) - 0.5) * cs
+
+    def field(x, y):
+        return 50.0 + 0.7 * x - 1.3 * y + 0.02 * x * y
+
+    f = field(xn[:, None], yn[None, :])
+    hdr, g = decks.read_esri_dem(_write_dem(tmp_path, f, xll, yll, cs))
+    assert np.array_equal(g, f)
+    # an off-grid computational grid: 13 x 9 cells of 2.7 m from (1.3, 4.1)
+    nx, ny, dx, x0, y0 = 13, 9, 2.7, 1.3, 4.1
+    B, nod = decks.dem_to_b_cent(hdr, g, nx, ny, x0, y0, dx)
+    xc = x0 + (np.arange(nx) + 0.5) * dx
+    yc = y0 + (np.arange(ny) + 0.5) * dx
+    np.testing.assert_allclose(B, field(xc[:, None], yc[None, :]), rtol=1e-13)
+    assert not nod.any()
+    # clipping and nodata
+    f2 = f - 60.0
+    f2[5, 4] = -9999.0
+    hdr2, g2 = decks.read_esri_dem(_write_dem(tmp_path, f2, xll, yll, cs))
+    B2, nod2 = decks.dem_to_b_cent(hdr2, g2, nx, ny, x0, y0, dx)
+    assert B2.min() >= 0.0
+    ix = np.floor((xc - xn[0]) / cs).astype(int)
+    iy = np.floor((yc - yn[0]) / cs).astype(int)
+    touch = ((ix[:, None] == 5) | (ix[:, None] == 4)) & ((iy[None, :] == 4) | (iy[None, :] == 3))
+    assert np.array_equal(nod2.astype(bool), touch)
+    pos = np.maximum(f2, 0.0)
+    j, k = 10, 7
+    ax = (xn[ix[j] + 1] - xc[j]) / cs
+    ay = (yn[iy[k] + 1] - yc[k]) / cs
+    expect = (ax * (ay * pos[ix[j], iy[k]] + (1 - ay) * pos[ix[j], iy[k] + 1])
+              + (1 - ax) * (ay * pos[ix[j] + 1, iy[k]] + (1 - ay) * pos[ix[j] + 1, iy[k] + 1]))
+    assert B2[j, k] == pytest.approx(expect, rel=1e-14)
+
+
+def test_reference_decks_topography():
+    """TEST_2D's DEM nodes sit on the grid's cell corners, so each B_cent is
+    the mean of its four corner heights; TEST_BW's 1-D DEM is a plane of
+    uniform slope (its cell centres are midway between nodes)."""
+    import gzip
+    from imex_sflow2d_v2_amd import decks
+    d = decks.reference_2d(os.path.join(GOLD, "INIT_2D_0000.q_2d.gz"),
+                           os.path.join(GOLD, "TEST_2D_topography_dem.asc.gz"))
+    with gzip.open(os.path.join(GOLD, "TEST_2D_topography_dem.asc.gz"), "rt") as fh:
+        rows = np.loadtxt(fh, skiprows=6)
+    f = rows[::-1].T                        # f[j, k], k = 0 at the bottom
+    corners = 0.25 * (f[:-1, :-1] + f[1:, :-1] + f[:-1, 1:] + f[1:, 1:])
+    np.testing.assert_allclose(d.B_cent, corners, rtol=1e-13, atol=1e-13)
+    assert d.params.nx == 100 and d.params.ny == 68
+    bw = decks.reference_bw(os.path.join(GOLD, "INIT_BW_0000.q_2d.gz"),
+                            os.path.join(GOLD, "TEST_BW_topography_dem.asc.gz"))
+    slope = np.diff(bw.B_cent[:, 0]) / 125.0
+    np.testing.assert_allclose(slope, slope.mean(), rtol=1e-9)
+    assert math.isclose(math.degrees(math.asin(-slope.mean())), 3.0, rel_tol=1e-9)
+    assert not bw.q.any()                   # the run starts from an empty domain
