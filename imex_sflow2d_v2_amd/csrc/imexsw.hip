@@ -710,6 +710,7 @@ int imexsw_create(const imexsw_params* params, imexsw_ctx** out) {
   rc |= dalloc(c, &D.tile_nact, (size_t)c->n_tiles);
   rc |= dalloc(c, &D.tile_list, (size_t)c->n_tiles);
   rc |= dalloc(c, &D.row_list, (size_t)c->n_tiles * kTileH);
+  rc |= dalloc(c, &D.cell_list, (size_t)P.nx * P.ny);
   c->strip_h = (P.ny > 1) ? kTileH : 1;
   c->row_unit = 1;
   {
@@ -887,6 +888,8 @@ int imexsw_check_solve(imexsw_ctx* c, int solve_all, double t, int32_t* solve_ce
                            kSlots * sizeof(unsigned long long), c->stream));
   HIPCHK(c, hipMemsetAsync(c->D.counters + C_TILES * kSlots, 0,
                            2 * kSlots * sizeof(unsigned long long), c->stream));  // tiles, rows
+  HIPCHK(c, hipMemsetAsync(c->D.counters + C_CELLS * kSlots, 0,
+                           kSlots * sizeof(unsigned long long), c->stream));
   if (run_mask(c, solve_all)) return -1;
   // the mask now reflects this call's solve_all: the next step's mask
   // scans every tile again

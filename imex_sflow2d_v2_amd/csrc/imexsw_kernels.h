@@ -75,6 +75,7 @@ struct DevPtrs {
   int* tile_nact;       // per work tile: active cells at its last mask scan
   int* tile_list;     // work tiles with a cell to compute (compacted)
   int* row_list;      // tile rows (tile * kTileH + row) with a cell to compute
+  int* cell_list;     // cells (plane index) to compute, or null
   double* guess;      // 2 planes: learned Newton start for the momenta (NaN: none)
   float* harvest;     // harvest_cap pairs of kHarvestWidth floats, or null
   long long harvest_cap;
@@ -87,7 +88,7 @@ enum CounterSlot { C_SOLVE_CELLS = 0, C_NEWTON_CELLS, C_NEWTON_ITERS, C_NONCONV,
                    C_HIST0, C_HIST1, C_HIST2, C_HIST3, C_HIST4, C_HIST5, C_HIST6,
                    C_HIST7, C_WARM_USED, C_WARM_FALLBACK, C_ITMAX, C_STEPS,
                    C_ACTIVE_UPD, C_COMPUTE_CELLS, C_TILES, C_ROWS, C_HARVEST,
-                   C_COMPUTE_UPD, C_NCOUNTERS };
+                   C_COMPUTE_UPD, C_CELLS, C_NCOUNTERS };
 // k_local's per-cell event counters, in CounterSlot order from C_NEWTON_CELLS
 constexpr int kLocalEvents = C_ITMAX - C_NEWTON_CELLS;  // summed ones
 // Newton iteration histogram buckets: 1, 2, 3, 4, 5-8, 9-16, 17-32, 33-100
@@ -168,17 +169,22 @@ __device__ __forceinline__ double shfl_d(double v, int src) {
 // ---------------------------------------------------------------------------
 constexpr int kMaskGroup = 4;
 // Where k_mask appends the tiles and tile rows that have a cell to compute:
-// entries are `tag | tile` and `tag | (tile * kTileH + row)`.
+// entries are `tag | tile` and `tag | (tile * kTileH + row)`; with `cells`,
+// also the cells themselves (plane indices, tile by tile, row-major within a
+// tile).
 struct ListSink {
   int* tiles;
   int* rows;
   unsigned long long* n_tiles;
   unsigned long long* n_rows;
   int tag;
+  int* cells;
+  unsigned long long* n_cells;
 };
 
 __device__ __forceinline__ ListSink own_sink(const DevPtrs& D) {
-  return ListSink{D.tile_list, D.row_list, ctr(D, C_TILES, 0), ctr(D, C_ROWS, 0), 0};
+  return ListSink{D.tile_list, D.row_list, ctr(D, C_TILES, 0), ctr(D, C_ROWS, 0), 0,
+                  D.cell_list, D.cell_list ? ctr(D, C_CELLS, 0) : nullptr};
 }
 
 // Bit-parallel check_solve for one tile (tx, ty), one 64-lane wavefront.
@@ -204,16 +210,19 @@ __device__ __forceinline__ unsigned lane_read(unsigned v, int src) {
 
 // Writes the tile's mask bytes, counters and tile flag; returns the
 // (wavefront-uniform) bit word of the tile rows with a cell to compute.
+// `comp_out` (when given) receives this lane's column of cells to compute,
+// bit r = tile row r.
 // `clean`: a partial scan found the tile not dirty, so it keeps its mask
 // bytes and counts (nothing within reach changed state since its last scan,
 // so neither can have changed, and it has no cell to compute).  The
 // active-cell counter is a running total kept by per-tile differences.
 __device__ __forceinline__ unsigned mask_tile(const DevParams& P, const DevPtrs& D, int tx,
                                               int tyy, int tiles_x, int solve_all,
-                                              bool clean) {
+                                              bool clean, unsigned* comp_out = nullptr) {
   constexpr int LH = kTileH + 4;
   const int lane = threadIdx.x & 63;
   const int tile = tyy * tiles_x + tx;
+  if (comp_out) *comp_out = 0u;
   if (clean) return 0u;
   // the tile's previous active count, loaded with the flags
   const int nact_old = D.tile_nact[tile];
@@ -290,6 +299,7 @@ __device__ __forceinline__ unsigned mask_tile(const DevParams& P, const DevPtrs&
       }
     }
   }
+  if (comp_out) *comp_out = own ? (comp >> 2) : 0u;
   int count = __popc(act), ccount = __popc(comp);
   unsigned rows_m = comp >> 2;
   for (int off = 32; off > 0; off >>= 1) {
@@ -311,39 +321,71 @@ __device__ __forceinline__ unsigned mask_tile(const DevParams& P, const DevPtrs&
 // Appends the workgroup's tiles with a row to compute (wavefront w: tile
 // `tile`, compute rows `rows_m`, 0 for none or for an idle wavefront) to the
 // work lists with one reservation per list for the whole workgroup.  Every
-// thread of the workgroup calls it.
+// thread of the workgroup calls it.  With a cell list in the sink, `comp` is
+// the lane's column of cells to compute (bit r = tile row r) and `cell0` the
+// plane index of its row-0 cell; the cells go to the list tile row by tile
+// row, in lane order within a row; entries past `cap` (the list's length)
+// are not written.
 template <int G>
-__device__ __forceinline__ void append_group(const ListSink& L, int tile, unsigned rows_m) {
+__device__ __forceinline__ void append_group(const ListSink& L, int tile, unsigned rows_m,
+                                             unsigned comp = 0u, int cell0 = 0,
+                                             int pitch = 0, long long cap = 0) {
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   __shared__ unsigned s_rows[G];
-  __shared__ int s_tb, s_rb;
-  if (lane == 0) s_rows[w] = rows_m;
+  __shared__ int s_cells[G];
+  __shared__ int s_tb, s_rb, s_cb;
+  const bool cells = L.cells != nullptr;
+  int nc_w = 0;
+  if (cells) {
+    nc_w = __popc(comp);
+    for (int off = 32; off > 0; off >>= 1) nc_w += __shfl_xor(nc_w, off, 64);
+  }
+  if (lane == 0) {
+    s_rows[w] = rows_m;
+    s_cells[w] = nc_w;
+  }
   __syncthreads();
   if (threadIdx.x == 0) {
-    int nt = 0, nr = 0;
+    int nt = 0, nr = 0, nc = 0;
     for (int g = 0; g < G; g++) {
       nt += s_rows[g] != 0u;
       nr += __popc(s_rows[g]);
+      nc += s_cells[g];
     }
-    // nr > 0 exactly when nt > 0: one branch, both reservations in flight
-    int tb = 0, rb = 0;
+    // nr > 0 exactly when nt > 0 (and nc > 0 only then): one branch, the
+    // reservations in flight together
+    int tb = 0, rb = 0, cb = 0;
     if (nt) {
       tb = (int)atomicAdd(L.n_tiles, (unsigned long long)nt);
       rb = (int)atomicAdd(L.n_rows, (unsigned long long)nr);
+      if (cells && nc) cb = (int)atomicAdd(L.n_cells, (unsigned long long)nc);
     }
     s_tb = tb;
     s_rb = rb;
+    s_cb = cb;
   }
   __syncthreads();
   if (rows_m) {
-    int tb = s_tb, rb = s_rb;
+    int tb = s_tb, rb = s_rb, cb = s_cb;
     for (int g = 0; g < w; g++) {
       tb += s_rows[g] != 0u;
       rb += __popc(s_rows[g]);
+      cb += s_cells[g];
     }
     if (lane == 0) L.tiles[tb] = L.tag | tile;
     if (lane < kTileH && ((rows_m >> lane) & 1u))
       L.rows[rb + __popc(rows_m & ((1u << lane) - 1u))] = L.tag | (tile * kTileH + lane);
+    if (cells) {
+      const unsigned long long below = (1ull << lane) - 1ull;
+      for (int r = 0; r < kTileH; r++) {
+        if (!((rows_m >> r) & 1u)) continue;
+        const bool mine = (comp >> r) & 1u;
+        const unsigned long long b = __ballot(mine);
+        const long long at = (long long)cb + __popcll(b & below);
+        if (mine && at < cap) L.cells[at] = cell0 + r * pitch;
+        cb += __popcll(b);
+      }
+    }
   }
 }
 
@@ -372,9 +414,13 @@ __global__ void __launch_bounds__(64 * kMaskGroup) k_mask(DevParams P, DevPtrs D
     if (!any) return;
   }
   const int tyy = blockIdx.y * kMaskGroup + w;
-  unsigned rows_m = 0;
-  if (tyy < tiles_y) rows_m = mask_tile(P, D, blockIdx.x, tyy, tiles_x, solve_all, clean);
-  append_group<kMaskGroup>(own_sink(D), tyy * tiles_x + blockIdx.x, rows_m);
+  unsigned rows_m = 0, comp = 0;
+  if (tyy < tiles_y)
+    rows_m = mask_tile(P, D, blockIdx.x, tyy, tiles_x, solve_all, clean, &comp);
+  const int j = blockIdx.x * kStripW - 2 + (int)(threadIdx.x & 63);
+  append_group<kMaskGroup>(own_sink(D), tyy * tiles_x + blockIdx.x, rows_m, comp,
+                           (int)cidx(P, max(j, 0), tyy * kTileH), (int)P.pitch,
+                           (long long)P.nx * P.ny);
 }
 
 // ---------------------------------------------------------------------------
@@ -1451,6 +1497,7 @@ __device__ __forceinline__ void reset_member(const DevParams& P, const DevPtrs& 
   for (int i = 0; i < kSlots; i++) *ctr(D, C_COMPUTE_CELLS, i) = 0;
   *ctr(D, C_TILES, 0) = 0;
   *ctr(D, C_ROWS, 0) = 0;
+  *ctr(D, C_CELLS, 0) = 0;
 }
 
 // Between two steps of imexsw_advance: commit the previous step (its time
@@ -1790,24 +1837,27 @@ __device__ __forceinline__ void local_row(const DevParams& P, const DevPtrs& D, 
   }
 }
 
-// One part of a listed tile row for PH 3: the row's 60 cells in
-// kSparseLanes parts of 60 / kSparseLanes, cell i of the part on lanes
-// kSparseLanes*i .. kSparseLanes*i + kSparseLanes-1 (lanes 60..63 idle).  No
-// per-row sums.
+// Cells kCellsPerWave * item .. of the step's list of cells to compute, for
+// PH 3 and 4: cell i of the item on lanes kSparseLanes*i .. kSparseLanes*i +
+// kSparseLanes-1, which also mark its tile and the tiles next to it for the
+// next step's mask.  No per-row sums.
+constexpr int kCellsPerWave = kWave / kSparseLanes;
 template <int NS, int PH>
-__device__ __forceinline__ void local_part_row(const DevParams& P, const DevPtrs& D, int row,
-                                               int part, double dt, int ops,
-                                               unsigned long long* e, int& imax) {
-  constexpr int kPart = kStripW / kSparseLanes;
+__device__ __forceinline__ void local_cells(const DevParams& P, const DevPtrs& D, int item,
+                                            int ncells, double dt, int ops,
+                                            unsigned long long* e, int& imax) {
   const int lane = threadIdx.x & 63, i = lane / kSparseLanes;
-  const int tiles_x = (P.nx + kStripW - 1) / kStripW;
-  const int tile = row / kTileH;
-  const int j = (tile % tiles_x) * kStripW + part * kPart + i;
-  const int k = (tile / tiles_x) * kTileH + row % kTileH;
+  const int idx = item * kCellsPerWave + i;
+  if (i >= kCellsPerWave || idx >= ncells) return;
+  const int c = D.cell_list[idx];
+  if ((lane % kSparseLanes) == 0) {
+    const int tiles_x = (P.nx + kStripW - 1) / kStripW;
+    const int k = c / (int)P.pitch, j = c - k * (int)P.pitch;
+    mark_dirty(P, D, (k / kTileH) * tiles_x + j / kStripW);
+  }
   double s_mass = 0.0, s_vol = 0.0, s_area = 0.0, s_sol = 0.0;
-  if (i < kPart && j < P.nx && k < P.ny)
-    local_cell<NS, PH>(P, D, cidx(P, j, k), dt, ops & ~(int)OP_SUMS, e, imax, s_mass, s_vol,
-                      s_area, s_sol, (lane % kSparseLanes) == 0);
+  local_cell<NS, PH>(P, D, (size_t)c, dt, ops & ~(int)OP_SUMS, e, imax, s_mass, s_vol, s_area,
+                     s_sol, (lane % kSparseLanes) == 0);
 }
 
 // Learned Newton starting points for this step: for every computed cell of
@@ -1921,16 +1971,17 @@ __global__ void __launch_bounds__(256, (!VP && (PH == 0 || PH == 1)) ? 2 : 1)
     live = (D.scal[S_DONE] == 0.0);
     dt = D.scal[S_DT];
   }
-  const int nrows = (int)*ctr(D, C_ROWS, 0);
-  // PH 3: kSparseLanes items (row parts) per listed row
-  const int items = (PH >= 3) ? kSparseLanes * nrows : nrows;
+  // PH 3, 4: items of kCellsPerWave listed cells; else listed tile rows
+  const int nlist = (PH >= 3) ? (int)min(*ctr(D, C_CELLS, 0), (unsigned long long)P.nx * P.ny)
+                              : (int)*ctr(D, C_ROWS, 0);
+  const int items = (PH >= 3) ? (nlist + kCellsPerWave - 1) / kCellsPerWave : nlist;
   for (int it = blockIdx.x * 4 + w; live && it < items; it += gridDim.x * 4) {
-    if (lane == 0) mark_dirty(P, D, D.row_list[(PH >= 3) ? it / kSparseLanes : it] / kTileH);
-    if (PH >= 3)
-      local_part_row<NS, (PH >= 3 ? PH : 3)>(P, D, D.row_list[it / kSparseLanes],
-                                             it % kSparseLanes, dt, ops, cnt[w], imax);
-    else
+    if (PH >= 3) {
+      local_cells<NS, (PH >= 3 ? PH : 3)>(P, D, it, nlist, dt, ops, cnt[w], imax);
+    } else {
+      if (lane == 0) mark_dirty(P, D, D.row_list[it] / kTileH);
       local_row<NS, PH, VP>(P, D, D.row_list[it], dt, ops, cnt[w], imax);
+    }
   }
   for (int off = 32; off > 0; off >>= 1) imax = max(imax, __shfl_xor(imax, off, 64));
   if (lane == 0) itmax[w] = imax;

@@ -63,16 +63,19 @@ void launch_local(imexsw_ctx* c, int ops, bool fused) {
     start = OP_CLOSED;
   }
   if (fused) {
-    // row parts, four per workgroup
+    // listed cells, kCellsPerWave per wavefront, four wavefronts per workgroup
+    const long long cells = (long long)c->P.nx * c->P.ny;
+    const int groups = (int)std::min<long long>((cells + 4 * kCellsPerWave - 1) /
+                                                    (4 * kCellsPerWave), 1 << 30);
     static int res3 = 0, res4 = 0;
     if (!res3) res3 = resident_grid(k_local<NS, 3>, 256, c->n_cu, 1 << 30);
     if (!res4) res4 = resident_grid(k_local<NS, 4>, 256, c->n_cu, 1 << 30);
     if (start)
-      hipLaunchKernelGGL((k_local<NS, 4>), dim3(std::min(kSparseLanes * items, res4)), dim3(256),
-                         0, c->stream, c->P, c->D, (ops & ~(int)OP_SUMS) | start);
+      hipLaunchKernelGGL((k_local<NS, 4>), dim3(std::min(groups, res4)), dim3(256), 0,
+                         c->stream, c->P, c->D, (ops & ~(int)OP_SUMS) | start);
     else
-      hipLaunchKernelGGL((k_local<NS, 3>), dim3(std::min(kSparseLanes * items, res3)), dim3(256),
-                         0, c->stream, c->P, c->D, ops & ~(int)OP_SUMS);
+      hipLaunchKernelGGL((k_local<NS, 3>), dim3(std::min(groups, res3)), dim3(256), 0,
+                         c->stream, c->P, c->D, ops & ~(int)OP_SUMS);
     return;
   }
   static int res0 = 0, res1 = 0, res2 = 0;
