@@ -580,7 +580,13 @@ def main():
     h.close()
 
     steps = max(n, 1)
-    rl = roofline(p, kt, st, steps, traffic_workload=None if develop else args.workload)
+    # traffic: the committed counter summary of the same stretch (the volcano
+    # window at steps 500-600, or a whole run of a workload without one)
+    if develop:
+        tw = "volcano_window" if (args.workload == "volcano" and develop == 500) else None
+    else:
+        tw = args.workload
+    rl = roofline(p, kt, st, steps, traffic_workload=tw)
     active = st["cell_updates_active"]
     computed = st["cell_updates_computed"]
     cfg = {
