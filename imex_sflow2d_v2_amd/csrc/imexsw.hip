@@ -156,7 +156,10 @@ int run_stencil(imexsw_ctx* c, int compute_dt, bool rows = false, int stage = -1
   if (c->general) {
     const size_t sp = (size_t)c->nv * c->plane;
     D.D = c->G.Dx + (size_t)std::max(stage, 0) * sp;
-    if (stage >= 0) D.q = c->G.qrk + (size_t)stage * sp;
+    if (stage >= 0) {
+      D.q = c->G.qrk + (size_t)stage * sp;
+      D.qpp = nullptr;  // the stored physical variables are q's, not the stage's
+    }
   }
   {
     Timed tm(c, K_STENCIL);
@@ -711,6 +714,8 @@ int imexsw_create(const imexsw_params* params, imexsw_ctx** out) {
   rc |= dalloc(c, &D.tile_list, (size_t)c->n_tiles);
   rc |= dalloc(c, &D.row_list, (size_t)c->n_tiles * kTileH);
   rc |= dalloc(c, &D.cell_list, (size_t)P.nx * P.ny);
+  // the stored physical variables of the state (vertical profiles only)
+  if (c->P.vp) rc |= dalloc(c, &D.qpp, (c->nv + 2) * pl);
   c->strip_h = (P.ny > 1) ? kTileH : 1;
   c->row_unit = 1;
   {
@@ -856,6 +861,12 @@ int imexsw_upload_q(imexsw_ctx* c, const double* q) {
   if (upload_aos(c, q, c->nv, c->D.q)) return -1;
   hipLaunchKernelGGL(k_sflag<>, c->grid_tiles, dim3(256), 0, c->stream, c->P, c->D);
   if (check_launch(c, "k_sflag")) return -1;
+  if (c->D.qpp) {
+#define CALL(N) launch_qp_planes<N>(c)
+    NS_SWITCH(c->ns, CALL)
+#undef CALL
+    if (check_launch(c, "k_qp_planes")) return -1;
+  }
   // every tile's neighbourhood may have changed: the next mask scans all
   HIPCHK(c, hipMemsetAsync(c->D.tile_dirty, 1, (size_t)c->n_tiles, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));

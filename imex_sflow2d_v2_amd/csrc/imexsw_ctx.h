@@ -136,6 +136,8 @@ void launch_local(imexsw_ctx* c, int ops, bool fused);
 // qp of rows k0 .. k1-1 into dst in the host layout
 template <int NS>
 void launch_qp(imexsw_ctx* c, double* dst, int k0, int k1);
+template <int NS>
+void launch_qp_planes(imexsw_ctx* c);
 // point conversions of n packed states (k_convert): mode 0 qp_to_qc,
 // 1 qc_to_qp, 2 the profile flux coefficients
 template <int NS>

@@ -76,6 +76,8 @@ struct DevPtrs {
   int* tile_list;     // work tiles with a cell to compute (compacted)
   int* row_list;      // tile rows (tile * kTileH + row) with a cell to compute
   int* cell_list;     // cells (plane index) to compute, or null
+  double* qpp;        // NV+2 planes: the physical variables of q (with the
+                      // vertical profiles, whose conversion iterates), or null
   double* guess;      // 2 planes: learned Newton start for the momenta (NaN: none)
   float* harvest;     // harvest_cap pairs of kHarvestWidth floats, or null
   long long harvest_cap;
@@ -702,6 +704,23 @@ __device__ __forceinline__ void load_raw(const DevParams& P, const DevPtrs& D, i
 #pragma unroll
   for (int i = 0; i < NV; i++) q[i] = D.q[i * pl + c];
 }
+// With the vertical profiles the row's values are the stored physical
+// variables of q when there are any (NV+2 planes, kept by every kernel that
+// writes q; not for a stage state), so the strip march does not run the
+// profile mean-speed iteration again.
+template <int NS, bool VP>
+__device__ __forceinline__ void load_raw_v(const DevParams& P, const DevPtrs& D, int j, int k,
+                                           double* r) {
+  constexpr int NV = 4 + NS;
+  if (VP && D.qpp) {
+    const size_t c = cidx(P, min(max(j, 0), P.nx - 1), min(max(k, 0), P.ny - 1)), pl = plane(P);
+#pragma unroll
+    for (int i = 0; i < NV + 2; i++) r[i] = D.qpp[i * pl + c];
+  } else {
+    load_raw<NS>(P, D, j, k, r);
+  }
+}
+
 // the profile mean-speed iteration of one cell, out of line for the same
 // reason as make_side_vp
 template <int NS>
@@ -709,13 +728,20 @@ __device__ __noinline__ void phys_or_dry_vp(const DevParams& P, const double* q,
   phys_or_dry<NS, true>(P, q, qp);
 }
 
+// `stored`: q holds the stored physical variables (load_raw_v)
 template <int NS, bool VP = false>
 __device__ __forceinline__ void finish_row(const DevParams& P, int j, int k, const double* q,
-                                           double* qp) {
+                                           double* qp, bool stored = false) {
   constexpr int NV = 4 + NS;
   if (j >= 0 && j < P.nx && k >= 0 && k < P.ny) {
-    if (VP) phys_or_dry_vp<NS>(P, q, qp);
-    else phys_or_dry<NS>(P, q, qp);
+    if (VP && stored) {
+#pragma unroll
+      for (int i = 0; i < NV + 2; i++) qp[i] = q[i];
+    } else if (VP) {
+      phys_or_dry_vp<NS>(P, q, qp);
+    } else {
+      phys_or_dry<NS>(P, q, qp);
+    }
   } else {
 #pragma unroll
     for (int i = 0; i < NV + 2; i++) qp[i] = 0.0;
@@ -729,11 +755,16 @@ __device__ __forceinline__ void load_row(const DevParams& P, const DevPtrs& D,
   constexpr int NV = 4 + NS;
   if (j >= 0 && j < P.nx && k >= 0 && k < P.ny) {
     const size_t c = cidx(P, j, k), pl = plane(P);
-    double q[NV];
+    if (VP && D.qpp) {
 #pragma unroll
-    for (int i = 0; i < NV; i++) q[i] = D.q[i * pl + c];
-    if (VP) phys_or_dry_vp<NS>(P, q, qp);
-    else phys_or_dry<NS>(P, q, qp);
+      for (int i = 0; i < NV + 2; i++) qp[i] = D.qpp[i * pl + c];
+    } else {
+      double q[NV];
+#pragma unroll
+      for (int i = 0; i < NV; i++) q[i] = D.q[i * pl + c];
+      if (VP) phys_or_dry_vp<NS>(P, q, qp);
+      else phys_or_dry<NS>(P, q, qp);
+    }
   } else {
 #pragma unroll
     for (int i = 0; i < NV + 2; i++) qp[i] = 0.0;
@@ -1052,12 +1083,13 @@ __device__ __forceinline__ void stencil_strip(const DevParams& P, const DevPtrs&
   }
   const size_t pl = plane(P);
   double dtmin = P.max_dt;
+  const bool stored = VP && D.qpp;
 
   double qm[NV + 2], qc[NV + 2], qn[NV + 2];  // rows k-1, k, k+1
   double hiy[NV + 2];                          // N value of row k
   double HS[NV];                               // y flux at the S face of row k
   double amaxS = 0.0;
-  double rnext[NV];  // raw state of row k+2, loaded one iteration ahead
+  double rnext[NV + 2];  // raw values of row k+2, loaded one iteration ahead
   // source data: row k (S, v0, v1) and row k+1 (raw byte bn, vectors), the
   // slope correction of the S face of row k+1, and row k's per-cell data
   int bc = 0, bn = 0;
@@ -1069,22 +1101,22 @@ __device__ __forceinline__ void stencil_strip(const DevParams& P, const DevPtrs&
     int bm;
     double vm0, vm1, g0 = 1.0;
     {
-      double r0[NV], r1[NV], r2[NV], r3[NV];
-      load_raw<NS>(P, D, j, k0 - 2, r0);
-      load_raw<NS>(P, D, j, k0 - 1, r1);
-      load_raw<NS>(P, D, j, k0, r2);
-      load_raw<NS>(P, D, j, k0 + 1, r3);
-      load_raw<NS>(P, D, j, k0 + 2, rnext);
+      double r0[NV + 2], r1[NV + 2], r2[NV + 2], r3[NV + 2];
+      load_raw_v<NS, VP>(P, D, j, k0 - 2, r0);
+      load_raw_v<NS, VP>(P, D, j, k0 - 1, r1);
+      load_raw_v<NS, VP>(P, D, j, k0, r2);
+      load_raw_v<NS, VP>(P, D, j, k0 + 1, r3);
+      load_raw_v<NS, VP>(P, D, j, k0 + 2, rnext);
       load_src(P, D, j, k0 - 1, bm, vm0, vm1);
       load_src(P, D, j, k0, bc, vc0, vc1);
       load_src(P, D, j, k0 + 1, bn, vn0, vn1);
       g0 = load_gsy(P, D, j, k0);
       gn = load_gsy(P, D, j, k0 + 1);
       load_rowdata(P, D, j, k0, R);
-      finish_row<NS, VP>(P, j, k0 - 2, r0, qmm);
-      finish_row<NS, VP>(P, j, k0 - 1, r1, qm);
-      finish_row<NS, VP>(P, j, k0, r2, qc);
-      finish_row<NS, VP>(P, j, k0 + 1, r3, qn);
+      finish_row<NS, VP>(P, j, k0 - 2, r0, qmm, stored);
+      finish_row<NS, VP>(P, j, k0 - 1, r1, qm, stored);
+      finish_row<NS, VP>(P, j, k0, r2, qc, stored);
+      finish_row<NS, VP>(P, j, k0 + 1, r3, qn, stored);
     }
     if (injx) S = src_info(bc);
     Side<NS> Bs, Ts;
@@ -1139,9 +1171,9 @@ __device__ __forceinline__ void stencil_strip(const DevParams& P, const DevPtrs&
     double vnn0 = 0.0, vnn1 = 0.0, gnn = 1.0;
     if (HASY) {
       double qnn[NV + 2], hin[NV + 2];
-      finish_row<NS, VP>(P, j, k + 2, rnext, qnn);
+      finish_row<NS, VP>(P, j, k + 2, rnext, qnn, stored);
       if (more) {
-        load_raw<NS>(P, D, j, k + 3, rnext);  // next iteration's rows
+        load_raw_v<NS, VP>(P, D, j, k + 3, rnext);  // next iteration's rows
         load_src(P, D, j, k + 2, bnn, vnn0, vnn1);
         gnn = load_gsy(P, D, j, k + 2);
       }
@@ -1528,10 +1560,12 @@ enum LocalOps { OP_RK = 1, OP_EROSION = 2, OP_POST = 4, OP_SUMS = 8, OP_WARM = 1
 // negative solids, low temperature, solids above the total; qp receives the
 // physical variables.  Events count into e (C_NEWTON_CELLS-relative) when
 // `counts`.
+// *qp_cur (when given) tells whether qp is that of the final q (false when
+// the solids were rescaled or the cell zeroed after the conversion).
 template <int NS, bool VP = false>
 __device__ __forceinline__ void assembly_checks(const DevParams& P, const DevPtrs& D, size_t c,
                                                 double* q, double* qp, unsigned long long* e,
-                                                bool counts) {
+                                                bool counts, bool* qp_cur = nullptr) {
   constexpr int NV = 4 + NS;
   bool isn = false;
 #pragma unroll
@@ -1563,6 +1597,7 @@ __device__ __forceinline__ void assembly_checks(const DevParams& P, const DevPtr
     }
   }
   phys<NS, VP>(P, q, qp);
+  bool cur = true;
   if (counts && qp[3] < 273.0) atomicAdd(&e[C_LOWT - C_NEWTON_CELLS], 1ull);
   double ssol = 0.0;
 #pragma unroll
@@ -1571,6 +1606,7 @@ __device__ __forceinline__ void assembly_checks(const DevParams& P, const DevPtr
     if (((ssol - q[0]) / q[0] < 1.0e-10) || (q[0] < kEps)) {
 #pragma unroll
       for (int i = 0; i < NS; i++) q[4 + i] = q[4 + i] / ssol * q[0];
+      cur = false;
     } else {
       if (counts) atomicAdd(&e[C_SUMS - C_NEWTON_CELLS], 1ull);
     }
@@ -1578,7 +1614,9 @@ __device__ __forceinline__ void assembly_checks(const DevParams& P, const DevPtr
   if (D.nodata && D.nodata[c]) {
 #pragma unroll
     for (int i = 0; i < NV; i++) q[i] = 0.0;
+    cur = false;
   }
+  if (qp_cur) *qp_cur = cur;
 }
 
 // one cell of k_local; e (the wavefront's counters in LDS) accumulates
@@ -1646,6 +1684,13 @@ __device__ __forceinline__ void local_cell(const DevParams& P, const DevPtrs& D,
     mvel = D.modvel[c];
     if (D.vuln) vbits = D.vuln[c];
   }
+  // the stored physical variables of q (vertical profiles): the erosion part
+  // starts from them instead of converting q again
+  const bool qpl = VP && ero && !rk && D.qpp;
+  if (qpl) {
+#pragma unroll
+    for (int i = 0; i < NV + 2; i++) qp[i] = D.qpp[i * pl + c];
+  }
   // the loaded values are needed in registers here: no load is moved into
   // the branches below
 #pragma unroll
@@ -1676,7 +1721,16 @@ __device__ __forceinline__ void local_cell(const DevParams& P, const DevPtrs& D,
     pin(pm);
     pin(mvel);
   }
+  if (qpl) {
+#pragma unroll
+    for (int i = 0; i < NV + 2; i++) pin(qp[i]);
+  }
   if (!(mk & M_COMPUTE)) return;
+  // qp holds the physical variables of the current q (and pd their dynamic
+  // pressure when set by the last conversion): a later part that needs them
+  // does not convert again
+  bool qp_cur = qpl, pd_cur = false;
+  double pd = 0.0;
   if (rk) {
     double q0[NV];
 #pragma unroll
@@ -1714,10 +1768,10 @@ __device__ __forceinline__ void local_cell(const DevParams& P, const DevPtrs& D,
       atomicAdd(&e[C_HIST0 - C_NEWTON_CELLS + b], 1ull);
       imax = max(imax, it);
     }
-    assembly_checks<NS, VP>(P, D, c, q, qp, e, counts);
+    assembly_checks<NS, VP>(P, D, c, q, qp, e, counts, &qp_cur);
   }
   if (ero) {
-    phys_or_dry<NS, VP>(P, q, qp);
+    if (!qp_cur) phys_or_dry<NS, VP>(P, q, qp);
     double er[NSX], dep[NSX], eq[NV], topo;
     mass_exchange<NS, (PH >= 3) ? kSparseLanes : 1, VP>(P, qp, bx, by, erod, dt, er, dep, eq,
                                                          topo);
@@ -1747,12 +1801,22 @@ __device__ __forceinline__ void local_cell(const DevParams& P, const DevPtrs& D,
     }
     double rg = 0.0;
     if (q[0] > 0.0) {
-      phys<NS, VP>(P, q, qp);
+      pd = phys<NS, VP>(P, q, qp);
       rg = mixt<NS, false>(P, qp).rg;
     }
     if (rg <= 0.0) {
 #pragma unroll
       for (int i = 0; i < NV; i++) q[i] = 0.0;
+#pragma unroll
+      for (int i = 0; i < NV + 2; i++) qp[i] = 0.0;
+      qp[3] = P.T_ambient;
+      pd = 0.0;
+    }
+    qp_cur = true;
+    pd_cur = true;
+    if (VP && D.qpp) {
+#pragma unroll
+      for (int i = 0; i < NV + 2; i++) D.qpp[i * pl + c] = qp[i];
     }
   }
   if (ops & (OP_RK | OP_EROSION)) {
@@ -1765,7 +1829,7 @@ __device__ __forceinline__ void local_cell(const DevParams& P, const DevPtrs& D,
     D.sflag[c] = state_flags(q[0], zero);
   }
   if (post) {
-    double pd = phys_or_dry<NS, VP>(P, q, qp);
+    if (!(qp_cur && pd_cur)) pd = phys_or_dry<NS, VP>(P, q, qp);
     if (q[0] > 0.0) {
       D.hmax[c] = fmax(hm, qp[0]);
       const double mv = sqrt(qp[NV] * qp[NV] + qp[NV + 1] * qp[NV + 1]);
@@ -2190,6 +2254,23 @@ __global__ void k_qp_out(DevParams P, const double* __restrict__ q, double* __re
   phys_or_dry<NS, VP>(P, qq, qp);
 #pragma unroll
   for (int i = 0; i < NV + 2; i++) dst[s + i] = qp[i];
+}
+
+// The stored physical variables of every cell (D.qpp), after the state was
+// set from the host; the kernels that change the state keep them after that.
+template <int NS, bool VP>
+__global__ void k_qp_planes(DevParams P, DevPtrs D) {
+  constexpr int NV = 4 + NS;
+  const int j = blockIdx.x * kTileX + (threadIdx.x & 63);
+  const int k = blockIdx.y * kTileY + (threadIdx.x >> 6);
+  if (j >= P.nx || k >= P.ny) return;
+  const size_t pl = plane(P), c = cidx(P, j, k);
+  double q[NV], qp[NV + 2];
+#pragma unroll
+  for (int i = 0; i < NV; i++) q[i] = D.q[i * pl + c];
+  phys_or_dry<NS, VP>(P, q, qp);
+#pragma unroll
+  for (int i = 0; i < NV + 2; i++) D.qpp[i * pl + c] = qp[i];
 }
 
 // H_crit_rel (src/inpout_2d.f90:3784-3791): with a = vonK / sqrt(f) + 1,

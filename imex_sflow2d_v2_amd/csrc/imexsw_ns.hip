@@ -112,6 +112,15 @@ void launch_qp(imexsw_ctx* c, double* dst, int k0, int k1) {
     hipLaunchKernelGGL((k_qp_out<NS>), grid, dim3(256), 0, c->stream, c->P, c->D.q, dst, k0, k1);
 }
 template <int NS>
+void launch_qp_planes(imexsw_ctx* c) {
+  if (c->P.vp)
+    hipLaunchKernelGGL((k_qp_planes<NS, true>), c->grid_tiles, dim3(256), 0, c->stream, c->P,
+                       c->D);
+  else
+    hipLaunchKernelGGL((k_qp_planes<NS, false>), c->grid_tiles, dim3(256), 0, c->stream, c->P,
+                       c->D);
+}
+template <int NS>
 void launch_convert(const DevParams& P, hipStream_t s, int n_cu, long long n, const double* in,
                     const double* bx, const double* by, double* out, int mode) {
   const int grid = (int)std::min<long long>((n + 255) / 256, (long long)n_cu * 8);
@@ -177,6 +186,7 @@ template void launch_stage<IMEXSW_NS>(imexsw_ctx*, int, int);
 template void launch_assemble<IMEXSW_NS>(imexsw_ctx*, int);
 template void launch_local<IMEXSW_NS>(imexsw_ctx*, int, bool);
 template void launch_qp<IMEXSW_NS>(imexsw_ctx*, double*, int, int);
+template void launch_qp_planes<IMEXSW_NS>(imexsw_ctx*);
 template void launch_convert<IMEXSW_NS>(const DevParams&, hipStream_t, int, long long,
                                         const double*, const double*, const double*, double*,
                                         int);

@@ -413,7 +413,8 @@ __global__ void __launch_bounds__(256) kg_assemble(DevParams P, DevPtrs D, RkPtr
       for (int r = 0; r < T.nrk; r++) s2 = s2 + G.Im[r * sp + i * pl + c] * T.omega[r];
       q[i] = T.stiff ? G.qrk[(T.nrk - 1) * sp + i * pl + c] : D.q[i * pl + c] - dt * (s1 - s2);
     }
-    assembly_checks<NS, VP>(P, D, c, q, qp, cnt[w], true);
+    bool cur = true;
+    assembly_checks<NS, VP>(P, D, c, q, qp, cnt[w], true, &cur);
     bool zero = true;
 #pragma unroll
     for (int i = 0; i < NV; i++) {
@@ -421,6 +422,12 @@ __global__ void __launch_bounds__(256) kg_assemble(DevParams P, DevPtrs D, RkPtr
       zero = zero && (q[i] == 0.0);
     }
     D.sflag[c] = state_flags(q[0], zero);
+    if (VP && D.qpp) {
+      // the physical variables of the assembled state, for the erosion pass
+      if (!cur) phys_or_dry<NS, VP>(P, q, qp);
+#pragma unroll
+      for (int i = 0; i < NV + 2; i++) D.qpp[i * pl + c] = qp[i];
+    }
   }
   __syncthreads();
   if (threadIdx.x == 0) {

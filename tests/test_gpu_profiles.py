@@ -164,6 +164,39 @@ def test_tableau_path_strips_match_oracle(kind):
         dev.close()
 
 
+def test_profile_entry_points_match_oracle():
+    """The solver_2d entry points one at a time on the profile channel
+    (check_solve, timestep, imex_RK_solver, update_erosion_deposition_cell,
+    post-step), with the initial state uploaded again half-way: the stored physical
+    variables that the profile stencil and erosion pass read follow every
+    kernel that changes the state and every upload."""
+    d = decks.profile_channel(convert=_conv_oracle)
+    print("case entry points", flush=True)
+    orc = Oracle(d.params, fresh=True).load_deck(d)
+    dev = Solver(d.params).load_deck(d)
+    try:
+        t = 0.0
+        for it in range(6):
+            if it == 3:
+                # back to the initial state: stale stored variables would
+                # still hold the state of step 3
+                dev.upload_q(d.q)
+                orc.set_q(d.q)
+            assert orc.check_solve(False) == dev.check_solve(False, t=t)
+            dt_o, dt_d = orc.timestep(), dev.timestep()
+            assert dt_o == dt_d, (dt_o, dt_d)
+            orc.imex_rk(t, dt_o)
+            dev.imex_RK_solver(dt_d, t=t)
+            orc.erosion_deposition(dt_o)
+            dev.update_erosion_deposition_cell(dt_d)
+            s_o, s_d = orc.post_step(), dev.post_step()
+            assert np.allclose(s_o, s_d, rtol=1e-12, atol=0.0), (s_o, s_d)
+            t += dt_o
+        _check(orc, dev, d.params)
+    finally:
+        dev.close()
+
+
 def test_profile_channel_1d_matches_oracle():
     d = decks.profile_channel(nx=200, ny=1, convert=_conv_oracle)
     orc, dev = _run_pair(d, 60)
