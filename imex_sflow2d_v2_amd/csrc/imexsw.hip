@@ -716,6 +716,8 @@ int imexsw_create(const imexsw_params* params, imexsw_ctx** out) {
   rc |= dalloc(c, &D.cell_list, (size_t)P.nx * P.ny);
   // the stored physical variables of the state (vertical profiles only)
   if (c->P.vp) rc |= dalloc(c, &D.qpp, (c->nv + 2) * pl);
+  // the face sides of the 2-D profile stencil
+  if (c->P.vp && P.ny > 1) rc |= dalloc(c, &D.sides, 4 * (2 * c->nv + 3) * pl);
   c->strip_h = (P.ny > 1) ? kTileH : 1;
   c->row_unit = 1;
   {

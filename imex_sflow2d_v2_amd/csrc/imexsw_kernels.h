@@ -78,6 +78,8 @@ struct DevPtrs {
   int* cell_list;     // cells (plane index) to compute, or null
   double* qpp;        // NV+2 planes: the physical variables of q (with the
                       // vertical profiles, whose conversion iterates), or null
+  double* sides;      // 4 x (2 NV + 3) planes: the W, E, S, N face sides of
+                      // each cell (vertical profiles, 2-D), or null
   double* guess;      // 2 planes: learned Newton start for the momenta (NaN: none)
   float* harvest;     // harvest_cap pairs of kHarvestWidth floats, or null
   long long harvest_cap;
@@ -595,8 +597,9 @@ struct Side {
 // side's h0 and settling speeds are shared by its momentum and its flux
 // coefficients (the carrier density of both is mixt_var's).
 template <int NS, int DIR>
-__device__ __noinline__ void make_side_vp(const DevParams& P, const double* qp, double gc,
-                                          double bx, double by, Side<NS>* s) {
+__device__ __forceinline__ void make_side_vp_body(const DevParams& P, const double* qp,
+                                                  double gc, double bx, double by,
+                                                  Side<NS>* s) {
   if (qp[0] > kEps) {
     const VpPre<NS> pre = vp_pre<NS>(P, qp[0], mixt<NS, false>(P, qp).rho_c);
     qp2qc<NS, true>(P, qp, s->qc, &pre);
@@ -607,6 +610,11 @@ __device__ __noinline__ void make_side_vp(const DevParams& P, const double* qp, 
   }
   speeds<NS, DIR>(P, qp, gc, s->vmin, s->vmax);
   s->zv = (DIR == 1) ? qp[1] : s->qc[2];
+}
+template <int NS, int DIR>
+__device__ __noinline__ void make_side_vp(const DevParams& P, const double* qp, double gc,
+                                          double bx, double by, Side<NS>* s) {
+  make_side_vp_body<NS, DIR>(P, qp, gc, bx, by, s);
 }
 
 // (VP: bx, by are the slope of the cell the state belongs to, read by the
@@ -1233,6 +1241,275 @@ __device__ __forceinline__ void stencil_strip(const DevParams& P, const DevPtrs&
     }
   }
   if (compute_dt) dt_reduce(D, P, dtmin);
+}
+
+// ---------------------------------------------------------------------------
+// The 2-D stencil pass with the vertical profiles, in two kernels.  Every
+// face side the strip march forms is one of the four sides of a cell (its W
+// and E values of the x reconstruction, its S and N values of the y
+// reconstruction, with the slope correction of that face and the cell's own
+// bed slope): k_vp_sides forms them for every cell of the listed tiles and
+// a one-cell ring around them, with the profile closures inlined and no
+// march around them, and the strip march (stencil_strip_vp) reads them.
+// The ring overrides pick the other cell's side of the face, which is the
+// side formed from the same state (the slope the closures read differs only
+// with slope correction, a combination refused with a radial source).
+// ---------------------------------------------------------------------------
+template <int NS>
+__device__ __forceinline__ void side_put(const DevParams& P, const DevPtrs& D, size_t c, int s,
+                                         const Side<NS>& v) {
+  constexpr int NV = 4 + NS, NSD = 2 * NV + 3;
+  const size_t pl = plane(P);
+  double* o = D.sides + (size_t)s * NSD * pl + c;
+#pragma unroll
+  for (int i = 0; i < NV; i++) {
+    o[i * pl] = v.f[i];
+    o[(NV + i) * pl] = v.qc[i];
+  }
+  o[(2 * NV) * pl] = v.vmin;
+  o[(2 * NV + 1) * pl] = v.vmax;
+  o[(2 * NV + 2) * pl] = v.zv;
+}
+template <int NS>
+__device__ __forceinline__ void side_get(const DevParams& P, const DevPtrs& D, int j, int k,
+                                         int s, Side<NS>& v) {
+  constexpr int NV = 4 + NS, NSD = 2 * NV + 3;
+  const size_t pl = plane(P);
+  const size_t c = cidx(P, min(max(j, 0), P.nx - 1), min(max(k, 0), P.ny - 1));
+  const double* o = D.sides + (size_t)s * NSD * pl + c;
+#pragma unroll
+  for (int i = 0; i < NV; i++) {
+    v.f[i] = o[i * pl];
+    v.qc[i] = o[(NV + i) * pl];
+  }
+  v.vmin = o[(2 * NV) * pl];
+  v.vmax = o[(2 * NV + 1) * pl];
+  v.zv = o[(2 * NV + 2) * pl];
+}
+enum SideSlot { SD_W = 0, SD_E = 1, SD_S = 2, SD_N = 3 };
+
+// physical state of cell (j, k) from the stored planes, dry outside the grid
+template <int NS>
+__device__ __forceinline__ void qpp_get(const DevParams& P, const DevPtrs& D, int j, int k,
+                                        double* qp) {
+  constexpr int NV = 4 + NS;
+  if (j >= 0 && j < P.nx && k >= 0 && k < P.ny) {
+    const size_t c = cidx(P, j, k), pl = plane(P);
+#pragma unroll
+    for (int i = 0; i < NV + 2; i++) qp[i] = D.qpp[i * pl + c];
+  } else {
+#pragma unroll
+    for (int i = 0; i < NV + 2; i++) qp[i] = 0.0;
+    qp[3] = P.T_ambient;
+  }
+}
+
+// the four sides of cell (j, k), with the operands the strip march gives
+// make_side (zero sides for a source-interior cell, whose own sides no face
+// uses)
+template <int NS>
+__device__ __forceinline__ void vp_cell_sides(const DevParams& P, const DevPtrs& D,
+                                              const SrcTime& T, int j, int k) {
+  constexpr int NV = 4 + NS;
+  const size_t c = cidx(P, j, k);
+  double qk[NV + 2], qkm[NV + 2], qkp[NV + 2], qm[NV + 2], qn[NV + 2];
+  qpp_get<NS>(P, D, j, k, qk);
+  qpp_get<NS>(P, D, j - 1, k, qkm);
+  qpp_get<NS>(P, D, j + 1, k, qkp);
+  qpp_get<NS>(P, D, j, k - 1, qm);
+  qpp_get<NS>(P, D, j, k + 1, qn);
+  int bc;
+  double v0, v1;
+  load_src(P, D, j, k, bc, v0, v1);
+  RowData R;
+  load_rowdata(P, D, j, k, R);
+  const double gS = load_gsy(P, D, j, k), gN = load_gsy(P, D, j, k + 1);
+  const SrcInfo S = src_info(bc);
+  Side<NS> a, b;
+  if (S.is_src) {
+    zero_side<NS>(a);
+#pragma unroll
+    for (int s = 0; s < 4; s++) side_put<NS>(P, D, c, s, a);
+    return;
+  }
+  // both reconstructions first: the neighbours' states are then dead while
+  // the closures run
+  double sb[NV + 2], lox[NV + 2], hix[NV + 2], loy[NV + 2], hiy[NV + 2];
+  ring_state<NS>(P, S, T, v0, v1, sb);
+  const bool typo = P.radial && S.src2 && k > 0 && k < P.ny - 1 && !S.sS && S.sN;
+  recon<NS, 1>(P, qk, qkm, qkp, j, P.nx, k, S, sb, typo, lox, hix);
+  recon<NS, 2>(P, qk, qm, qn, k, P.ny, j, S, sb, false, loy, hiy);
+  make_side_vp_body<NS, 1>(P, lox, R.gW, R.bx, R.by, &a);
+  side_put<NS>(P, D, c, SD_W, a);
+  make_side_vp_body<NS, 1>(P, hix, R.gE, R.bx, R.by, &b);
+  side_put<NS>(P, D, c, SD_E, b);
+  make_side_vp_body<NS, 2>(P, loy, gS, R.bx, R.by, &a);
+  side_put<NS>(P, D, c, SD_S, a);
+  make_side_vp_body<NS, 2>(P, hiy, gN, R.bx, R.by, &b);
+  side_put<NS>(P, D, c, SD_N, b);
+}
+
+// one workgroup per listed tile in turn: the tile's cells and the ring of
+// cells around it
+template <int NS>
+__global__ void __launch_bounds__(256, 2) k_vp_sides(DevParams P, DevPtrs D) {
+  double done = D.scal[S_DONE], t = D.scal[S_T];
+  int ntiles = (int)*ctr(D, C_TILES, 0);
+  pin(done);
+  pin(t);
+  pin(ntiles);
+  if (done != 0.0) return;
+  const SrcTime T = P.radial ? src_time(P, t) : SrcTime{};
+  const int tiles_x = (P.nx + kStripW - 1) / kStripW;
+  constexpr int W = kStripW + 2, H = kTileH + 2;
+  for (int item = blockIdx.x; item < ntiles; item += gridDim.x) {
+    const int tile = D.tile_list[item];
+    const int x0 = (tile % tiles_x) * kStripW - 1, y0 = (tile / tiles_x) * kTileH - 1;
+    for (int e = threadIdx.x; e < W * H; e += blockDim.x) {
+      const int j = x0 + e % W, k = y0 + e / W;
+      if (j >= 0 && j < P.nx && k >= 0 && k < P.ny) vp_cell_sides<NS>(P, D, T, j, k);
+    }
+  }
+}
+
+// the sides of the y face between rows k-1 (B) and k (T) from the stored
+// sides: Bn, the N side of the cell below, and Ts, the S side of the cell
+// above, with the ring overrides of yface_bottom / yface_top
+template <int NS>
+__device__ __forceinline__ void vp_yface(const DevParams& P, const Side<NS>& Bn,
+                                         const Side<NS>& Ts_, const SrcInfo& Sb,
+                                         const SrcInfo& St, int k, Side<NS>& Bs, Side<NS>& Ts) {
+  if (Sb.is_src) {
+    if (P.radial && St.src2 && St.sS && k < P.ny - 1) Bs = Ts_;
+    else zero_side<NS>(Bs);
+  } else {
+    Bs = Bn;
+  }
+  if (St.is_src) {
+    if (P.radial && Sb.src2 && !Sb.sS && Sb.sN && k - 1 > 0) Ts = Bn;
+    else zero_side<NS>(Ts);
+  } else {
+    Ts = Ts_;
+  }
+}
+
+// One strip of the profile stencil from the stored sides: lane l owns
+// column j = 60 * tile_x - 2 + l, rows k0 .. k1-1.
+template <int NS>
+__device__ __forceinline__ void stencil_strip_vp(const DevParams& P, const DevPtrs& D, int j,
+                                                 int k0, int k1, bool skip_empty,
+                                                 int compute_dt) {
+  constexpr int NV = 4 + NS;
+  const int lane = threadIdx.x;
+  const bool injx = (j >= 0 && j < P.nx);
+  const bool own = (lane >= 2 && lane < 2 + kStripW && injx);
+  if (skip_empty) {
+    int any = 0;
+    if (own)
+      for (int k = k0; k < k1; k++) any |= D.mask[cidx(P, j, k)] & M_COMPUTE;
+    if (!__any(any)) return;
+  }
+  double dtmin = P.max_dt;
+  double HS[NV], amaxS = 0.0;
+  int bc = 0;
+  double v0, v1;
+  load_src(P, D, j, k0, bc, v0, v1);
+  SrcInfo S = injx ? src_info(bc) : SrcInfo{};
+  {
+    Side<NS> Bs, Ts;
+    if (!injx) {
+      zero_side<NS>(Bs);
+      zero_side<NS>(Ts);
+    } else if (k0 == 0) {
+      // boundary face: both sides carry the S side of row 0
+      side_get<NS>(P, D, j, 0, SD_S, Ts);
+      Bs = Ts;
+    } else {
+      int bm;
+      double w0, w1;
+      load_src(P, D, j, k0 - 1, bm, w0, w1);
+      Side<NS> Bn, Tn;
+      side_get<NS>(P, D, j, k0 - 1, SD_N, Bn);
+      side_get<NS>(P, D, j, k0, SD_S, Tn);
+      vp_yface<NS>(P, Bn, Tn, src_info(bm), S, k0, Bs, Ts);
+    }
+    amaxS = face_flux<NS, 2>(Bs, Ts, HS);
+  }
+  for (int k = k0; k < k1; k++) {
+    const size_t c = injx ? cidx(P, j, k) : 0;
+    RowData R;
+    load_rowdata(P, D, j, k, R);
+    double qk[NV + 2];
+    qpp_get<NS>(P, D, j, k, qk);
+    int bn = 0;
+    double n0, n1;
+    load_src(P, D, j, k + 1, bn, n0, n1);
+    const SrcInfo Sn = (injx && k + 1 < P.ny) ? src_info(bn) : SrcInfo{};
+    // N face of row k
+    double HN[NV], amaxN;
+    {
+      Side<NS> Bs, Ts;
+      if (!injx) {
+        zero_side<NS>(Bs);
+        zero_side<NS>(Ts);
+      } else if (k == P.ny - 1) {
+        side_get<NS>(P, D, j, k, SD_N, Bs);
+        Ts = Bs;
+      } else {
+        Side<NS> Bn, Tn;
+        side_get<NS>(P, D, j, k, SD_N, Bn);
+        side_get<NS>(P, D, j, k + 1, SD_S, Tn);
+        vp_yface<NS>(P, Bn, Tn, S, Sn, k + 1, Bs, Ts);
+      }
+      amaxN = face_flux<NS, 2>(Bs, Ts, HN);
+    }
+    // x faces of row k
+    Side<NS> Wself, Eself, Eleft;
+    if (injx) {
+      side_get<NS>(P, D, j, k, SD_W, Wself);
+      side_get<NS>(P, D, j, k, SD_E, Eself);
+    } else {
+      zero_side<NS>(Wself);
+      zero_side<NS>(Eself);
+    }
+    shfl_side<NS>(Eself, Eleft, (lane + 63) & 63);
+    double HW[NV], HE[NV], amaxW, amaxE;
+    x_face_fluxes<NS>(P, S, j, Wself, Eself, Eleft, HW, HE, amaxW, amaxE);
+    if (own && (R.m & M_COMPUTE))
+      cell_update<NS, true>(P, D, c, qk, R, HW, HE, HS, HN, amaxW, amaxE, amaxS, amaxN,
+                            compute_dt, dtmin);
+#pragma unroll
+    for (int i = 0; i < NV; i++) HS[i] = HN[i];
+    amaxS = amaxN;
+    S = Sn;
+  }
+  if (compute_dt) dt_reduce(D, P, dtmin);
+}
+
+// the strip items of k_stencil over the listed tiles, with the stored sides
+template <int NS>
+__global__ void __launch_bounds__(64) k_stencil_vp(DevParams P, DevPtrs D, int unit,
+                                                   int max_strip, int target_strips,
+                                                   int compute_dt) {
+  const int lane = threadIdx.x;
+  double done = D.scal[S_DONE];
+  unsigned long long w = *ctr(D, C_COMPUTE_CELLS, lane);
+  int ntiles = (int)*ctr(D, C_TILES, 0);
+  pin(done);
+  pin(w);
+  pin(ntiles);
+  if (done != 0.0) return;
+  const int strip_h = strip_height(P, w, unit, max_strip, target_strips);
+  const int tiles_x = (P.nx + kStripW - 1) / kStripW;
+  const int per = kTileH / strip_h;
+  for (int item = blockIdx.x; item < ntiles * per; item += gridDim.x) {
+    const int tile = D.tile_list[item / per];
+    const int tx = tile % tiles_x, tyy = tile / tiles_x;
+    const int k0 = tyy * kTileH + (item % per) * strip_h;
+    if (k0 >= P.ny) continue;
+    const int k1 = min(k0 + strip_h, P.ny);
+    stencil_strip_vp<NS>(P, D, tx * kStripW - 2 + lane, k0, k1, true, compute_dt);
+  }
 }
 
 // ---------------------------------------------------------------------------

@@ -20,8 +20,17 @@ void launch_stencil(imexsw_ctx* c, const DevPtrs& D, int compute_dt, bool rows) 
                        dim3(384), 0, c->stream, c->P, D, compute_dt);
   } else if (c->P.vp) {
     // the profile closures (general path only)
-    static int res2 = 0, res1 = 0;
-    if (c->P.ny > 1) {
+    static int res2 = 0, res1 = 0, res_s = 0, res_v = 0;
+    if (c->P.ny > 1 && D.sides && D.qpp) {
+      // the state's stored physical variables: its face sides first
+      if (!res_s) res_s = resident_grid(k_vp_sides<NS>, 256, c->n_cu, 1 << 30);
+      if (!res_v) res_v = resident_grid(k_stencil_vp<NS>, 64, c->n_cu, 1 << 30);
+      hipLaunchKernelGGL((k_vp_sides<NS>), dim3(std::min(c->n_tiles, res_s)), dim3(256), 0,
+                         c->stream, c->P, D);
+      hipLaunchKernelGGL((k_stencil_vp<NS>), dim3(std::min(items, res_v)), dim3(64), 0,
+                         c->stream, c->P, D, c->row_unit, c->strip_h, c->target_strips,
+                         compute_dt);
+    } else if (c->P.ny > 1) {
       if (!res2) res2 = resident_grid(k_stencil<NS, true, true>, 64, c->n_cu, 1 << 30);
       hipLaunchKernelGGL((k_stencil<NS, true, true>), dim3(std::min(items, res2)), dim3(64), 0,
                          c->stream, c->P, D, c->row_unit, c->strip_h, c->target_strips,
