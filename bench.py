@@ -29,7 +29,10 @@ measured with HIP events on the library's stream in a second pass (per-kernel
 events need direct launches, which slow the step, so that pass's wall time
 is reported separately).  The fully wet 2048^2 grid runs in the same
 invocation and its roofline (with rocprofv3 FETCH_SIZE/WRITE_SIZE traffic
-from the committed summaries) is in ``config.wet_2048``.  ``cpu_baseline``
+from the committed summaries) is in ``config.wet_2048``; the batched
+256-member ensemble (members/s) and the 2048^2 vertical-profile channel
+(ms per step) are in ``config.ensemble_256`` and ``config.profiles_2048``.
+``cpu_baseline``
 times the C oracle (the reference algorithm restated, OpenMP over cells,
 reference interface semantics) on the host cores, started from the device's
 state at the start of the window.
@@ -272,6 +275,57 @@ def bench_ensemble(args):
     }
 
 
+def extra_ensemble(steps, members=256, warmup=6):
+    """The batched ensemble (members of 128^2..1024^2, `steps` steps each)
+    for the volcano line's config: members/s of the batched launches after
+    the strip-height autotuner, without the sequential comparison."""
+    from imex_sflow2d_v2_amd import decks
+    from imex_sflow2d_v2_amd.solver import Ensemble
+    ds = decks.ensemble_members(count=members)
+    params = [d.params for d in ds]
+    cells = sum(p.nx * p.ny for p in params)
+    dt0 = ds[0].dt0
+    warm = Ensemble(params).load(ds)
+    best = warm.autotune(max(1, warmup // 3), dt0)
+    warm.close()
+    ens = Ensemble(params).load(ds)
+    ens.set_tuning(best, 25)
+    ens.advance(25, dt0)
+    ens.load(ds)
+    for m in ens.members:
+        m.sync()
+    t0 = time.perf_counter()
+    n = ens.advance(steps, dt0)
+    for m in ens.members:
+        m.sync()
+    el = time.perf_counter() - t0
+    ens.close()
+    return {"members": len(ds), "steps_each": n, "members_per_s": len(ds) / el,
+            "member_steps_per_s": len(ds) * n / el,
+            "grid_cell_updates_per_s": cells * n / el, "total_cells": cells,
+            "strip_rows_autotuned": best}
+
+
+def extra_profiles(steps):
+    """The vertical-profile channel at 2048^2 (rheology model 8, two
+    classes) for the volcano line's config: ms per step through the step
+    graphs after a warm-up that captures them."""
+    from imex_sflow2d_v2_amd.solver import Solver
+    d = make_deck("profiles")
+    p = d.params
+    h = Solver(p).load_deck(d)
+    h.advance(20, dt0=d.dt0)
+    h.load_deck(d)
+    h.sync()
+    t0 = time.perf_counter()
+    n = h.advance(steps, dt0=d.dt0)
+    h.sync()
+    el = time.perf_counter() - t0
+    h.close()
+    return {"steps": n, "ms_per_step": el / max(n, 1) * 1e3,
+            "cell_updates_per_s": p.nx * p.ny * n / el}
+
+
 def bench_warmstart(args):
     """Newton starting points on the 2048^2 volcano (and the fully wet grid):
     the plain predictor q_si (the reference's), the learned warm start (an
@@ -464,6 +518,10 @@ def main():
     ap.add_argument("--wet-steps", type=int, default=10,
                     help="steps of the fully wet 2048^2 grid run for its roofline")
     ap.add_argument("--no-wet", action="store_true")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="volcano workload: skip the ensemble and profile-path lines of config")
+    ap.add_argument("--extra-ens-steps", type=int, default=100,
+                    help="steps per member of the ensemble line in the volcano config")
     ap.add_argument("--workload", default="volcano",
                     choices=["volcano", "wet", "profiles", "ensemble", "warmstart"])
     ap.add_argument("--ens-steps", type=int, default=500,
@@ -618,6 +676,10 @@ def main():
                      % (develop, develop + steps, t_win))
     if args.workload == "volcano" and not args.no_wet:
         cfg["wet_2048"] = bench_wet_roofline(args.wet_steps)
+    if args.workload == "volcano" and not args.no_extra:
+        # the next workloads of the same path, measured in the same invocation
+        cfg["ensemble_256"] = extra_ensemble(args.extra_ens_steps)
+        cfg["profiles_2048"] = extra_profiles(20)
     out = {
         "metric": "FP64 cell-updates/s at 2048^2",
         "value": cells * steps / el,

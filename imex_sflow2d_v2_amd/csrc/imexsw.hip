@@ -1645,6 +1645,13 @@ int imexsw_ensemble_advance(imexsw_ensemble* e, int32_t nsteps, double dt0, doub
                             double t_end, int32_t* steps_done) {
   if (!e || nsteps < 0) return efail(e, "bad argument");
   if (seed_members(e, dt0, t, t_end)) return -1;
+  // member 0's step count before the call (steps_done is this call's)
+  unsigned long long before = 0;
+  if (!e->members.empty()) {
+    EHIPCHK(e, hipMemcpyAsync(&before, e->members[0]->D.counters + C_STEPS * kSlots,
+                              sizeof before, hipMemcpyDeviceToHost, e->stream));
+    EHIPCHK(e, hipStreamSynchronize(e->stream));
+  }
   hipEvent_t a, b;
   EHIPCHK(e, hipEventCreate(&a));
   EHIPCHK(e, hipEventCreate(&b));
@@ -1666,7 +1673,7 @@ int imexsw_ensemble_advance(imexsw_ensemble* e, int32_t nsteps, double dt0, doub
     if (m == 0) {
       unsigned long long v = 0;
       EHIPCHK(e, hipMemcpy(&v, c->D.counters + C_STEPS * kSlots, sizeof v, hipMemcpyDeviceToHost));
-      steps0 = (int64_t)v;
+      steps0 = (int64_t)(v - before);
     }
   }
   if (steps_done) *steps_done = (int32_t)steps0;

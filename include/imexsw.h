@@ -423,7 +423,7 @@ int imexsw_ensemble_set_tuning(imexsw_ensemble *ens, int strip_rows, int graph_s
  * of src/IMEX_SfloW2D.f90:357-381 per member, dt history initialised from
  * dt0 on a member's first run).  t[n_members] holds each member's time in
  * and out (NULL: keep the members' device times); members stop at t_end.
- * steps_done = steps taken by member 0. */
+ * steps_done = steps member 0 took in this call. */
 int imexsw_ensemble_advance(imexsw_ensemble *ens, int32_t nsteps, double dt0, double *t,
                             double t_end, int32_t *steps_done);
 

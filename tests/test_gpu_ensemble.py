@@ -40,7 +40,8 @@ def test_ensemble_autotune_then_advance():
     ens = Ensemble([d.params for d in ds]).load(ds)
     best = ens.autotune(3, dt0=ds[0].dt0)
     assert best in (4, 8, 16)
-    ens.advance(11, dt0=ds[0].dt0)
+    # the count is this call's steps, not the run's (the trials took 9)
+    assert ens.advance(11, dt0=ds[0].dt0) == 11
     for i in (0, 3):
         alone = Solver(ds[i].params).load_deck(ds[i])
         alone.advance(20, dt0=ds[i].dt0)
