@@ -237,15 +237,21 @@ __device__ __forceinline__ unsigned mask_tile(const DevParams& P, const DevPtrs&
   const bool radial = P.radial && D.sinfo;
   unsigned S = 0, W = 0, I = 0, rowin = 0;
   // unconditional loads from clamped (always valid) addresses, selected
-  // afterwards: straight-line code lets all loads be in flight at once
+  // afterwards: straight-line code lets all loads be in flight at once (a
+  // load under a condition is waited for where the branches join, which
+  // would serialise the rows); without a source the source bytes are read
+  // from the state flags and not used
   const int jc = min(max(j, 0), P.nx - 1);
+  const uint8_t* sinfo = radial ? D.sinfo : D.sflag;
   int fv[LH], sv[LH];
 #pragma unroll
   for (int r = 0; r < LH; r++) {
     const int kc = min(max(k0 - 2 + r, 0), P.ny - 1);
     fv[r] = D.sflag[cidx(P, jc, kc)];
-    sv[r] = radial ? D.sinfo[cidx(P, jc, kc)] : 0;
+    sv[r] = sinfo[cidx(P, jc, kc)];
   }
+#pragma unroll
+  for (int r = 0; r < LH; r++) sv[r] = radial ? sv[r] : 0;
 #pragma unroll
   for (int r = 0; r < LH; r++) {
     const int kk = k0 - 2 + r;
@@ -408,10 +414,16 @@ __global__ void __launch_bounds__(64 * kMaskGroup) k_mask(DevParams P, DevPtrs D
     // a workgroup whose tiles are all clean has nothing to do (the same
     // test on every wavefront, so none waits at a barrier)
     bool any = false;
+    // the group's flags from clamped addresses, all in flight together
+    uint8_t fl[kMaskGroup];
+#pragma unroll
+    for (int g = 0; g < kMaskGroup; g++)
+      fl[g] = D.tile_dirty[min((int)blockIdx.y * kMaskGroup + g, tiles_y - 1) * tiles_x +
+                           (int)blockIdx.x];
 #pragma unroll
     for (int g = 0; g < kMaskGroup; g++) {
       const int ty = blockIdx.y * kMaskGroup + g;
-      const bool dirty = ty < tiles_y && D.tile_dirty[ty * tiles_x + blockIdx.x];
+      const bool dirty = ty < tiles_y && fl[g];
       any = any || dirty;
       if (g == w) clean = !dirty;
     }
@@ -668,15 +680,19 @@ __device__ __forceinline__ double face_flux(const Side<NS>& L, const Side<NS>& R
 // address (zero without a radial source).
 __device__ __forceinline__ void load_src(const DevParams& P, const DevPtrs& D, int j, int k,
                                          int& b, double& v0, double& v1) {
-  b = 0;
-  v0 = 0.0;
-  v1 = 0.0;
-  if (P.radial && D.sinfo) {
-    const size_t c = cidx(P, min(max(j, 0), P.nx - 1), min(max(k, 0), P.ny - 1));
-    b = D.sinfo[c];
-    v0 = D.sring[c];
-    v1 = D.sring[plane(P) + c];
-  }
+  // unconditional loads (from the state planes when there is no source),
+  // selected afterwards: a load under a condition is waited for where the
+  // branches join, which would serialise it with the row's other loads
+  const bool on = P.radial && D.sinfo;
+  const uint8_t* bi = on ? D.sinfo : D.sflag;
+  const double* vr = on ? D.sring : D.q;
+  const size_t c = cidx(P, min(max(j, 0), P.nx - 1), min(max(k, 0), P.ny - 1));
+  b = bi[c];
+  v0 = vr[c];
+  v1 = vr[plane(P) + c];
+  b = on ? b : 0;
+  v0 = on ? v0 : 0.0;
+  v1 = on ? v1 : 0.0;
 }
 
 // boundary state of a ring cell (zero elsewhere), src/solver_2d.f90:3879-3889
@@ -841,21 +857,38 @@ __device__ __forceinline__ void load_rowdata(const DevParams& P, const DevPtrs& 
                                              RowData& R) {
   const int jc = min(max(j, 0), P.nx - 1), kc = min(max(k, 0), P.ny - 1);
   const size_t c = cidx(P, jc, kc);
+  // the optional fields from the state planes when absent (NV planes cover
+  // every index here), selected afterwards, so that all loads are in flight
+  // together (see load_src)
+  const double* gsx = D.gsx ? D.gsx : D.q;
+  const double* bxx = D.Bxx ? D.Bxx : D.q;
+  const double* bxy = D.Bxy ? D.Bxy : D.q;
+  const double* byy = D.Byy ? D.Byy : D.q;
+  const double* gco = D.gcoef ? D.gcoef : D.q;
+  const double* dgd = D.dgdx ? D.dgdx : D.q;
   R.m = D.mask[c];
-  R.gW = D.gsx ? D.gsx[(size_t)kc * (P.pitch + 1) + jc] : 1.0;
-  R.gE = D.gsx ? D.gsx[(size_t)kc * (P.pitch + 1) + jc + 1] : 1.0;
+  R.gW = gsx[(size_t)kc * (P.pitch + 1) + jc];
+  R.gE = gsx[(size_t)kc * (P.pitch + 1) + jc + 1];
   R.bx = D.Bx[c];
   R.by = D.By[c];
-  R.bxx = D.Bxx ? D.Bxx[c] : 0.0;
-  R.bxy = D.Bxy ? D.Bxy[c] : 0.0;
-  R.byy = D.Byy ? D.Byy[c] : 0.0;
-  R.gc = D.gcoef ? D.gcoef[c] : 1.0;
-  R.dg = D.dgdx ? D.dgdx[c] : 0.0;
+  R.bxx = bxx[c];
+  R.bxy = bxy[c];
+  R.byy = byy[c];
+  R.gc = gco[c];
+  R.dg = dgd[c];
+  R.gW = D.gsx ? R.gW : 1.0;
+  R.gE = D.gsx ? R.gE : 1.0;
+  R.bxx = D.Bxx ? R.bxx : 0.0;
+  R.bxy = D.Bxy ? R.bxy : 0.0;
+  R.byy = D.Byy ? R.byy : 0.0;
+  R.gc = D.gcoef ? R.gc : 1.0;
+  R.dg = D.dgdx ? R.dg : 0.0;
 }
 // y-face slope correction at the S face of row k (k = ny: the N boundary)
 __device__ __forceinline__ double load_gsy(const DevParams& P, const DevPtrs& D, int j, int k) {
-  if (!D.gsy) return 1.0;
-  return D.gsy[cidx(P, min(max(j, 0), P.nx - 1), min(max(k, 0), P.ny))];
+  const double* g = D.gsy ? D.gsy : D.q;  // (see load_rowdata)
+  const double v = g[cidx(P, min(max(j, 0), P.nx - 1), min(max(k, 0), P.ny))];
+  return D.gsy ? v : 1.0;
 }
 
 template <int NS>
@@ -1586,17 +1619,27 @@ __global__ void __launch_bounds__(384) k_stencil_rows(DevParams P, DevPtrs D, in
     RowData R;
     load_rowdata(P, D, j, k, R);
     const double g = load_gsy(P, D, j, (wv < 2) ? k : k + 1);
+    // the raw states this wavefront reads, issued with the loads above
+    // (rows kr-1, kr, kr+1; the waves of the x pass use row k alone) and
+    // held in registers, so that the row waits for memory once
+    const int kr = (wv <= 2) ? k - 1 + wv : k;
+    double r0[NV], r1[NV], r2[NV];
+    load_raw<NS>(P, D, j, kr - 1, r0);
+    load_raw<NS>(P, D, j, kr, r1);
+    load_raw<NS>(P, D, j, kr + 1, r2);
+#pragma unroll
+    for (int i = 0; i < NV; i++) {
+      pin(r0[i]);
+      pin(r1[i]);
+      pin(r2[i]);
+    }
     const SrcInfo S = injx ? src_info(bc) : SrcInfo{};
     const SrcInfo So = injx ? src_info(bo) : SrcInfo{};
     double qk[NV + 2], ex[NV];
     // ---- phase A
     if (wv <= 2) {
       // y reconstruction of row kr = k - 1 + wv from rows kr-1, kr, kr+1
-      const int kr = k - 1 + wv;
-      double r0[NV], r1[NV], r2[NV], qa[NV + 2], qb[NV + 2], qc[NV + 2];
-      load_raw<NS>(P, D, j, kr - 1, r0);
-      load_raw<NS>(P, D, j, kr, r1);
-      load_raw<NS>(P, D, j, kr + 1, r2);
+      double qa[NV + 2], qb[NV + 2], qc[NV + 2];
       finish_row<NS>(P, j, kr - 1, r0, qa);
       finish_row<NS>(P, j, kr, r1, qb);
       finish_row<NS>(P, j, kr + 1, r2, qc);
@@ -1625,9 +1668,7 @@ __global__ void __launch_bounds__(384) k_stencil_rows(DevParams P, DevPtrs D, in
         for (int i = 0; i < NQ; i++) V[3][i][lane] = lo[i];  // lot of the N face
       }
     } else {
-      double r[NV];
-      load_raw<NS>(P, D, j, k, r);
-      finish_row<NS>(P, j, k, r, qk);
+      finish_row<NS>(P, j, k, r1, qk);
       if (wv == 3) {
         double sb[NV + 2], qkm[NV + 2], qkp[NV + 2], lo[NV + 2], hi[NV + 2];
 #pragma unroll
@@ -2439,14 +2480,23 @@ __global__ void __launch_bounds__(256) k_topo_list(DevParams P, DevPtrs D) {
     if (tx < 0 || tx >= tiles_x || ty < 0 || ty >= tiles_y) continue;
     const int nt = ty * tiles_x + tx;
     if (nt != t) {
-      if (D.tile_flag[nt]) continue;  // listed: done by itself
+      // the flags of nt and the tiles around it in one round of loads from
+      // clamped addresses (conditional loads would be waited for one by one)
+      uint8_t f[9];
+#pragma unroll
+      for (int m = 0; m < 9; m++) {
+        const int x = min(max(tx + m % 3 - 1, 0), tiles_x - 1);
+        const int y = min(max(ty + m / 3 - 1, 0), tiles_y - 1);
+        f[m] = D.tile_flag[y * tiles_x + x];
+      }
+      if (f[4]) continue;  // listed: done by itself
       int first = t;
-      for (int dy = -1; dy <= 1; dy++)
-        for (int dx = -1; dx <= 1; dx++) {
-          const int x = tx + dx, y = ty + dy;
-          if (x >= 0 && x < tiles_x && y >= 0 && y < tiles_y && D.tile_flag[y * tiles_x + x])
-            first = min(first, y * tiles_x + x);
-        }
+#pragma unroll
+      for (int m = 0; m < 9; m++) {
+        const int x = tx + m % 3 - 1, y = ty + m / 3 - 1;
+        if (x >= 0 && x < tiles_x && y >= 0 && y < tiles_y && f[m])
+          first = min(first, y * tiles_x + x);
+      }
       if (first != t) continue;
     }
     const int j = tx * kStripW - 2 + lane;
