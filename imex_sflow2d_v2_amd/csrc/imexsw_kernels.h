@@ -403,23 +403,28 @@ __device__ __forceinline__ void append_group(const ListSink& L, int tile, unsign
 template <int = 0>
 __global__ void __launch_bounds__(64 * kMaskGroup) k_mask(DevParams P, DevPtrs D,
                                                           int solve_all, int partial) {
-  // a run that has stopped (t_output or t_end reached) keeps the mask of
-  // its last step, the solve_mask the reference leaves behind
-  if (D.scal[S_DONE] != 0.0) return;
   const int tiles_x = (P.nx + kStripW - 1) / kStripW;
   const int tiles_y = (P.ny + kTileH - 1) / kTileH;
   const int w = threadIdx.x >> 6;
+  // the group's dirty flags from clamped addresses, in flight together with
+  // the done flag
+  int fl[kMaskGroup];
+#pragma unroll
+  for (int g = 0; g < kMaskGroup; g++)
+    fl[g] = D.tile_dirty[min((int)blockIdx.y * kMaskGroup + g, tiles_y - 1) * tiles_x +
+                         (int)blockIdx.x];
+  double done = D.scal[S_DONE];
+#pragma unroll
+  for (int g = 0; g < kMaskGroup; g++) pin(fl[g]);
+  pin(done);
+  // a run that has stopped (t_output or t_end reached) keeps the mask of
+  // its last step, the solve_mask the reference leaves behind
+  if (done != 0.0) return;
   bool clean = false;
   if (partial) {
     // a workgroup whose tiles are all clean has nothing to do (the same
     // test on every wavefront, so none waits at a barrier)
     bool any = false;
-    // the group's flags from clamped addresses, all in flight together
-    uint8_t fl[kMaskGroup];
-#pragma unroll
-    for (int g = 0; g < kMaskGroup; g++)
-      fl[g] = D.tile_dirty[min((int)blockIdx.y * kMaskGroup + g, tiles_y - 1) * tiles_x +
-                           (int)blockIdx.x];
 #pragma unroll
     for (int g = 0; g < kMaskGroup; g++) {
       const int ty = blockIdx.y * kMaskGroup + g;
