@@ -1791,13 +1791,21 @@ struct StepDt {
   double dt, dt_cfl, t_out;
 };
 __device__ __forceinline__ StepDt step_dt(const DevParams& P, const DevPtrs& D) {
-  const double* s = D.scal;
+  // every scalar in one round of loads (the tests below would otherwise
+  // wait for each load in turn)
+  double s[S_NSLOTS];
+#pragma unroll
+  for (int i = 0; i < S_NSLOTS; i++) s[i] = D.scal[i];
+  unsigned long long bits = *D.dt_bits;
+#pragma unroll
+  for (int i = 0; i < S_NSLOTS; i++) pin(s[i]);
+  pin(bits);
   StepDt r{true, 0.0, 0.0, 0.0};
   if (s[S_DONE] != 0.0) return r;
   const double t = s[S_T], t_end = s[S_T_END];
   double t_out = s[S_T_OUT];
   if (!(t < t_end)) return r;
-  double dt = (P.cfl != -1.0) ? __longlong_as_double((long long)*D.dt_bits) : P.max_dt;
+  double dt = (P.cfl != -1.0) ? __longlong_as_double((long long)bits) : P.max_dt;
   r.dt_cfl = dt;
   if (t_end - t_out < 1.0e-7) t_out = t_end;
   // paused at an output time: the host writes its output and raises t_output
@@ -2230,13 +2238,14 @@ __device__ __forceinline__ void local_row(const DevParams& P, const DevPtrs& D, 
 // next step's mask.  No per-row sums.
 constexpr int kCellsPerWave = kWave / kSparseLanes;
 template <int NS, int PH>
+// (c_pre: the entry of the list for this lane, when already loaded, else -1)
 __device__ __forceinline__ void local_cells(const DevParams& P, const DevPtrs& D, int item,
                                             int ncells, double dt, int ops,
-                                            unsigned long long* e, int& imax) {
+                                            unsigned long long* e, int& imax, int c_pre) {
   const int lane = threadIdx.x & 63, i = lane / kSparseLanes;
   const int idx = item * kCellsPerWave + i;
   if (i >= kCellsPerWave || idx >= ncells) return;
-  const int c = D.cell_list[idx];
+  const int c = (c_pre >= 0) ? c_pre : D.cell_list[idx];
   if ((lane % kSparseLanes) == 0) {
     const int tiles_x = (P.nx + kStripW - 1) / kStripW;
     const int k = c / (int)P.pitch, j = c - k * (int)P.pitch;
@@ -2348,6 +2357,14 @@ __global__ void __launch_bounds__(256, (!VP && (PH == 0 || PH == 1)) ? 2 : 1)
   for (int m = lane; m < kLocalEvents; m += 64) cnt[w][m] = 0;
   __syncthreads();
   int imax = 0;
+  // PH 3, 4: the list entry of this lane's first item, from a clamped
+  // address, in flight with the step's scalars
+  int c_first = 0;
+  if (PH >= 3) {
+    const int idx = (int)min((long long)(blockIdx.x * 4 + w) * kCellsPerWave + lane / kSparseLanes,
+                             (long long)P.nx * P.ny - 1);
+    c_first = D.cell_list[idx];
+  }
   bool live;
   double dt;
   if (ops & OP_POLICY) {
@@ -2364,7 +2381,8 @@ __global__ void __launch_bounds__(256, (!VP && (PH == 0 || PH == 1)) ? 2 : 1)
   const int items = (PH >= 3) ? (nlist + kCellsPerWave - 1) / kCellsPerWave : nlist;
   for (int it = blockIdx.x * 4 + w; live && it < items; it += gridDim.x * 4) {
     if (PH >= 3) {
-      local_cells<NS, (PH >= 3 ? PH : 3)>(P, D, it, nlist, dt, ops, cnt[w], imax);
+      local_cells<NS, (PH >= 3 ? PH : 3)>(P, D, it, nlist, dt, ops, cnt[w], imax,
+                                           (it == (int)blockIdx.x * 4 + w) ? c_first : -1);
     } else {
       if (lane == 0) mark_dirty(P, D, D.row_list[it] / kTileH);
       local_row<NS, PH, VP>(P, D, D.row_list[it], dt, ops, cnt[w], imax);
