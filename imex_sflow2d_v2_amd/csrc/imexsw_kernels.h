@@ -152,6 +152,25 @@ __device__ __forceinline__ void pin(double& x) { asm volatile("" : "+v"(x)); }
 __device__ __forceinline__ void pin(int& x) { asm volatile("" : "+v"(x)); }
 __device__ __forceinline__ void pin(unsigned long long& x) { asm volatile("" : "+v"(x)); }
 
+// The first BYTES of the kernel's argument block (DevParams and DevPtrs are
+// passed by value) touched once per 64-byte line, all loads in flight
+// together.  Short of scalar registers, the compiler reloads single
+// arguments from that block throughout a large kernel, each reload waited
+// for on its own; with the block's lines in the scalar cache from the start
+// those reloads no longer go to L2 one at a time.
+template <unsigned BYTES>
+__device__ __forceinline__ void warm_kernargs() {
+  typedef const __attribute__((address_space(4))) uint32_t karg_t;
+  karg_t* a = (karg_t*)__builtin_amdgcn_kernarg_segment_ptr();
+  constexpr unsigned N = (BYTES + 63) / 64;
+  uint32_t x[N];
+#pragma unroll
+  for (unsigned i = 0; i < N; i++) x[i] = a[i * 16];  // i * 64 < BYTES
+#pragma unroll
+  for (unsigned i = 0; i < N; i++) asm volatile("" ::"s"(x[i]));
+}
+constexpr unsigned kArgBytes = sizeof(DevParams) + sizeof(DevPtrs);
+
 // ---------------------------------------------------------------------------
 // wave shuffles of doubles (ds_bpermute on 32-bit halves)
 // ---------------------------------------------------------------------------
@@ -485,7 +504,7 @@ __device__ __forceinline__ void qp2_uv(const double* qp, double& u, double& v) {
 template <int NS, int DIR>
 __device__ __forceinline__ void recon(const DevParams& P, const double* c,
                                       const double* m, const double* p, int idx,
-                                      int n, int other, const SrcInfo& S,
+                                      int n, int other, const SrcInfo S,
                                       const double* sb, bool typo, double* lo,
                                       double* hi) {
   constexpr int NV = 4 + NS;
@@ -702,7 +721,7 @@ __device__ __forceinline__ void load_src(const DevParams& P, const DevPtrs& D, i
 
 // boundary state of a ring cell (zero elsewhere), src/solver_2d.f90:3879-3889
 template <int NS>
-__device__ __forceinline__ void ring_state(const DevParams& P, const SrcInfo& S,
+__device__ __forceinline__ void ring_state(const DevParams& P, const SrcInfo S,
                                            const SrcTime& T, double v0, double v1,
                                            double* sb) {
   constexpr int NV = 4 + NS;
@@ -808,8 +827,8 @@ __device__ __forceinline__ void load_row(const DevParams& P, const DevPtrs& D,
 // combination the library refuses)
 template <int NS, bool VP = false>
 __device__ __forceinline__ void yface_bottom(const DevParams& P, const double* hib,
-                                             const double* lot, const SrcInfo& Sb,
-                                             const SrcInfo& St, int k, double g, Side<NS>& Bs,
+                                             const double* lot, const SrcInfo Sb,
+                                             const SrcInfo St, int k, double g, Side<NS>& Bs,
                                              const double* pb = nullptr) {
   const double bx = VP ? pb[0] : 0.0, by = VP ? pb[1] : 0.0;
   if (Sb.is_src) {
@@ -821,8 +840,8 @@ __device__ __forceinline__ void yface_bottom(const DevParams& P, const double* h
 }
 template <int NS, bool VP = false>
 __device__ __forceinline__ void yface_top(const DevParams& P, const double* hib,
-                                          const double* lot, const SrcInfo& Sb,
-                                          const SrcInfo& St, int k, double g, Side<NS>& Ts,
+                                          const double* lot, const SrcInfo Sb,
+                                          const SrcInfo St, int k, double g, Side<NS>& Ts,
                                           const double* pt = nullptr) {
   const double bx = VP ? pt[0] : 0.0, by = VP ? pt[1] : 0.0;
   if (St.is_src) {
@@ -835,8 +854,8 @@ __device__ __forceinline__ void yface_top(const DevParams& P, const double* hib,
 }
 template <int NS, bool VP = false>
 __device__ __forceinline__ void yface_sides(const DevParams& P, const double* hib,
-                                            const double* lot, const SrcInfo& Sb,
-                                            const SrcInfo& St, int k, double g,
+                                            const double* lot, const SrcInfo Sb,
+                                            const SrcInfo St, int k, double g,
                                             Side<NS>& Bs, Side<NS>& Ts,
                                             const double* pb = nullptr,
                                             const double* pt = nullptr) {
@@ -897,7 +916,7 @@ __device__ __forceinline__ double load_gsy(const DevParams& P, const DevPtrs& D,
 }
 
 template <int NS>
-__device__ __forceinline__ void x_face_fluxes(const DevParams& P, const SrcInfo& S, int j,
+__device__ __forceinline__ void x_face_fluxes(const DevParams& P, const SrcInfo S, int j,
                                               const Side<NS>& Wself, const Side<NS>& Eself,
                                               const Side<NS>& Eleft, double* HW, double* HE,
                                               double& amaxW, double& amaxE);
@@ -906,7 +925,7 @@ __device__ __forceinline__ void x_face_fluxes(const DevParams& P, const SrcInfo&
 // shuffle from the right neighbour (a boundary face on the last column).
 // S is row k's source data, sb its ring state, qk its physical state.
 template <int NS, bool HASY, bool VP = false>
-__device__ __forceinline__ void x_fluxes(const DevParams& P, const SrcInfo& S, const double* sb,
+__device__ __forceinline__ void x_fluxes(const DevParams& P, const SrcInfo S, const double* sb,
                                          const double* qk, int j, int k, const RowData& R,
                                          double* HW, double* HE, double& amaxW,
                                          double& amaxE) {
@@ -947,7 +966,7 @@ __device__ __forceinline__ void x_fluxes(const DevParams& P, const SrcInfo& S, c
 // the left neighbour): the W face on every lane, the E face by shuffle from
 // the right neighbour (a boundary face on the last column).
 template <int NS>
-__device__ __forceinline__ void x_face_fluxes(const DevParams& P, const SrcInfo& S, int j,
+__device__ __forceinline__ void x_face_fluxes(const DevParams& P, const SrcInfo S, int j,
                                               const Side<NS>& Wself, const Side<NS>& Eself,
                                               const Side<NS>& Eleft, double* HW, double* HE,
                                               double& amaxW, double& amaxE) {
@@ -1415,8 +1434,8 @@ __global__ void __launch_bounds__(256, 2) k_vp_sides(DevParams P, DevPtrs D) {
 // above, with the ring overrides of yface_bottom / yface_top
 template <int NS>
 __device__ __forceinline__ void vp_yface(const DevParams& P, const Side<NS>& Bn,
-                                         const Side<NS>& Ts_, const SrcInfo& Sb,
-                                         const SrcInfo& St, int k, Side<NS>& Bs, Side<NS>& Ts) {
+                                         const Side<NS>& Ts_, const SrcInfo Sb,
+                                         const SrcInfo St, int k, Side<NS>& Bs, Side<NS>& Ts) {
   if (Sb.is_src) {
     if (P.radial && St.src2 && St.sS && k < P.ny - 1) Bs = Ts_;
     else zero_side<NS>(Bs);
@@ -1587,15 +1606,201 @@ __device__ __forceinline__ void side_load(double (*buf)[64], Side<NS>& s, int la
   s.zv = buf[10 + 2 * NS][lane];
 }
 
+// One listed row of k_stencil_rows (all six wavefronts of the workgroup).
+template <int NS>
+__device__ __forceinline__ void stencil_row(const DevParams& P, const DevPtrs& D, int row,
+                                            int tiles_x, double t, int compute_dt,
+                                            double& dtmin, double (*V)[NS + 6][64],
+                                            double (*SD)[2 * NS + 11][64]) {
+  constexpr int NV = 4 + NS;
+  constexpr int NQ = NV + 2;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int tile = row / kTileH;
+  const int k = (tile / tiles_x) * kTileH + row % kTileH;
+  const int j = (tile % tiles_x) * kStripW - 2 + lane;
+  const bool injx = (j >= 0 && j < P.nx);
+  const bool own = (lane >= 2 && lane < 2 + kStripW && injx);
+  // row k's source data, the neighbour rows' (k-1 for the S face, k+1
+  // for the N face), and row k's per-cell data
+  int bc, bo;
+  double vc0, vc1, vo0, vo1;
+  load_src(P, D, j, k, bc, vc0, vc1);
+  load_src(P, D, j, (wv < 2) ? k - 1 : k + 1, bo, vo0, vo1);
+  RowData R;
+  load_rowdata(P, D, j, k, R);
+  const double g = load_gsy(P, D, j, (wv < 2) ? k : k + 1);
+  // the raw states this wavefront reads, issued with the loads above
+  // (rows kr-1, kr, kr+1; the waves of the x pass use row k alone) and
+  // held in registers, so that the row waits for memory once
+  const int kr = (wv <= 2) ? k - 1 + wv : k;
+  double r0[NV], r1[NV], r2[NV];
+  load_raw<NS>(P, D, j, kr - 1, r0);
+  load_raw<NS>(P, D, j, kr, r1);
+  load_raw<NS>(P, D, j, kr + 1, r2);
+  // the source's time factors, evaluated while those loads are in flight
+  // (t passes through a register pin, so that the evaluation stays behind
+  // the loads)
+  double tl = t;
+  pin(tl);
+  const SrcTime T = P.radial ? src_time(P, tl) : SrcTime{};
+#pragma unroll
+  for (int i = 0; i < NV; i++) {
+    pin(r0[i]);
+    pin(r1[i]);
+    pin(r2[i]);
+  }
+  const SrcInfo S = injx ? src_info(bc) : SrcInfo{};
+  const SrcInfo So = injx ? src_info(bo) : SrcInfo{};
+  double qk[NV + 2], ex[NV];
+  // ---- phase A
+  if (wv <= 2) {
+    // y reconstruction of row kr = k - 1 + wv from rows kr-1, kr, kr+1
+    double qa[NV + 2], qb[NV + 2], qc[NV + 2];
+    finish_row<NS>(P, j, kr - 1, r0, qa);
+    finish_row<NS>(P, j, kr, r1, qb);
+    finish_row<NS>(P, j, kr + 1, r2, qc);
+    const bool need = injx && (wv == 1 || (wv == 0 && k > 0) || (wv == 2 && k < P.ny - 1));
+    double lo[NV + 2], hi[NV + 2];
+    if (need) {
+      // a copy, not a reference: a reference chosen at run time would put
+      // both structures in scratch memory
+      const SrcInfo Sr = (wv == 1) ? S : So;
+      double sb[NV + 2];
+      ring_state<NS>(P, Sr, T, (wv == 1) ? vc0 : vo0, (wv == 1) ? vc1 : vo1, sb);
+      recon<NS, 2>(P, qb, qa, qc, kr, P.ny, j, Sr, sb, false, lo, hi);
+    } else {
+#pragma unroll
+      for (int i = 0; i < NV + 2; i++) lo[i] = hi[i] = 0.0;
+    }
+    if (wv == 0) {
+#pragma unroll
+      for (int i = 0; i < NQ; i++) V[0][i][lane] = hi[i];  // hib of the S face
+    } else if (wv == 1) {
+#pragma unroll
+      for (int i = 0; i < NQ; i++) {
+        V[1][i][lane] = lo[i];  // lot of the S face
+        V[2][i][lane] = hi[i];  // hib of the N face
+      }
+    } else {
+#pragma unroll
+      for (int i = 0; i < NQ; i++) V[3][i][lane] = lo[i];  // lot of the N face
+    }
+  } else {
+    finish_row<NS>(P, j, k, r1, qk);
+    if (wv == 3) {
+      double sb[NV + 2], qkm[NV + 2], qkp[NV + 2], lo[NV + 2], hi[NV + 2];
+#pragma unroll
+      for (int i = 0; i < NV + 2; i++) {
+        qkm[i] = shfl_d(qk[i], (lane + 63) & 63);
+        qkp[i] = shfl_d(qk[i], (lane + 1) & 63);
+      }
+      const bool typo = injx && P.radial && S.src2 && k > 0 && k < P.ny - 1 && !S.sS && S.sN;
+      if (injx) {
+        ring_state<NS>(P, S, T, vc0, vc1, sb);
+        recon<NS, 1>(P, qk, qkm, qkp, j, P.nx, k, S, sb, typo, lo, hi);
+      } else {
+#pragma unroll
+        for (int i = 0; i < NV + 2; i++) lo[i] = hi[i] = 0.0;
+      }
+#pragma unroll
+      for (int i = 0; i < NQ; i++) {
+        V[4][i][lane] = lo[i];
+        V[5][i][lane] = hi[i];
+      }
+    } else if (wv == 4 && own && (R.m & M_COMPUTE)) {
+      expl_terms<NS>(P, R.bx, R.by, R.bxx, R.bxy, R.byy, R.gc, R.dg, qk, ex);
+    }
+  }
+  __syncthreads();
+  // ---- phase B
+  Side<NS> Wself;
+  {
+    double va[NQ], vb[NQ];
+    const int ia = (wv < 2) ? 0 : (wv < 4) ? 2 : 4 + (wv - 4);
+#pragma unroll
+    for (int i = 0; i < NQ; i++) va[i] = V[ia][i][lane];
+    if (wv < 4) {
+#pragma unroll
+      for (int i = 0; i < NQ; i++) vb[i] = V[ia + 1][i][lane];
+    }
+    Side<NS> sd;
+    if (wv < 2) {
+      // S face of row k: bottom = row k-1 (So), top = row k (S)
+      if (!injx) {
+        zero_side<NS>(sd);
+      } else if (k == 0) {
+        // boundary face: both sides carry the S value of row 0
+        if (S.is_src) zero_side<NS>(sd); else make_side<NS, 2>(P, vb, g, sd);
+      } else if (wv == 0) {
+        yface_bottom<NS>(P, va, vb, So, S, k, g, sd);
+      } else {
+        yface_top<NS>(P, va, vb, So, S, k, g, sd);
+      }
+      side_store<NS>(SD[wv], sd, lane);
+    } else if (wv < 4) {
+      // N face of row k: bottom = row k (S), top = row k+1 (So)
+      if (!injx) {
+        zero_side<NS>(sd);
+      } else if (k == P.ny - 1) {
+        if (S.is_src) zero_side<NS>(sd); else make_side<NS, 2>(P, va, g, sd);
+      } else if (wv == 2) {
+        yface_bottom<NS>(P, va, vb, S, So, k + 1, g, sd);
+      } else {
+        yface_top<NS>(P, va, vb, S, So, k + 1, g, sd);
+      }
+      side_store<NS>(SD[wv], sd, lane);
+    } else {
+      // x faces: W side (w4, kept), E side (w5, to LDS)
+      if (!injx || S.is_src) zero_side<NS>(sd);
+      else make_side<NS, 1>(P, va, (wv == 4) ? R.gW : R.gE, sd);
+      if (wv == 4) Wself = sd;
+      else side_store<NS>(SD[4], sd, lane);
+    }
+  }
+  __syncthreads();
+  // ---- phase C
+  double HW[NV], HE[NV], amaxW = 0.0, amaxE = 0.0;
+  if (wv == 0 || wv == 2) {
+    Side<NS> Bs, Ts;
+    side_load<NS>(SD[wv], Bs, lane);
+    side_load<NS>(SD[wv + 1], Ts, lane);
+    double H[NV];
+    const double a = face_flux<NS, 2>(Bs, Ts, H);
+    const int o = (wv == 0) ? 0 : 1;
+#pragma unroll
+    for (int i = 0; i < NV; i++) V[o][i][lane] = H[i];
+    V[o][NV][lane] = a;
+  } else if (wv == 4) {
+    Side<NS> Eself, Eleft;
+    side_load<NS>(SD[4], Eself, lane);
+    side_load<NS>(SD[4], Eleft, (lane + 63) & 63);
+    x_face_fluxes<NS>(P, S, j, Wself, Eself, Eleft, HW, HE, amaxW, amaxE);
+  }
+  __syncthreads();
+  // ---- phase D
+  if (wv == 4 && own && (R.m & M_COMPUTE)) {
+    double HS[NV], HN[NV];
+#pragma unroll
+    for (int i = 0; i < NV; i++) {
+      HS[i] = V[0][i][lane];
+      HN[i] = V[1][i][lane];
+    }
+    cell_update<NS, true>(P, D, cidx(P, j, k), qk, R, HW, HE, HS, HN, amaxW, amaxE,
+                          V[0][NV][lane], V[1][NV][lane], compute_dt, dtmin, ex);
+  }
+  __syncthreads();  // LDS free for the next row
+}
+
 template <int NS>
 __global__ void __launch_bounds__(384) k_stencil_rows(DevParams P, DevPtrs D, int compute_dt) {
+  warm_kernargs<kArgBytes>();
   constexpr int NV = 4 + NS;
   constexpr int NQ = NV + 2;       // reconstructed values per side
   constexpr int NSD = 2 * NV + 3;  // doubles per face side
   // phase A values (then the y fluxes of phase C), face sides
   __shared__ double V[6][NQ][64];
   __shared__ double SD[5][NSD][64];  // S bottom, S top, N bottom, N top, E of x
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int wv = threadIdx.x >> 6;
   // the step's scalars, list size and first row in one round of loads (the
   // grid is never larger than the row list, so row_list[blockIdx.x] exists)
   double done = D.scal[S_DONE], t = D.scal[S_T];
@@ -1604,179 +1809,15 @@ __global__ void __launch_bounds__(384) k_stencil_rows(DevParams P, DevPtrs D, in
   pin(t);
   pin(nrows);
   pin(row);
-  if (done != 0.0) return;
-  const SrcTime T = P.radial ? src_time(P, t) : SrcTime{};
+  if (done != 0.0 || (int)blockIdx.x >= nrows) return;
   const int tiles_x = (P.nx + kStripW - 1) / kStripW;
   double dtmin = P.max_dt;
-  for (int item = blockIdx.x; item < nrows; item += gridDim.x) {
-    if (item != blockIdx.x) row = D.row_list[item];
-    const int tile = row / kTileH;
-    const int k = (tile / tiles_x) * kTileH + row % kTileH;
-    const int j = (tile % tiles_x) * kStripW - 2 + lane;
-    const bool injx = (j >= 0 && j < P.nx);
-    const bool own = (lane >= 2 && lane < 2 + kStripW && injx);
-    // row k's source data, the neighbour rows' (k-1 for the S face, k+1
-    // for the N face), and row k's per-cell data
-    int bc, bo;
-    double vc0, vc1, vo0, vo1;
-    load_src(P, D, j, k, bc, vc0, vc1);
-    load_src(P, D, j, (wv < 2) ? k - 1 : k + 1, bo, vo0, vo1);
-    RowData R;
-    load_rowdata(P, D, j, k, R);
-    const double g = load_gsy(P, D, j, (wv < 2) ? k : k + 1);
-    // the raw states this wavefront reads, issued with the loads above
-    // (rows kr-1, kr, kr+1; the waves of the x pass use row k alone) and
-    // held in registers, so that the row waits for memory once
-    const int kr = (wv <= 2) ? k - 1 + wv : k;
-    double r0[NV], r1[NV], r2[NV];
-    load_raw<NS>(P, D, j, kr - 1, r0);
-    load_raw<NS>(P, D, j, kr, r1);
-    load_raw<NS>(P, D, j, kr + 1, r2);
-#pragma unroll
-    for (int i = 0; i < NV; i++) {
-      pin(r0[i]);
-      pin(r1[i]);
-      pin(r2[i]);
-    }
-    const SrcInfo S = injx ? src_info(bc) : SrcInfo{};
-    const SrcInfo So = injx ? src_info(bo) : SrcInfo{};
-    double qk[NV + 2], ex[NV];
-    // ---- phase A
-    if (wv <= 2) {
-      // y reconstruction of row kr = k - 1 + wv from rows kr-1, kr, kr+1
-      double qa[NV + 2], qb[NV + 2], qc[NV + 2];
-      finish_row<NS>(P, j, kr - 1, r0, qa);
-      finish_row<NS>(P, j, kr, r1, qb);
-      finish_row<NS>(P, j, kr + 1, r2, qc);
-      const bool need = injx && (wv == 1 || (wv == 0 && k > 0) || (wv == 2 && k < P.ny - 1));
-      double lo[NV + 2], hi[NV + 2];
-      if (need) {
-        const SrcInfo& Sr = (wv == 1) ? S : So;
-        double sb[NV + 2];
-        ring_state<NS>(P, Sr, T, (wv == 1) ? vc0 : vo0, (wv == 1) ? vc1 : vo1, sb);
-        recon<NS, 2>(P, qb, qa, qc, kr, P.ny, j, Sr, sb, false, lo, hi);
-      } else {
-#pragma unroll
-        for (int i = 0; i < NV + 2; i++) lo[i] = hi[i] = 0.0;
-      }
-      if (wv == 0) {
-#pragma unroll
-        for (int i = 0; i < NQ; i++) V[0][i][lane] = hi[i];  // hib of the S face
-      } else if (wv == 1) {
-#pragma unroll
-        for (int i = 0; i < NQ; i++) {
-          V[1][i][lane] = lo[i];  // lot of the S face
-          V[2][i][lane] = hi[i];  // hib of the N face
-        }
-      } else {
-#pragma unroll
-        for (int i = 0; i < NQ; i++) V[3][i][lane] = lo[i];  // lot of the N face
-      }
-    } else {
-      finish_row<NS>(P, j, k, r1, qk);
-      if (wv == 3) {
-        double sb[NV + 2], qkm[NV + 2], qkp[NV + 2], lo[NV + 2], hi[NV + 2];
-#pragma unroll
-        for (int i = 0; i < NV + 2; i++) {
-          qkm[i] = shfl_d(qk[i], (lane + 63) & 63);
-          qkp[i] = shfl_d(qk[i], (lane + 1) & 63);
-        }
-        const bool typo = injx && P.radial && S.src2 && k > 0 && k < P.ny - 1 && !S.sS && S.sN;
-        if (injx) {
-          ring_state<NS>(P, S, T, vc0, vc1, sb);
-          recon<NS, 1>(P, qk, qkm, qkp, j, P.nx, k, S, sb, typo, lo, hi);
-        } else {
-#pragma unroll
-          for (int i = 0; i < NV + 2; i++) lo[i] = hi[i] = 0.0;
-        }
-#pragma unroll
-        for (int i = 0; i < NQ; i++) {
-          V[4][i][lane] = lo[i];
-          V[5][i][lane] = hi[i];
-        }
-      } else if (wv == 4 && own && (R.m & M_COMPUTE)) {
-        expl_terms<NS>(P, R.bx, R.by, R.bxx, R.bxy, R.byy, R.gc, R.dg, qk, ex);
-      }
-    }
-    __syncthreads();
-    // ---- phase B
-    Side<NS> Wself;
-    {
-      double va[NQ], vb[NQ];
-      const int ia = (wv < 2) ? 0 : (wv < 4) ? 2 : 4 + (wv - 4);
-#pragma unroll
-      for (int i = 0; i < NQ; i++) va[i] = V[ia][i][lane];
-      if (wv < 4) {
-#pragma unroll
-        for (int i = 0; i < NQ; i++) vb[i] = V[ia + 1][i][lane];
-      }
-      Side<NS> sd;
-      if (wv < 2) {
-        // S face of row k: bottom = row k-1 (So), top = row k (S)
-        if (!injx) {
-          zero_side<NS>(sd);
-        } else if (k == 0) {
-          // boundary face: both sides carry the S value of row 0
-          if (S.is_src) zero_side<NS>(sd); else make_side<NS, 2>(P, vb, g, sd);
-        } else if (wv == 0) {
-          yface_bottom<NS>(P, va, vb, So, S, k, g, sd);
-        } else {
-          yface_top<NS>(P, va, vb, So, S, k, g, sd);
-        }
-        side_store<NS>(SD[wv], sd, lane);
-      } else if (wv < 4) {
-        // N face of row k: bottom = row k (S), top = row k+1 (So)
-        if (!injx) {
-          zero_side<NS>(sd);
-        } else if (k == P.ny - 1) {
-          if (S.is_src) zero_side<NS>(sd); else make_side<NS, 2>(P, va, g, sd);
-        } else if (wv == 2) {
-          yface_bottom<NS>(P, va, vb, S, So, k + 1, g, sd);
-        } else {
-          yface_top<NS>(P, va, vb, S, So, k + 1, g, sd);
-        }
-        side_store<NS>(SD[wv], sd, lane);
-      } else {
-        // x faces: W side (w4, kept), E side (w5, to LDS)
-        if (!injx || S.is_src) zero_side<NS>(sd);
-        else make_side<NS, 1>(P, va, (wv == 4) ? R.gW : R.gE, sd);
-        if (wv == 4) Wself = sd;
-        else side_store<NS>(SD[4], sd, lane);
-      }
-    }
-    __syncthreads();
-    // ---- phase C
-    double HW[NV], HE[NV], amaxW = 0.0, amaxE = 0.0;
-    if (wv == 0 || wv == 2) {
-      Side<NS> Bs, Ts;
-      side_load<NS>(SD[wv], Bs, lane);
-      side_load<NS>(SD[wv + 1], Ts, lane);
-      double H[NV];
-      const double a = face_flux<NS, 2>(Bs, Ts, H);
-      const int o = (wv == 0) ? 0 : 1;
-#pragma unroll
-      for (int i = 0; i < NV; i++) V[o][i][lane] = H[i];
-      V[o][NV][lane] = a;
-    } else if (wv == 4) {
-      Side<NS> Eself, Eleft;
-      side_load<NS>(SD[4], Eself, lane);
-      side_load<NS>(SD[4], Eleft, (lane + 63) & 63);
-      x_face_fluxes<NS>(P, S, j, Wself, Eself, Eleft, HW, HE, amaxW, amaxE);
-    }
-    __syncthreads();
-    // ---- phase D
-    if (wv == 4 && own && (R.m & M_COMPUTE)) {
-      double HS[NV], HN[NV];
-#pragma unroll
-      for (int i = 0; i < NV; i++) {
-        HS[i] = V[0][i][lane];
-        HN[i] = V[1][i][lane];
-      }
-      cell_update<NS, true>(P, D, cidx(P, j, k), qk, R, HW, HE, HS, HN, amaxW, amaxE,
-                            V[0][NV][lane], V[1][NV][lane], compute_dt, dtmin, ex);
-    }
-    __syncthreads();  // LDS free for the next row
-  }
+  // the workgroup's first row outside the loop over further rows: in a loop
+  // the compiler hoists the row's uniform operands ahead of it, where, short
+  // of scalar registers, it fetches the kernel arguments one at a time
+  stencil_row<NS>(P, D, row, tiles_x, t, compute_dt, dtmin, V, SD);
+  for (int item = blockIdx.x + gridDim.x; item < nrows; item += gridDim.x)
+    stencil_row<NS>(P, D, D.row_list[item], tiles_x, t, compute_dt, dtmin, V, SD);
   if (wv == 4 && compute_dt) dt_reduce(D, P, dtmin);
 }
 
@@ -2351,6 +2392,7 @@ __global__ void __launch_bounds__(64) k_mlp_guess(DevParams P, DevPtrs D, MlpDev
 template <int NS, int PH, bool VP = false>
 __global__ void __launch_bounds__(256, (!VP && (PH == 0 || PH == 1)) ? 2 : 1)
     k_local(DevParams P, DevPtrs D, int ops) {
+  warm_kernargs<kArgBytes>();
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   __shared__ unsigned long long cnt[4][kLocalEvents];
   __shared__ int itmax[4];
