@@ -593,6 +593,9 @@ int imexsw_create(const imexsw_params* params, imexsw_ctx** out) {
   P.reconstr_coeff = p.reconstr_coeff;
   P.theta = p.theta;
   for (int i = 0; i < IMEXSW_MAX_VARS + 2; i++) P.limiter[i] = p.limiter[i];
+  P.lim_c = 0;
+  for (int i = 0; i < IMEXSW_MAX_VARS + 2; i++)
+    P.lim_c |= (p.limiter[i] >= 3 && p.limiter[i] <= 5) ? 1 : 0;
   for (int s = 0; s < 4; s++)
     for (int i = 0; i < IMEXSW_MAX_VARS; i++) {
       P.bc_flag[s][i] = p.bc_flag[s][i];

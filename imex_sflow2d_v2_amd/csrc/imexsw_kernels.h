@@ -534,29 +534,57 @@ __device__ __forceinline__ void recon(const DevParams& P, const double* c,
     z2 = ovr_hi ? stag(idx + 1) : comp(idx + 1);
   }
   (void)other;
+  // the limiter's operands of every variable, then all their divisions
+  // (no branch between them), then the limiter and the boundary cases per
+  // variable: the values of limit_ on the same operands
+  double v0[NV], v2[NV], A[NV], B[NV], C[NV];
+#pragma unroll
+  for (int i = 0; i < NV; i++) {
+    // the candidates in registers first: a choice between two addresses
+    // (of the argument block, or of two of the caller's arrays) would move
+    // the arrays to scratch memory
+    double blo = P.bc_value[side_lo][i], bhi = P.bc_value[side_hi][i];
+    double sbi = sb[i], mi = m[i], pi = p[i];
+    pin(blo);
+    pin(bhi);
+    pin(sbi);
+    pin(mi);
+    pin(pi);
+    if (first) {
+      v0[i] = srcb ? sbi : blo;
+      v2[i] = pi;
+    } else if (last) {
+      v0[i] = mi;
+      v2[i] = srcb ? sbi : bhi;
+    } else {
+      v0[i] = ovr_lo ? sbi : mi;
+      v2[i] = ovr_hi ? sbi : pi;
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < NV; i++) {
+    A[i] = (v2[i] - c[i]) / (z2 - z1);
+    B[i] = (c[i] - v0[i]) / (z1 - z0);
+  }
+#pragma unroll
+  for (int i = 0; i < NV; i++) C[i] = 0.0;
+  if (P.lim_c) {
+#pragma unroll
+    for (int i = 0; i < NV; i++) C[i] = (v2[i] - v0[i]) / (z2 - z0);
+  }
 #pragma unroll
   for (int i = 0; i < NV; i++) {
     const double s1 = c[i];
-    const int lim = P.limiter[i];
-    double slope = 0.0;
+    const double lim_slope = limit_pick(P, A[i], B[i], C[i], P.limiter[i]);
+    double slope = lim_slope;
     if (first) {
       const int f = P.bc_flag[side_lo][i];
       const double bv = P.bc_value[side_lo][i];
-      if (srcb) slope = limit_(P, sb[i], s1, p[i], z0, z1, z2, lim);
-      else if (f == 0) slope = limit_(P, bv, s1, p[i], z0, z1, z2, lim);
-      else if (f == 1) slope = bv;
-      else if (f == 2) slope = (p[i] - s1) * one_by;
+      if (!srcb && f != 0) slope = (f == 1) ? bv : (f == 2) ? (p[i] - s1) * one_by : 0.0;
     } else if (last) {
       const int f = P.bc_flag[side_hi][i];
       const double bv = P.bc_value[side_hi][i];
-      if (srcb) slope = limit_(P, m[i], s1, sb[i], z0, z1, z2, lim);
-      else if (f == 0) slope = limit_(P, m[i], s1, bv, z0, z1, z2, lim);
-      else if (f == 1) slope = bv;
-      else if (f == 2) slope = (s1 - m[i]) * one_by;
-    } else {
-      const double v0 = ovr_lo ? sb[i] : m[i];
-      const double v2 = ovr_hi ? sb[i] : p[i];
-      slope = limit_(P, v0, s1, v2, z0, z1, z2, lim);
+      if (!srcb && f != 0) slope = (f == 1) ? bv : (f == 2) ? (s1 - m[i]) * one_by : 0.0;
     }
     const double dq = P.reconstr_coeff * d2 * slope;
     lo[i] = s1 - dq;
@@ -574,39 +602,53 @@ __device__ __forceinline__ void recon(const DevParams& P, const double* c,
       else if (i == DIR) hi[i] = fmax(hi[i], 0.0);
     }
   }
-  // u and v with the limiters of hu and hv (:3496-3661)
+  // u and v with the limiters of hu and hv (:3496-3661), from the face
+  // values of h, hu, hv just formed when the cell is on the boundary
+  double ul = 0.0, wl = 0.0, uh = 0.0, wh = 0.0;
+  if (first) qp2_uv(lo, ul, wl);
+  if (last) qp2_uv(hi, uh, wh);
+  double u0[2], u2v[2], zz2[2], UA[2], UB[2], UC[2];
 #pragma unroll
-  for (int i = NV; i < NV + 2; i++) {
-    const double s1 = c[i];
-    double v0, v2, zz2 = z2;
-    if (DIR == 1 && i == NV + 1 && typo) zz2 = (!first && !last && radial2 && S.sE) ? stag(idx + 1) : ystag(P, other + 1);
+  for (int r = 0; r < 2; r++) {
+    const int i = NV + r;
+    zz2[r] = z2;
+    if (DIR == 1 && i == NV + 1 && typo)
+      zz2[r] = (!first && !last && radial2 && S.sE) ? stag(idx + 1) : ystag(P, other + 1);
+    double sbi = sb[i], mi = m[i], pi = p[i];
+    pin(sbi);
+    pin(mi);
+    pin(pi);
     if (first) {
-      double u2, w2;
-      qp2_uv(lo, u2, w2);
-      v0 = (i == NV) ? u2 : w2;
-      v2 = p[i];
+      u0[r] = (r == 0) ? ul : wl;
+      u2v[r] = pi;
     } else if (last) {
-      double u2, w2;
-      qp2_uv(hi, u2, w2);
-      v0 = m[i];
-      v2 = (i == NV) ? u2 : w2;
+      u0[r] = mi;
+      u2v[r] = (r == 0) ? uh : wh;
     } else {
-      v0 = ovr_lo ? sb[i] : m[i];
-      v2 = ovr_hi ? sb[i] : p[i];
+      u0[r] = ovr_lo ? sbi : mi;
+      u2v[r] = ovr_hi ? sbi : pi;
     }
-    const double slope = limit_(P, v0, s1, v2, z0, z1, zz2, P.limiter[i]);
+  }
+#pragma unroll
+  for (int r = 0; r < 2; r++) {
+    UA[r] = (u2v[r] - c[NV + r]) / (zz2[r] - z1);
+    UB[r] = (c[NV + r] - u0[r]) / (z1 - z0);
+    UC[r] = 0.0;
+  }
+  if (P.lim_c) {
+#pragma unroll
+    for (int r = 0; r < 2; r++) UC[r] = (u2v[r] - u0[r]) / (zz2[r] - z0);
+  }
+#pragma unroll
+  for (int r = 0; r < 2; r++) {
+    const int i = NV + r;
+    const double s1 = c[i];
+    const double slope = limit_pick(P, UA[r], UB[r], UC[r], P.limiter[i]);
     const double dq = P.reconstr_coeff * d2 * slope;
     lo[i] = s1 - dq;
     hi[i] = s1 + dq;
-    if (first) {
-      double u2, w2;
-      qp2_uv(lo, u2, w2);
-      lo[i] = (i == NV) ? u2 : w2;
-    } else if (last) {
-      double u2, w2;
-      qp2_uv(hi, u2, w2);
-      hi[i] = (i == NV) ? u2 : w2;
-    }
+    if (first) lo[i] = (r == 0) ? ul : wl;
+    else if (last) hi[i] = (r == 0) ? uh : wh;
   }
   if (!first && !last) {
     if (ovr_hi) {

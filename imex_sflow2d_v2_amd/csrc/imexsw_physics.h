@@ -31,6 +31,7 @@ struct DevParams {
   double eps_sing, eps_sing4;
   double cfl, max_dt, reconstr_coeff, theta;
   int limiter[IMEXSW_MAX_VARS + 2];
+  int lim_c;  // some limiter (3, 4, 5) uses the centred difference
   int bc_flag[4][IMEXSW_MAX_VARS];
   double bc_value[4][IMEXSW_MAX_VARS];
   // the side's condition on h keeps a dry neighbourhood dry: zero-gradient
@@ -144,6 +145,23 @@ __device__ __forceinline__ double limit_(const DevParams& P, double v0, double v
   if (lim == 3) return minmod_(c, P.theta * minmod_(a, b));
   if (lim == 4) return minmod_(c, 2.0 * minmod_(a, b));
   return c;  // 5: centred
+}
+// limit_ from its differences a, b and (when P.lim_c) c, already divided:
+// the same value, with the divisions of all variables of a reconstruction
+// issued together rather than one variable's after another's branches.
+__device__ __forceinline__ double limit_pick(const DevParams& P, double a, double b, double c,
+                                             int lim) {
+  if (lim == 0) return 0.0;
+  switch (lim) {
+    case 1: return minmod_(a, b);
+    case 2: return maxmod_(minmod_(a, 2.0 * b), minmod_(2.0 * a, b));
+    case 6: return a;
+    case 7: return b;
+    default: break;
+  }
+  if (lim == 3) return minmod_(c, P.theta * minmod_(a, b));
+  if (lim == 4) return minmod_(c, 2.0 * minmod_(a, b));
+  return c;
 }
 
 // settling_velocity (src/constitutive_2d.f90:5148-5214)
