@@ -374,24 +374,29 @@ __device__ __forceinline__ void append_group(const ListSink& L, int tile, unsign
     s_cells[w] = nc_w;
   }
   __syncthreads();
-  if (threadIdx.x == 0) {
+  if (w == 0) {
     int nt = 0, nr = 0, nc = 0;
     for (int g = 0; g < G; g++) {
       nt += s_rows[g] != 0u;
       nr += __popc(s_rows[g]);
       nc += s_cells[g];
     }
-    // nr > 0 exactly when nt > 0 (and nc > 0 only then): one branch, the
-    // reservations in flight together
-    int tb = 0, rb = 0, cb = 0;
-    if (nt) {
-      tb = (int)atomicAdd(L.n_tiles, (unsigned long long)nt);
-      rb = (int)atomicAdd(L.n_rows, (unsigned long long)nr);
-      if (cells && nc) cb = (int)atomicAdd(L.n_cells, (unsigned long long)nc);
-    }
-    s_tb = tb;
-    s_rb = rb;
-    s_cb = cb;
+    // the three reservations as one atomic instruction on three lanes
+    // (tiles, rows, cells), so that they are in flight together: three
+    // single-lane atomics each wait for their result in turn.  nr > 0
+    // exactly when nt > 0, and nc > 0 only then.
+    // (the three addresses in registers first: a choice between the
+    // fields' addresses would move L to memory)
+    unsigned long long *c_t = L.n_tiles, *c_r = L.n_rows, *c_c = L.n_cells;
+    asm volatile("" : "+v"(c_t), "+v"(c_r), "+v"(c_c));
+    unsigned long long* ctr_l = (lane == 0) ? c_t : (lane == 1) ? c_r : c_c;
+    const unsigned long long add = (lane == 0) ? nt : (lane == 1) ? nr : nc;
+    const bool take = nt && (lane < 2 || (lane == 2 && cells && nc));
+    unsigned long long old = 0ull;
+    if (take) old = atomicAdd(ctr_l, add);
+    if (lane == 0) s_tb = (int)old;
+    if (lane == 1) s_rb = (int)old;
+    if (lane == 2) s_cb = (int)old;
   }
   __syncthreads();
   if (rows_m) {
