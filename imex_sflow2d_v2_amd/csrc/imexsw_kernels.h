@@ -1771,30 +1771,49 @@ __device__ __forceinline__ void stencil_row(const DevParams& P, const DevPtrs& D
       for (int i = 0; i < NQ; i++) vb[i] = V[ia + 1][i][lane];
     }
     Side<NS> sd;
-    if (wv < 2) {
-      // S face of row k: bottom = row k-1 (So), top = row k (S)
+    if (wv < 4) {
+      // y faces: S face of row k (w0 bottom = row k-1, w1 top = row k) and
+      // N face (w2 bottom = row k, w3 top = row k+1).  The cases of
+      // yface_bottom / yface_top and the domain boundary only choose the
+      // input (the bottom row's value hib = va or the top row's lot = vb)
+      // or a zero side, so one make_side on the chosen values, zeroed
+      // afterwards where the case says so, gives the same side; distinct
+      // calls per case had left parts of the side in scratch memory.
+      const bool sface = wv < 2, bottom = (wv == 0 || wv == 2);
+      const int kf = sface ? k : k + 1;
+      const SrcInfo Sb = sface ? So : S, St = sface ? S : So;
+      bool zero, lot;
       if (!injx) {
-        zero_side<NS>(sd);
-      } else if (k == 0) {
-        // boundary face: both sides carry the S value of row 0
-        if (S.is_src) zero_side<NS>(sd); else make_side<NS, 2>(P, vb, g, sd);
-      } else if (wv == 0) {
-        yface_bottom<NS>(P, va, vb, So, S, k, g, sd);
+        zero = true;
+        lot = false;
+      } else if (sface ? (k == 0) : (k == P.ny - 1)) {
+        // boundary face: both sides carry row k's value
+        zero = S.is_src;
+        lot = sface;
+      } else if (bottom) {
+        zero = Sb.is_src && !(P.radial && St.src2 && St.sS && kf < P.ny - 1);
+        lot = Sb.is_src;
       } else {
-        yface_top<NS>(P, va, vb, So, S, k, g, sd);
+        zero = St.is_src && !(P.radial && Sb.src2 && !Sb.sS && Sb.sN && kf - 1 > 0);
+        lot = !St.is_src;
       }
-      side_store<NS>(SD[wv], sd, lane);
-    } else if (wv < 4) {
-      // N face of row k: bottom = row k (S), top = row k+1 (So)
-      if (!injx) {
-        zero_side<NS>(sd);
-      } else if (k == P.ny - 1) {
-        if (S.is_src) zero_side<NS>(sd); else make_side<NS, 2>(P, va, g, sd);
-      } else if (wv == 2) {
-        yface_bottom<NS>(P, va, vb, S, So, k + 1, g, sd);
-      } else {
-        yface_top<NS>(P, va, vb, S, So, k + 1, g, sd);
+      double qs[NQ];
+#pragma unroll
+      for (int i = 0; i < NQ; i++) {
+        double x = va[i], y = vb[i];
+        pin(x);
+        pin(y);
+        qs[i] = lot ? y : x;
       }
+      make_side<NS, 2>(P, qs, g, sd);
+#pragma unroll
+      for (int i = 0; i < NV; i++) {
+        sd.f[i] = zero ? 0.0 : sd.f[i];
+        sd.qc[i] = zero ? 0.0 : sd.qc[i];
+      }
+      sd.vmin = zero ? 0.0 : sd.vmin;
+      sd.vmax = zero ? 0.0 : sd.vmax;
+      sd.zv = zero ? 0.0 : sd.zv;
       side_store<NS>(SD[wv], sd, lane);
     } else {
       // x faces: W side (w4, kept), E side (w5, to LDS)
