@@ -872,32 +872,55 @@ __device__ __forceinline__ void load_row(const DevParams& P, const DevPtrs& D,
 // (VP: pb, pt are the slopes (bx, by) of rows k-1 and k; the ring overrides
 // keep them by position, which matters only with slope correction, a
 // combination the library refuses)
+// A face side from one of two value sets (b when use_b), or a zero side
+// when `zero`: one make_side on the chosen values, zeroed afterwards.  The
+// same side as a make_side call per case; calls per case had left parts of
+// the side in scratch memory.
+template <int NS, int DIR, bool VP>
+__device__ __forceinline__ void side_of(const DevParams& P, const double* a, const double* b,
+                                        bool use_b, bool zero, double g, Side<NS>& s,
+                                        double bx, double by) {
+  constexpr int NV = 4 + NS;
+  double q[NV + 2];
+#pragma unroll
+  for (int i = 0; i < NV + 2; i++) {
+    double x = a[i], y = b[i];
+    pin(x);
+    pin(y);
+    q[i] = use_b ? y : x;
+  }
+  make_side<NS, DIR, VP>(P, q, g, s, bx, by);
+#pragma unroll
+  for (int i = 0; i < NV; i++) {
+    s.f[i] = zero ? 0.0 : s.f[i];
+    s.qc[i] = zero ? 0.0 : s.qc[i];
+  }
+  s.vmin = zero ? 0.0 : s.vmin;
+  s.vmax = zero ? 0.0 : s.vmax;
+  s.zv = zero ? 0.0 : s.zv;
+}
+
+// Bottom side of the y face between rows k-1 (hib, source data Sb) and k
+// (lot, St): hib, except next to a source cell (the ring's own value, or
+// none).
 template <int NS, bool VP = false>
 __device__ __forceinline__ void yface_bottom(const DevParams& P, const double* hib,
                                              const double* lot, const SrcInfo Sb,
                                              const SrcInfo St, int k, double g, Side<NS>& Bs,
                                              const double* pb = nullptr) {
   const double bx = VP ? pb[0] : 0.0, by = VP ? pb[1] : 0.0;
-  if (Sb.is_src) {
-    if (P.radial && St.src2 && St.sS && k < P.ny - 1) make_side<NS, 2, VP>(P, lot, g, Bs, bx, by);
-    else zero_side<NS>(Bs);
-  } else {
-    make_side<NS, 2, VP>(P, hib, g, Bs, bx, by);
-  }
+  const bool zero = Sb.is_src && !(P.radial && St.src2 && St.sS && k < P.ny - 1);
+  side_of<NS, 2, VP>(P, hib, lot, Sb.is_src, zero, g, Bs, bx, by);
 }
+// Top side of the same face: lot, except next to a source cell.
 template <int NS, bool VP = false>
 __device__ __forceinline__ void yface_top(const DevParams& P, const double* hib,
                                           const double* lot, const SrcInfo Sb,
                                           const SrcInfo St, int k, double g, Side<NS>& Ts,
                                           const double* pt = nullptr) {
   const double bx = VP ? pt[0] : 0.0, by = VP ? pt[1] : 0.0;
-  if (St.is_src) {
-    if (P.radial && Sb.src2 && !Sb.sS && Sb.sN && k - 1 > 0)
-      make_side<NS, 2, VP>(P, hib, g, Ts, bx, by);
-    else zero_side<NS>(Ts);
-  } else {
-    make_side<NS, 2, VP>(P, lot, g, Ts, bx, by);
-  }
+  const bool zero = St.is_src && !(P.radial && Sb.src2 && !Sb.sS && Sb.sN && k - 1 > 0);
+  side_of<NS, 2, VP>(P, hib, lot, !St.is_src, zero, g, Ts, bx, by);
 }
 template <int NS, bool VP = false>
 __device__ __forceinline__ void yface_sides(const DevParams& P, const double* hib,
