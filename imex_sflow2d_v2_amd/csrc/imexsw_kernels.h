@@ -2496,6 +2496,8 @@ __global__ void __launch_bounds__(256, (!VP && (PH == 0 || PH == 1)) ? 2 : 1)
                              (long long)P.nx * P.ny - 1);
     c_first = D.cell_list[idx];
   }
+  // the list size, issued ahead of the step's scalars (one round of loads)
+  unsigned long long nl_raw = (PH >= 3) ? *ctr(D, C_CELLS, 0) : *ctr(D, C_ROWS, 0);
   bool live;
   double dt;
   if (ops & OP_POLICY) {
@@ -2506,9 +2508,9 @@ __global__ void __launch_bounds__(256, (!VP && (PH == 0 || PH == 1)) ? 2 : 1)
     live = (D.scal[S_DONE] == 0.0);
     dt = D.scal[S_DT];
   }
+  pin(nl_raw);
   // PH 3, 4: items of kCellsPerWave listed cells; else listed tile rows
-  const int nlist = (PH >= 3) ? (int)min(*ctr(D, C_CELLS, 0), (unsigned long long)P.nx * P.ny)
-                              : (int)*ctr(D, C_ROWS, 0);
+  const int nlist = (PH >= 3) ? (int)min(nl_raw, (unsigned long long)P.nx * P.ny) : (int)nl_raw;
   const int items = (PH >= 3) ? (nlist + kCellsPerWave - 1) / kCellsPerWave : nlist;
   for (int it = blockIdx.x * 4 + w; live && it < items; it += gridDim.x * 4) {
     if (PH >= 3) {
