@@ -72,6 +72,11 @@ def test_dambreak_1d_parity():
     dict(energy=True, n_solid=3, wet_floor=1e-3),
     dict(energy=True, limiter=3),
     dict(energy=True, limiter=2),
+    dict(energy=True, limiter=4),
+    dict(energy=True, limiter=5),
+    dict(energy=True, limiter=6),
+    dict(energy=True, limiter=7),
+    dict(energy=True, limiter=3, source=True),
     dict(energy=True, rheology=False),
 ])
 def test_small_2d_parity(kw):
